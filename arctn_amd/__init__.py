@@ -1,0 +1,11 @@
+"""arctn_amd: MI355X (gfx950) executor for ArcTN contraction plans.
+
+Public surface (mirrors the reference's arctn_py / arctn Python API):
+``CompiledContraction``, ``contract_sliced``, ``arctn_contract`` and the
+lower-level ``Plan``; ``contract`` (autograd) lives in ``arctn_amd.autograd``.
+"""
+
+from .api import CompiledContraction, Plan, arctn_contract, contract_sliced  # noqa: F401
+
+__version__ = "0.1.0"
+__all__ = ["CompiledContraction", "Plan", "arctn_contract", "contract_sliced"]

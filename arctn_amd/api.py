@@ -1,0 +1,363 @@
+"""Python surface of the HIP executor, mirroring the reference's binding.
+
+* ``CompiledContraction.compile(inputs, output, size_dict, *, ssa_path)``,
+  ``.execute(arrays)``, ``.execute_stripped(arrays)``, ``.stats()`` mirror
+  ``arctn_py.CompiledContraction`` (pybind/src/lib.rs:361-454).
+* ``contract_sliced(inputs, output, size_dict, arrays, sliced_legs, path=None,
+  ssa_path=None)`` mirrors ``arctn_py.contract_sliced`` (pybind/src/lib.rs:692-765).
+* ``arctn_contract(..., backend="hip")`` mirrors the planning + execution entry
+  point (pybind/python/arctn/__init__.py:903-1072) with the CPU planner of
+  arctn_amd.planner and the GPU executor.
+
+Arrays may be NumPy arrays (copied to the GPU; the result comes back as a
+NumPy array, like the reference) or torch tensors on the GPU (used in place;
+the result is a torch tensor on the same device). Errors are ValueError with
+the reference's message prefixes. The GPU library is mandatory: without it
+every entry point raises.
+"""
+
+from __future__ import annotations
+
+import math
+import operator
+import time
+
+import numpy as np
+
+from . import _lib
+from . import planner as _planner
+
+DTYPE_CODE = {np.dtype(np.float32): 0, np.dtype(np.float64): 1, np.dtype(np.complex64): 2,
+              np.dtype(np.complex128): 3}
+DTYPE_NAME = {0: "float32", 1: "float64", 2: "complex64", 3: "complex128"}
+
+
+def _torch():
+    import torch
+
+    return torch
+
+
+def _torch_dtype(code):
+    torch = _torch()
+    return {0: torch.float32, 1: torch.float64, 2: torch.complex64, 3: torch.complex128}[code]
+
+
+def _code_of(arr):
+    torch = _torch() if not isinstance(arr, np.ndarray) else None
+    if torch is not None and isinstance(arr, torch.Tensor):
+        m = {torch.float32: 0, torch.float64: 1, torch.complex64: 2, torch.complex128: 3}
+        if arr.dtype not in m:
+            return None
+        return m[arr.dtype]
+    return DTYPE_CODE.get(np.asarray(arr).dtype)
+
+
+def _build_net(inputs, output, size_dict):
+    """pybind/src/lib.rs:609-637: legs are integers; unused size_dict entries rejected."""
+    inputs = [[int(l) for l in t] for t in inputs]
+    output = [int(l) for l in output]
+    size_dict = {int(k): int(v) for k, v in dict(size_dict).items()}
+    if not inputs:
+        raise ValueError("inputs 为空 (inputs is empty)")
+    used = {l for t in inputs for l in t} | set(output)
+    extra = [k for k in size_dict if k not in used]
+    if extra:
+        raise ValueError(f"size_dict 含未在 inputs/output 使用的腿 {extra[0]} (unused size_dict leg)")
+    return inputs, output, size_dict
+
+
+def _device_inputs(net_inputs, size_dict, arrays, context):
+    """Full dtype/shape preflight before any copy (pybind/src/lib.rs:241-330).
+
+    Returns (device tensors, dtype code, return_numpy)."""
+    torch = _torch()
+    arrays = list(arrays)
+    if len(arrays) != len(net_inputs):
+        raise ValueError(f"{context}: arrays 个数 {len(arrays)} 与 inputs 个数 {len(net_inputs)} 不符")
+    code = _code_of(arrays[0]) if arrays else None
+    if code is None:
+        raise ValueError(f"{context}: arrays 的 dtype 不支持；支持 float32 / float64 / complex64 / complex128")
+    numpy_in = False
+    for i, (a, legs) in enumerate(zip(arrays, net_inputs)):
+        c = _code_of(a)
+        if c != code:
+            raise ValueError(f"{context}: arrays[{i}] 的 dtype 与首张量不一致（预期 {DTYPE_NAME[code]}）")
+        shape = tuple(a.shape)
+        want = tuple(size_dict[l] for l in legs)
+        if shape != want:
+            raise ValueError(f"{context}: arrays[{i}] shape {list(shape)} 与网络预期 {list(want)} 不一致（输入数组布局非法）")
+        numpy_in = numpy_in or not isinstance(a, torch.Tensor)
+    if not torch.cuda.is_available():
+        raise _lib.HipError("no GPU device is available for the HIP executor")
+    dev = None
+    for a in arrays:
+        if isinstance(a, torch.Tensor) and a.is_cuda:
+            dev = a.device
+            break
+    dev = dev or torch.device("cuda", torch.cuda.current_device())
+    out = []
+    for a in arrays:
+        if isinstance(a, torch.Tensor):
+            t = a.to(dev)
+        else:
+            a = np.asarray(a)
+            # np.ascontiguousarray would turn a rank-0 array into shape (1,)
+            a = np.array(a, order="C", copy=True) if a.ndim == 0 else np.ascontiguousarray(a)
+            t = torch.from_numpy(a).to(dev)
+        out.append(t.contiguous())
+    return out, code, numpy_in
+
+
+class _Executor:
+    def __init__(self, plan_handle, opts):
+        self.h = _lib.C.c_void_p()
+        _lib.check(_lib.lib().atn_executor_create(plan_handle, _lib.C.byref(opts), _lib.C.byref(self.h)))
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib._lib is not None:
+            _lib.lib().atn_executor_destroy(self.h)
+            self.h = None
+
+
+class Plan:
+    """A validated plan (network + SSA path + sliced legs) and its executors."""
+
+    def __init__(self, inputs, output, size_dict, ssa_path, sliced_legs=()):
+        self.inputs, self.output, self.size_dict = _build_net(inputs, output, size_dict)
+        self.ssa_path = [tuple(int(x) for x in p) for p in ssa_path]
+        self.sliced_legs = [int(l) for l in sliced_legs]
+        self._h = _lib.plan_create(self.inputs, self.output, self.size_dict, self.ssa_path, self.sliced_legs)
+        self._executors = {}
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _lib._lib is not None:
+            self._executors.clear()
+            _lib.lib().atn_plan_destroy(self._h)
+            self._h = None
+
+    @property
+    def handle(self):
+        return self._h
+
+    def info(self):
+        i = _lib.PlanInfo()
+        _lib.check(_lib.lib().atn_plan_stats(self._h, _lib.C.byref(i)))
+        return {k: getattr(i, k) for k, _ in i._fields_}
+
+    def describe(self):
+        import json
+
+        p = _lib.C.c_void_p()
+        _lib.check(_lib.lib().atn_plan_describe(self._h, _lib.C.byref(p)))
+        try:
+            return json.loads(_lib.C.string_at(p.value).decode())
+        finally:
+            _lib.lib().atn_free(p)
+
+    def executor(self, dtype_code, precision="native", batched=False, use_graph=False, autotune=False, strip=False,
+                 max_batch=0, workspace_limit=0):
+        if precision not in _lib.PREC:
+            raise ValueError(f"unknown precision {precision!r}; use one of {sorted(_lib.PREC)}")
+        key = (dtype_code, precision, bool(batched), bool(use_graph), bool(autotune), bool(strip), max_batch,
+               workspace_limit)
+        ex = self._executors.get(key)
+        if ex is None:
+            o = _lib.ExecOptions(dtype_code, _lib.PREC[precision], int(batched), int(use_graph), int(autotune),
+                                 int(strip), max_batch, workspace_limit)
+            ex = _Executor(self._h, o)
+            self._executors[key] = ex
+        return ex
+
+    def run(self, dev_inputs, dtype_code, out=None, report=False, stream=None, **opts):
+        """Execute on device tensors; returns (output tensor, report dict or None)."""
+        torch = _torch()
+        ex = self.executor(dtype_code, **opts)
+        if out is None:
+            shape = [self.size_dict[l] for l in self.output]
+            out = torch.empty(shape, dtype=_torch_dtype(dtype_code), device=dev_inputs[0].device)
+        ptrs = _lib.pointer_array([t.data_ptr() for t in dev_inputs])
+        s = stream if stream is not None else torch.cuda.current_stream(dev_inputs[0].device)
+        rep = _lib.Report()
+        _lib.check(_lib.lib().atn_executor_run(ex.h, ptrs, out.data_ptr(), s.cuda_stream,
+                                               _lib.C.byref(rep) if report else None))
+        return out, (rep.as_dict() if report else None)
+
+
+def _finish(out, numpy_in):
+    if numpy_in:
+        return out.cpu().numpy()
+    return out
+
+
+class CompiledContraction:
+    """A reusable exact contraction compiled from an SSA path (pybind/src/lib.rs:355-454).
+
+    ``execute`` runs the plain per-step path on the GPU; keyword options select
+    the precision form (``precision=``) and the batched/graph forms.
+    """
+
+    def __init__(self, plan):
+        self._plan = plan
+
+    @staticmethod
+    def compile(inputs, output, size_dict, *, ssa_path):
+        try:
+            return CompiledContraction(Plan(inputs, output, size_dict, ssa_path))
+        except ValueError as e:
+            raise ValueError(f"预编译路径非法: {e}") from None
+
+    def _prepare(self, arrays):
+        p = self._plan
+        arrays = list(arrays)
+        if len(arrays) != len(p.inputs):
+            raise ValueError(f"arrays 个数 {len(arrays)} 与编译计划输入数 {len(p.inputs)} 不符")
+        return _device_inputs(p.inputs, p.size_dict, arrays, "预编译收缩失败")
+
+    def execute(self, arrays, **opts):
+        dev, code, numpy_in = self._prepare(arrays)
+        try:
+            out, _ = self._plan.run(dev, code, **opts)
+        except ValueError as e:
+            raise ValueError(f"预编译收缩失败: {e}") from None
+        return _finish(out, numpy_in)
+
+    def execute_stripped(self, arrays, **opts):
+        """(mantissa, log10_exponent): result = mantissa * 10**log10_exponent.
+
+        Every step result is renormalised by a power of two (exact), unlike
+        the reference's division by the exact max magnitude (compiled.rs:412-430);
+        mantissa * 10**exponent is the same value.
+        """
+        dev, code, numpy_in = self._prepare(arrays)
+        try:
+            out, rep = self._plan.run(dev, code, report=True, strip=True, **opts)
+        except ValueError as e:
+            raise ValueError(f"预编译收缩失败: {e}") from None
+        return _finish(out, numpy_in), rep["log10_exponent"]
+
+    def stats(self):
+        i = self._plan.info()
+        return {"path_format": "ssa-v1", "n_inputs": i["n_inputs"], "n_steps": i["n_steps"],
+                "log10_flops": i["log10_flops"], "log2_peak_size": i["log2_peak_size"]}
+
+    def describe(self):
+        return self._plan.describe()
+
+    @property
+    def plan(self):
+        return self._plan
+
+    def __repr__(self):
+        i = self._plan.info()
+        return f"CompiledContraction(path_format='ssa-v1', n_inputs={i['n_inputs']}, n_steps={i['n_steps']})"
+
+
+def contract_sliced(inputs, output, size_dict, arrays, sliced_legs, path=None, ssa_path=None, **opts):
+    """Execute a supplied path over the requested slices (pybind/src/lib.rs:692-765)."""
+    n = len(inputs)
+    arrays = list(arrays)
+    if len(arrays) != n:
+        raise ValueError(f"arrays 个数 {len(arrays)} 与 inputs 个数 {n} 不符")
+    if path is not None and ssa_path is not None:
+        raise ValueError("path 与 ssa_path 互斥，只能给一个（path=linear 口径，ssa_path=SSA 口径）")
+    inputs_i, output_i, sd = _build_net(inputs, output, size_dict)
+    for leg in sliced_legs:
+        if int(leg) not in sd:
+            raise ValueError(f"切片腿 {leg} 没有出现在网络 size_dict 中")
+    dev, code, numpy_in = _device_inputs(inputs_i, sd, arrays, "切片收缩失败")
+    if ssa_path is not None:
+        ssa = [tuple(p) for p in ssa_path]
+    elif path is not None:
+        ssa = _planner.linear_to_ssa([tuple(p) for p in path], n)
+    else:
+        raise ValueError("必须给 path=（linear 口径）或 ssa_path=（SSA 口径）之一。")
+    try:
+        plan = Plan(inputs_i, output_i, sd, ssa, sliced_legs)
+        out, _ = plan.run(dev, code, **opts)
+    except ValueError as e:
+        raise ValueError(f"切片收缩失败: {e}") from None
+    return _finish(out, numpy_in)
+
+
+def _to_int_labels(inputs, output, size_dict):
+    """pybind/python/arctn/__init__.py:186-235: hashable labels -> dense u32 ids."""
+    label_of = {}
+
+    def idx(lab):
+        i = label_of.get(lab)
+        if i is None:
+            i = len(label_of)
+            label_of[lab] = i
+        return i
+
+    inputs_i = [[idx(l) for l in term] for term in inputs]
+    output_i = [idx(l) for l in output]
+    missing = [lab for lab in label_of if lab not in size_dict]
+    if missing:
+        raise ValueError(f"size_dict 缺少已使用腿的维度: {missing!r}")
+    unused = [lab for lab in size_dict if lab not in label_of]
+    if unused:
+        raise ValueError(f"size_dict 含未在 inputs/output 使用的腿: {unused!r}")
+    size_i = {}
+    for lab, i in label_of.items():
+        raw = size_dict[lab]
+        if isinstance(raw, bool):
+            raise ValueError(f"腿 {lab!r} 的维度必须是正整数，不能是 bool")
+        dim = operator.index(raw)
+        if dim <= 0:
+            raise ValueError(f"腿 {lab!r} 的维度必须为正整数，得到 {dim}")
+        size_i[i] = dim
+    return inputs_i, output_i, size_i, label_of
+
+
+def arctn_contract(inputs, output, size_dict, arrays, *, preset="greedy", seed=0, target_size=None,
+                   slicing_mode="fixed", max_time=None, backend="hip", return_info=False,
+                   allow_output_slicing=False, ntrials=1, **opts):
+    """Plan on the CPU, optionally slice to ``target_size``, execute on the GPU.
+
+    Mirrors pybind/python/arctn/__init__.py:903-1072 with ``backend="hip"``.
+    Planning uses the reference's open greedy (``preset="greedy"``) or its
+    random-greedy trials (``preset="rgreedy"``, ``ntrials``); the proprietary
+    Light/Heavy engine is not available. Only ``slicing_mode="fixed"`` exists.
+    """
+    if backend != "hip":
+        raise ValueError(f"backend 必须是 'hip'（this package is the HIP backend），收到 {backend!r}")
+    if slicing_mode != "fixed":
+        raise ValueError("only slicing_mode='fixed' is available")
+    if target_size is not None:
+        if isinstance(target_size, bool) or operator.index(target_size) <= 0:
+            raise ValueError("target_size 必须大于 0")
+    inputs_i, output_i, size_i, _ = _to_int_labels(inputs, output, size_dict)
+    t0 = time.perf_counter()
+    dev, code, numpy_in = _device_inputs(inputs_i, size_i, list(arrays), "切片收缩失败")
+    prepare_s = time.perf_counter() - t0
+    net = {"inputs": inputs_i, "output": output_i, "size_dict": size_i}
+    t0 = time.perf_counter()
+    if preset == "greedy":
+        path = _planner.greedy(net)
+        sliced = []
+        if target_size is not None:
+            sliced = _planner.find_slices_to_size(net, path, target_size, allow_output_slicing)
+    elif preset == "rgreedy":
+        best = _planner.random_greedy_sliced(net, ntrials, seed, target_size or (1 << 62), allow_output_slicing)
+        path, sliced = best[0], best[1]
+    else:
+        raise ValueError(f"preset 只支持 'greedy' 或 'rgreedy'，收到 {preset!r}")
+    if target_size is not None and sliced is None:
+        raise ValueError(f"target_size 上限（log2={math.log2(target_size):.6g}）不可达")
+    plan_s = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    plan = Plan(inputs_i, output_i, size_i, path, sliced or [])
+    out, rep = plan.run(dev, code, report=True, **opts)
+    exec_s = time.perf_counter() - t0
+    result = _finish(out, numpy_in)
+    if not return_info:
+        return result
+    info = {"path": [list(p) for p in path], "sliced": bool(sliced), "sliced_legs": list(sliced or []),
+            "target_size": target_size, "allow_output_slicing": allow_output_slicing,
+            "execution_backend": backend, "execution_implementation": "arctn_amd.hip_executor",
+            "execution_input_prepare_wall_s": prepare_s, "planning_wall_s": plan_s,
+            "execution_dispatch_wall_s": exec_s, "execution_device_ms": rep["elapsed_ms"],
+            "execution_timing_scope": "hip_events", "report": rep}
+    return result, info

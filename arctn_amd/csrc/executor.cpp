@@ -1,0 +1,817 @@
+// Executors of a contraction plan on one GPU.
+//
+// Plain path (the drop-in for src/slice.rs:960-1065 + src/compiled.rs:383-441):
+// one slice at a time, one step at a time: gather the slice's inputs, then per
+// step the planned sums, the two permutes and one GEMM; the per-slice results
+// are summed in T in the reference's chunk order (src/slice.rs:1018-1064) and
+// output blocks are scattered into the full output (src/slice.rs:1139-1171).
+//
+// Batched path: the per-slice plan is identical for every slice, so
+//   * steps whose operands do not depend on the slice index run once (hoisted);
+//   * every other step runs for B slices at once: permutes see the slice index
+//     as an extra outer axis, GEMMs are strided-batched with stride 0 for the
+//     shared operand (or folded into m when only A varies);
+//   * slice results are summed into an FP64 accumulator in slice order;
+//   * the whole sequence can be captured into one hipGraph.
+#include "executor.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <map>
+#include <set>
+
+namespace atn {
+
+static size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
+
+// ------------------------------------------------------------ Executor build --
+int Executor::newbuf(size_t bytes) {
+  if (replay_) {
+    if (replay_pos_ >= body_ids_.size()) throw Error("internal: tail replay out of buffers", -3);
+    int id = body_ids_[replay_pos_++];
+    if (bufs_[id].bytes < bytes) throw Error("internal: tail buffer larger than body buffer", -3);
+    return id;
+  }
+  bufs_.push_back(Buf{std::max<size_t>(bytes, 4), t_, t_});
+  int id = (int)bufs_.size() - 1;
+  if (record_) body_ids_.push_back(id);
+  return id;
+}
+
+void Executor::use(const Ref& r) {
+  if (replay_ || r.kind != Ref::WS) return;
+  bufs_[r.id].last = std::max(bufs_[r.id].last, t_);
+}
+
+static GatherArgs make_gather(const Plan& P, size_t i) {
+  GatherArgs g{};
+  const auto& legs = P.net.inputs[i];
+  auto shape = P.net.shape_of(legs);
+  std::vector<size_t> strides(shape.size(), 1);
+  for (size_t a = shape.size(); a-- > 1;) strides[a - 1] = strides[a] * shape[a];
+  std::set<size_t> sel_axes;
+  for (auto& s : P.sel[i]) sel_axes.insert(s.axis);
+  // kept axes, merging neighbours that are adjacent in the full tensor
+  std::vector<std::pair<size_t, size_t>> ax;  // (dim, stride)
+  for (size_t a = 0; a < shape.size(); ++a) {
+    if (sel_axes.count(a) || shape[a] == 1) continue;
+    if (!ax.empty() && ax.back().second == strides[a] * shape[a]) {
+      ax.back().first *= shape[a];
+      ax.back().second = strides[a];
+    } else {
+      ax.push_back({shape[a], strides[a]});
+    }
+  }
+  if (ax.size() > (size_t)MAXR || P.sel[i].size() > (size_t)MAXR) throw Error("gather rank exceeds kernel limit");
+  g.n_ax = (int)ax.size();
+  g.inner = 1;
+  for (size_t q = 0; q < ax.size(); ++q) {
+    g.dims[q] = (uint32_t)ax[q].first;
+    g.str[q] = (int64_t)ax[q].second;
+    g.inner *= ax[q].first;
+  }
+  g.n_sel = (int)P.sel[i].size();
+  for (size_t q = 0; q < P.sel[i].size(); ++q) {
+    g.sel_div[q] = P.sel[i][q].div;
+    g.sel_mod[q] = P.sel[i][q].mod;
+    g.sel_str[q] = (int64_t)P.sel[i][q].stride;
+  }
+  return g;
+}
+
+static ScatterArgs make_scatter(const Plan& P) {
+  ScatterArgs a{};
+  if (P.block_div.size() > (size_t)MAXR || P.kept_dims.size() > (size_t)MAXR) throw Error("output rank exceeds kernel limit");
+  a.n_blk = (int)P.block_div.size();
+  for (size_t q = 0; q < P.block_div.size(); ++q) {
+    a.blk_div[q] = P.block_div[q];
+    a.blk_mod[q] = P.block_mod[q];
+    a.blk_str[q] = (int64_t)P.block_stride[q];
+  }
+  a.n_kept = (int)P.kept_dims.size();
+  for (size_t q = 0; q < P.kept_dims.size(); ++q) {
+    a.kept_dims[q] = (uint32_t)P.kept_dims[q];
+    a.kept_str[q] = (int64_t)P.kept_strides[q];
+  }
+  return a;
+}
+
+void Executor::gen_inputs(bool body, size_t nb, std::vector<Ref>& slots, std::vector<std::vector<size_t>>& shapes) {
+  const Plan& P = *P_;
+  const int dt = o_.dtype, es = es_;
+  for (size_t i = 0; i < P.cp.n_inputs; ++i) {
+    bool varying = !P.sel[i].empty();
+    if (o_.batched && varying != body) continue;
+    size_t nbI = (o_.batched && varying) ? nb : 1;
+    Ref ref;
+    auto shape = P.cp.preps[i].input_shape;
+    if (varying) {
+      GatherArgs ga = make_gather(P, i);
+      ref = Ref{Ref::WS, newbuf(nbI * ga.inner * es)};
+      Ref src{Ref::INPUT, (int)i};
+      use(ref);
+      emit([=](const Ctx& c) { launch_gather(es, ga, c.g0, nbI, c.ptr(src), c.ptr(ref), c.s); });
+    } else {
+      ref = Ref{Ref::INPUT, (int)i};
+    }
+    for (auto& op : P.cp.preps[i].traces) {
+      size_t ai = std::get<0>(op), aj = std::get<1>(op);
+      bool diag = std::get<2>(op);
+      size_t A = nbI, d = shape[ai], Bm = 1, C = 1;
+      for (size_t q = 0; q < ai; ++q) A *= shape[q];
+      for (size_t q = ai + 1; q < aj; ++q) Bm *= shape[q];
+      for (size_t q = aj + 1; q < shape.size(); ++q) C *= shape[q];
+      size_t out_numel = diag ? A * d * Bm * C : A * Bm * C;
+      Ref dst{Ref::WS, newbuf(out_numel * es)};
+      Ref src = ref;
+      use(src);
+      use(dst);
+      if (diag)
+        emit([=](const Ctx& c) { launch_diag(es, A, d, Bm, C, c.ptr(src), c.ptr(dst), c.s); });
+      else
+        emit([=](const Ctx& c) { launch_trace(dt, A, d, Bm, C, c.ptr(src), c.ptr(dst), c.s); });
+      shape.erase(shape.begin() + aj);
+      if (!diag) shape.erase(shape.begin() + ai);
+      ref = dst;
+    }
+    slots[i] = ref;
+    shapes[i] = shape;
+  }
+}
+
+void Executor::gen_gemm(const PairStep& st, size_t nbo, bool va, bool vb, const Ref& a, const Ref& b, const Ref& c) {
+  GemmArgs g{};
+  g.m = (int64_t)st.m, g.k = (int64_t)st.k, g.n = (int64_t)st.n;
+  g.lda = g.k, g.ldb = g.n, g.ldc = g.n;
+  g.nbi = (int64_t)st.bt;
+  g.sai = g.m * g.k, g.sbi = g.k * g.n, g.sci = g.m * g.n;
+  g.nbo = (int64_t)nbo;
+  g.sao = va ? (int64_t)st.bt * g.m * g.k : 0;
+  g.sbo = vb ? (int64_t)st.bt * g.k * g.n : 0;
+  g.sco = (int64_t)st.bt * g.m * g.n;
+  if (st.bt == 1 && va && !vb && nbo > 1) {  // A rows of consecutive slices are contiguous, as are C rows
+    g.m *= (int64_t)nbo;
+    g.nbo = 1;
+    g.sao = g.sbo = g.sco = 0;
+    g.sai = g.m * g.k, g.sci = g.m * g.n;
+  }
+  int idx = (int)gemm_choice_.size();
+  gemm_choice_.push_back(GEMM_AUTO);
+  gemm_sites_.push_back(GemmSite{g, a, b, c});
+  int dt = o_.dtype, prec = o_.precision;
+  use(a);
+  use(b);
+  use(c);
+  emit([=](const Ctx& cx) {
+    GemmArgs q = g;
+    q.a = cx.ptr(a), q.b = cx.ptr(b), q.c = cx.ptr(c);
+    launch_gemm(dt, prec, gemm_choice_[idx], q, cx.s);
+  });
+}
+
+void Executor::gen_steps(bool body, size_t nb, std::vector<Ref>& slots, std::vector<std::vector<size_t>>& shapes) {
+  const Plan& P = *P_;
+  const int dt = o_.dtype, es = es_;
+  for (size_t si = 0; si < P.cp.steps.size(); ++si) {
+    const PairStep& st = P.cp.steps[si];
+    bool v = P.varying[st.out];
+    if (o_.batched && v != body) continue;
+    bool va = o_.batched && P.varying[st.li], vb = o_.batched && P.varying[st.ri];
+    size_t nbA = va ? nb : 1, nbB = vb ? nb : 1, nbC = (va || vb) ? nb : 1;
+    std::vector<Ref> temps;
+    auto chain = [&](Ref cur, std::vector<size_t> shape, const std::vector<size_t>& sums, const std::vector<size_t>& perm,
+                     size_t nbX) -> Ref {
+      for (size_t ax : sums) {
+        size_t outer = nbX, inner = 1, d = shape[ax];
+        for (size_t q = 0; q < ax; ++q) outer *= shape[q];
+        for (size_t q = ax + 1; q < shape.size(); ++q) inner *= shape[q];
+        Ref dst{Ref::WS, newbuf(outer * inner * es)};
+        Ref src = cur;
+        use(src);
+        use(dst);
+        emit([=](const Ctx& c) { launch_sum_axis(dt, outer, d, inner, c.ptr(src), c.ptr(dst), c.s); });
+        temps.push_back(dst);
+        shape.erase(shape.begin() + ax);
+        cur = dst;
+      }
+      PermPlan pp = plan_permute(shape, perm, nbX, es);
+      if (pp.kind != 0) {
+        Ref dst{Ref::WS, newbuf(pp.numel * es)};
+        Ref src = cur;
+        use(src);
+        use(dst);
+        emit([=](const Ctx& c) { launch_permute(pp, c.ptr(src), c.ptr(dst), c.s); });
+        temps.push_back(dst);
+        cur = dst;
+      }
+      return cur;
+    };
+    Ref a = chain(slots[st.li], shapes[st.li], st.sum_a, st.a_perm, nbA);
+    Ref b = chain(slots[st.ri], shapes[st.ri], st.sum_b, st.b_perm, nbB);
+    Ref c{Ref::WS, newbuf(nbC * st.bt * st.m * st.n * es)};
+    gen_gemm(st, nbC, va, vb, a, b, c);
+    if (o_.strip) {
+      size_t numel = nbC * st.bt * st.m * st.n;
+      Ref mb{Ref::WS, maxbits_buf_}, ex{Ref::WS, expo_buf_};
+      use(c);
+      emit([=](const Ctx& cx) {
+        launch_fill_zero(cx.ptr(mb), 8, cx.s);
+        launch_maxabs(dt, cx.ptr(c), numel, (uint64_t*)cx.ptr(mb), cx.s);
+        launch_scale_pow2(dt, cx.ptr(c), numel, (const uint64_t*)cx.ptr(mb), (int64_t*)cx.ptr(ex), cx.s);
+      });
+    }
+    slots[st.out] = c;
+    shapes[st.out] = st.result_shape;
+  }
+}
+
+Ref Executor::gen_finalize(size_t nb, const Ref& src0, bool to_output) {
+  const Plan& P = *P_;
+  const int dt = o_.dtype, es = es_;
+  const Finalize& f = P.cp.fin;
+  auto shape = f.input_shape;
+  Ref cur = src0;
+  auto sh = shape;
+  for (size_t ax : f.sum) sh.erase(sh.begin() + ax);
+  PermPlan pp = plan_permute(sh, f.perm, nb, es);
+  for (size_t qi = 0; qi < f.sum.size(); ++qi) {
+    size_t ax = f.sum[qi];
+    size_t outer = nb, inner = 1, d = shape[ax];
+    for (size_t q = 0; q < ax; ++q) outer *= shape[q];
+    for (size_t q = ax + 1; q < shape.size(); ++q) inner *= shape[q];
+    bool last = qi + 1 == f.sum.size() && pp.kind == 0;
+    Ref dst = (last && to_output) ? Ref{Ref::OUTPUT, 0} : Ref{Ref::WS, newbuf(outer * inner * es)};
+    Ref src = cur;
+    use(src);
+    use(dst);
+    emit([=](const Ctx& c) { launch_sum_axis(dt, outer, d, inner, c.ptr(src), c.ptr(dst), c.s); });
+    shape.erase(shape.begin() + ax);
+    cur = dst;
+  }
+  size_t numel = nb * prod(shape);
+  if (pp.kind != 0) {
+    Ref dst = to_output ? Ref{Ref::OUTPUT, 0} : Ref{Ref::WS, newbuf(numel * es)};
+    Ref src = cur;
+    use(src);
+    use(dst);
+    emit([=](const Ctx& c) { launch_permute(pp, c.ptr(src), c.ptr(dst), c.s); });
+    cur = dst;
+  } else if (to_output && cur.kind != Ref::OUTPUT) {
+    Ref src = cur;
+    use(src);
+    emit([=](const Ctx& c) { launch_copy(numel * es, c.ptr(src), c.out, c.s); });
+    cur = Ref{Ref::OUTPUT, 0};
+  }
+  return cur;
+}
+
+void Executor::build(size_t B) {
+  const Plan& P = *P_;
+  const int dt = o_.dtype, es = es_;
+  B_ = B;
+  bufs_.clear();
+  pre_.clear(), body_.clear(), tail_.clear(), post_.clear();
+  gemm_choice_.clear();
+  gemm_sites_.clear();
+  body_ids_.clear();
+  replay_ = record_ = false;
+  t_ = 0;
+  const size_t nslot = P.cp.n_inputs + P.cp.steps.size();
+  const bool sliced = P.n_slices > 1;
+  const size_t out_numel = P.sub_out_numel;
+  cur_ = &pre_;
+  if (o_.strip) {
+    maxbits_buf_ = newbuf(8);
+    expo_buf_ = newbuf(8);
+    Ref ex{Ref::WS, expo_buf_};
+    use(ex);
+    emit([=](const Ctx& c) { launch_fill_zero(c.ptr(ex), 8, c.s); });
+  }
+  std::vector<Ref> slots(nslot);
+  std::vector<std::vector<size_t>> shapes(nslot);
+  ScatterArgs sc = make_scatter(P);
+  if (o_.batched) {
+    gen_inputs(false, 1, slots, shapes);
+    gen_steps(false, 1, slots, shapes);
+    if (sliced) {
+      size_t wbytes = dtype_complex(dt) ? 16 : 8;
+      acc_buf_ = newbuf(P.n_blocks * out_numel * wbytes);
+      Ref acc{Ref::WS, acc_buf_};
+      size_t nbytes = P.n_blocks * out_numel * wbytes;
+      use(acc);
+      emit([=](const Ctx& c) { launch_fill_zero(c.ptr(acc), nbytes, c.s); });
+    }
+    int pre_end = t_;
+    body_begin_ = t_;
+    auto gen_body = [&](size_t nb) {
+      std::vector<Ref> sl = slots;
+      auto sh = shapes;
+      gen_inputs(true, nb, sl, sh);
+      gen_steps(true, nb, sl, sh);
+      Ref res = gen_finalize(nb, sl[P.cp.fin.slot], !sliced);
+      if (sliced) {
+        Ref acc{Ref::WS, acc_buf_};
+        size_t n_int = P.n_internal;
+        use(res);
+        use(acc);
+        emit([=](const Ctx& c) { launch_reduce_f64(dt, c.ptr(res), c.ptr(acc), c.g0, nb, n_int, out_numel, c.s); });
+      }
+    };
+    cur_ = &body_;
+    record_ = true;
+    gen_body(B);
+    record_ = false;
+    body_end_ = t_;
+    tail_nb_ = P.n_slices % B;
+    if (tail_nb_) {
+      cur_ = &tail_;
+      replay_ = true;
+      replay_pos_ = 0;
+      int saved = t_;
+      gen_body(tail_nb_);
+      t_ = saved;
+      replay_ = false;
+    }
+    // pre buffers read by the body stay alive across all batches
+    for (auto& b : bufs_)
+      if (b.first < pre_end && b.last >= body_begin_) b.last = std::max(b.last, body_end_);
+    cur_ = &post_;
+    if (sliced) {
+      Ref acc{Ref::WS, acc_buf_};
+      size_t nblk = P.n_blocks;
+      use(acc);
+      emit([=](const Ctx& c) { launch_scatter(dt, true, sc, 0, nblk, out_numel, c.ptr(acc), c.out, c.s); });
+    }
+  } else {
+    body_begin_ = t_;
+    cur_ = &body_;
+    gen_inputs(true, 1, slots, shapes);
+    gen_steps(true, 1, slots, shapes);
+    Ref res = gen_finalize(1, slots[P.cp.fin.slot], !sliced);
+    if (sliced) {
+      size_t nch = P.n_chunks, n_int = P.n_internal;
+      size_t chunk_len = (n_int + nch - 1) / nch;
+      std::vector<Ref> parts;
+      for (size_t c = 0; c < nch; ++c) parts.push_back(Ref{Ref::WS, newbuf(out_numel * es)});
+      Ref blk{Ref::WS, newbuf(out_numel * es)};
+      use(res);
+      for (auto& r : parts) use(r);
+      use(blk);
+      emit([=](const Ctx& c) {
+        uint64_t g = c.g0, b = g / n_int, s = g % n_int, ch = s / chunk_len;
+        launch_accum(dt, s % chunk_len == 0, c.ptr(res), c.ptr(parts[ch]), out_numel, c.s);
+        if (s == n_int - 1) {
+          for (size_t q = 0; q < nch; ++q) launch_accum(dt, q == 0, c.ptr(parts[q]), c.ptr(blk), out_numel, c.s);
+          launch_scatter(dt, false, sc, b, 1, out_numel, c.ptr(blk), c.out, c.s);
+        }
+      });
+      // chunk partials and the block accumulator persist across slices
+      for (auto& r : parts) bufs_[r.id].first = body_begin_;
+      bufs_[blk.id].first = body_begin_;
+    }
+    body_end_ = t_;
+  }
+  // strip bookkeeping lives for the whole run
+  for (int id : {maxbits_buf_, expo_buf_})
+    if (o_.strip && id >= 0) bufs_[id].first = 0, bufs_[id].last = t_ + 1;
+  if (acc_buf_ >= 0 && o_.batched && sliced) bufs_[acc_buf_].last = t_ + 1;
+  // Offline first-fit placement of buffers with overlapping lifetimes.
+  std::vector<int> order(bufs_.size());
+  for (size_t i = 0; i < order.size(); ++i) order[i] = (int)i;
+  std::sort(order.begin(), order.end(), [&](int x, int y) {
+    if (bufs_[x].first != bufs_[y].first) return bufs_[x].first < bufs_[y].first;
+    return bufs_[x].bytes > bufs_[y].bytes;
+  });
+  offs_.assign(bufs_.size(), 0);
+  struct Live {
+    size_t off, end;
+    int last;
+  };
+  std::vector<Live> live;
+  size_t total = 0;
+  for (int id : order) {
+    const Buf& b = bufs_[id];
+    live.erase(std::remove_if(live.begin(), live.end(), [&](const Live& l) { return l.last < b.first; }), live.end());
+    std::sort(live.begin(), live.end(), [](const Live& x, const Live& y) { return x.off < y.off; });
+    size_t need = align_up(b.bytes, 256), pos = 0;
+    for (auto& l : live) {
+      if (pos + need <= l.off) break;
+      pos = std::max(pos, l.end);
+    }
+    offs_[id] = pos;
+    live.push_back(Live{pos, pos + need, b.last});
+    total = std::max(total, pos + need);
+  }
+  ws_bytes_ = total;
+}
+
+Executor::Executor(std::shared_ptr<const Plan> plan, const ExecOptions& o) : P_(std::move(plan)), o_(o) {
+  const Plan& P = *P_;
+  es_ = (int)dtype_size(o_.dtype);
+  if (o_.precision != NATIVE) {
+    if (o_.precision == C3M && o_.dtype != C64) throw Error("precision c3m applies to complex64 only");
+    if ((o_.precision == BF16X3 || o_.precision == FP16X2) && !(o_.dtype == C64 || o_.dtype == F32))
+      throw Error("split precision applies to float32 / complex64 only");
+  }
+  if (o_.strip && P.n_slices > 1) throw Error("strip is available for unsliced plans (execute_stripped)");
+  if (!o_.batched) o_.use_graph = false;
+  size_t limit = o_.workspace_limit;
+  if (!limit) {
+    size_t fr = 0, tot = 0;
+    ATN_HIP(hipMemGetInfo(&fr, &tot));
+    limit = (size_t)(0.6 * (double)fr);
+  }
+  size_t B = 1;
+  if (o_.batched) {
+    B = P.n_slices;
+    if (o_.max_batch) B = std::min(B, o_.max_batch);
+  }
+  for (;;) {
+    build(B);
+    if (ws_bytes_ <= limit || B == 1) break;
+    B = (B + 1) / 2;
+  }
+  if (ws_bytes_ > limit) throw Error(cat("workspace of ", ws_bytes_, " bytes exceeds the limit of ", limit, " bytes"));
+  ATN_HIP(hipMalloc(&ws_, std::max<size_t>(ws_bytes_, 256)));
+  if (o_.use_graph) ATN_HIP(hipStreamCreateWithFlags(&cap_stream_, hipStreamNonBlocking));
+}
+
+Executor::~Executor() {
+  if (gexec_) (void)hipGraphExecDestroy(gexec_);
+  if (graph_) (void)hipGraphDestroy(graph_);
+  if (cap_stream_) (void)hipStreamDestroy(cap_stream_);
+  if (ws_) (void)hipFree(ws_);
+}
+
+void Executor::run_ops(const std::vector<Op>& ops, const Ctx& c) {
+  for (auto& op : ops) op(c);
+}
+
+void Executor::enqueue(const Ctx& c0) {
+  const Plan& P = *P_;
+  Ctx c = c0;
+  run_ops(pre_, c);
+  if (o_.batched) {
+    size_t nfull = P.n_slices / B_;
+    for (size_t q = 0; q < nfull; ++q) {
+      c.g0 = q * B_;
+      run_ops(body_, c);
+    }
+    if (tail_nb_) {
+      c.g0 = nfull * B_;
+      run_ops(tail_, c);
+    }
+  } else {
+    for (size_t g = 0; g < P.n_slices; ++g) {
+      c.g0 = g;
+      run_ops(body_, c);
+    }
+  }
+  c.g0 = 0;
+  run_ops(post_, c);
+}
+
+void Executor::autotune(const Ctx& c, hipStream_t s) {
+  // Time the candidate kernels of every GEMM site on its real operands once,
+  // keyed by shape, and keep the fastest (the executor's workspace holds
+  // operands of the right size; values do not influence timing).
+  std::map<std::tuple<int64_t, int64_t, int64_t, int64_t, int64_t>, int> best;
+  hipEvent_t e0, e1;
+  ATN_HIP(hipEventCreate(&e0));
+  ATN_HIP(hipEventCreate(&e1));
+  for (size_t i = 0; i < gemm_sites_.size(); ++i) {
+    auto& site = gemm_sites_[i];
+    auto key = std::make_tuple(site.g.m, site.g.k, site.g.n, site.g.nbo, site.g.nbi);
+    auto it = best.find(key);
+    if (it != best.end()) {
+      gemm_choice_[i] = it->second;
+      continue;
+    }
+    GemmArgs q = site.g;
+    q.a = c.ptr(site.a), q.b = c.ptr(site.b), q.c = c.ptr(site.c);
+    float best_ms = 1e30f;
+    int choice = GEMM_SIMPLE;
+    for (int cand : {GEMM_SIMPLE, GEMM_MFMA}) {
+      launch_gemm(o_.dtype, o_.precision, cand, q, s);
+      ATN_HIP(hipEventRecord(e0, s));
+      for (int r = 0; r < 3; ++r) launch_gemm(o_.dtype, o_.precision, cand, q, s);
+      ATN_HIP(hipEventRecord(e1, s));
+      ATN_HIP(hipEventSynchronize(e1));
+      float ms = 0;
+      ATN_HIP(hipEventElapsedTime(&ms, e0, e1));
+      if (ms < best_ms) best_ms = ms, choice = cand;
+    }
+    gemm_choice_[i] = best[key] = choice;
+  }
+  ATN_HIP(hipEventDestroy(e0));
+  ATN_HIP(hipEventDestroy(e1));
+  tuned_ = true;
+}
+
+void Executor::run(const void* const* inputs, void* out, hipStream_t s, Report* rep) {
+  const Plan& P = *P_;
+  Ctx c;
+  c.ws = ws_;
+  c.offs = &offs_;
+  c.inputs = inputs;
+  c.out = out;
+  c.s = s;
+  if (o_.autotune && !tuned_) autotune(c, s);
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (rep) {
+    ATN_HIP(hipEventCreate(&e0));
+    ATN_HIP(hipEventCreate(&e1));
+    ATN_HIP(hipEventRecord(e0, s));
+  }
+  size_t l0 = g_launch_count;
+  if (o_.use_graph) {
+    std::vector<const void*> ins(inputs, inputs + P.net.inputs.size());
+    if (!gexec_ || ins != graph_inputs_ || out != graph_out_) {
+      if (gexec_) (void)hipGraphExecDestroy(gexec_), gexec_ = nullptr;
+      if (graph_) (void)hipGraphDestroy(graph_), graph_ = nullptr;
+      Ctx cc = c;
+      cc.s = cap_stream_;
+      size_t lc = g_launch_count;
+      ATN_HIP(hipStreamBeginCapture(cap_stream_, hipStreamCaptureModeThreadLocal));
+      try {
+        enqueue(cc);
+      } catch (...) {
+        hipGraph_t g = nullptr;
+        (void)hipStreamEndCapture(cap_stream_, &g);
+        if (g) (void)hipGraphDestroy(g);
+        throw;
+      }
+      ATN_HIP(hipStreamEndCapture(cap_stream_, &graph_));
+      ATN_HIP(hipGraphInstantiate(&gexec_, graph_, nullptr, nullptr, 0));
+      launches_per_run_ = g_launch_count - lc;
+      graph_inputs_ = ins;
+      graph_out_ = out;
+    }
+    ATN_HIP(hipGraphLaunch(gexec_, s));
+  } else {
+    enqueue(c);
+    launches_per_run_ = g_launch_count - l0;
+  }
+  if (rep) {
+    ATN_HIP(hipEventRecord(e1, s));
+    ATN_HIP(hipEventSynchronize(e1));
+    float ms = 0;
+    ATN_HIP(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    rep->elapsed_ms = ms;
+    double c8 = dtype_complex(o_.dtype) ? 8.0 : 2.0;
+    rep->flops = c8 * P.flops_per_slice * (double)P.n_slices;
+    rep->tflops = ms > 0 ? rep->flops / (ms * 1e-3) / 1e12 : 0;
+    rep->slices_per_s = ms > 0 ? (double)P.n_slices / (ms * 1e-3) : 0;
+    rep->n_slices = P.n_slices;
+    rep->batch = B_;
+    rep->n_launches = launches_per_run_;
+    rep->workspace_bytes = ws_bytes_;
+    rep->log10_exponent = 0;
+    if (o_.strip) {
+      int64_t e = 0;
+      ATN_HIP(hipMemcpy(&e, ws_ + offs_[expo_buf_], 8, hipMemcpyDeviceToHost));
+      rep->log10_exponent = (double)e * std::log10(2.0);
+    }
+  }
+}
+
+// --------------------------------------------------------------- stages ------
+namespace {
+struct Scratch {
+  hipStream_t s;
+  std::vector<void*> ptrs;
+  explicit Scratch(hipStream_t st) : s(st) {}
+  void* get(size_t bytes) {
+    void* p = nullptr;
+    ATN_HIP(hipMallocAsync(&p, std::max<size_t>(bytes, 256), s));
+    ptrs.push_back(p);
+    return p;
+  }
+  ~Scratch() {
+    for (void* p : ptrs) (void)hipFreeAsync(p, s);
+  }
+};
+
+// sums then permute of one operand; returns the pointer holding the GEMM operand
+const void* operand(Scratch& sc, int dt, const void* x, std::vector<size_t> shape, const std::vector<size_t>& sums,
+                    const std::vector<size_t>& perm) {
+  int es = (int)dtype_size(dt);
+  for (size_t ax : sums) {
+    size_t outer = 1, inner = 1, d = shape[ax];
+    for (size_t q = 0; q < ax; ++q) outer *= shape[q];
+    for (size_t q = ax + 1; q < shape.size(); ++q) inner *= shape[q];
+    void* dst = sc.get(outer * inner * es);
+    launch_sum_axis(dt, outer, d, inner, x, dst, sc.s);
+    shape.erase(shape.begin() + ax);
+    x = dst;
+  }
+  PermPlan pp = plan_permute(shape, perm, 1, es);
+  if (pp.kind == 0) return x;
+  void* dst = sc.get(pp.numel * es);
+  launch_permute(pp, x, dst, sc.s);
+  return dst;
+}
+
+// inverse of `operand`: g has the permuted shape; writes the gradient of x.
+void operand_adjoint(Scratch& sc, int dt, const void* g, std::vector<size_t> shape0, const std::vector<size_t>& sums,
+                     const std::vector<size_t>& perm, void* out) {
+  int es = (int)dtype_size(dt);
+  std::vector<std::vector<size_t>> shapes{shape0};
+  for (size_t ax : sums) {
+    auto s = shapes.back();
+    s.erase(s.begin() + ax);
+    shapes.push_back(s);
+  }
+  auto& sh = shapes.back();
+  std::vector<size_t> pshape;
+  for (size_t ax : perm) pshape.push_back(sh[ax]);
+  std::vector<size_t> inv(perm.size());
+  for (size_t i = 0; i < perm.size(); ++i) inv[perm[i]] = i;
+  PermPlan pp = plan_permute(pshape, inv, 1, es);
+  const void* cur = g;
+  if (sums.empty()) {
+    if (pp.kind == 0) launch_copy(prod(sh) * es, g, out, sc.s);
+    else launch_permute(pp, g, out, sc.s);
+    return;
+  }
+  if (pp.kind != 0) {
+    void* dst = sc.get(prod(sh) * es);
+    launch_permute(pp, g, dst, sc.s);
+    cur = dst;
+  }
+  for (size_t q = sums.size(); q-- > 0;) {
+    auto& full = shapes[q];
+    size_t ax = sums[q], outer = 1, inner = 1, d = full[ax];
+    for (size_t i = 0; i < ax; ++i) outer *= full[i];
+    for (size_t i = ax + 1; i < full.size(); ++i) inner *= full[i];
+    void* dst = q == 0 ? out : sc.get(prod(full) * es);
+    launch_broadcast_axis(es, outer, d, inner, cur, dst, sc.s);
+    cur = dst;
+  }
+}
+}  // namespace
+
+void stage_prep_forward(const Plan& p, size_t i, int dt, const void* in, void* out, hipStream_t s) {
+  if (i >= p.cp.n_inputs) throw Error("input index out of range");
+  const InputPrep& q = p.cp.preps[i];
+  int es = (int)dtype_size(dt);
+  Scratch sc(s);
+  auto shape = q.input_shape;
+  const void* cur = in;
+  if (q.traces.empty()) {
+    launch_copy(prod(shape) * es, in, out, s);
+    return;
+  }
+  for (size_t t = 0; t < q.traces.size(); ++t) {
+    size_t ai = std::get<0>(q.traces[t]), aj = std::get<1>(q.traces[t]);
+    bool diag = std::get<2>(q.traces[t]);
+    size_t A = 1, d = shape[ai], Bm = 1, C = 1;
+    for (size_t k = 0; k < ai; ++k) A *= shape[k];
+    for (size_t k = ai + 1; k < aj; ++k) Bm *= shape[k];
+    for (size_t k = aj + 1; k < shape.size(); ++k) C *= shape[k];
+    void* dst = t + 1 == q.traces.size() ? out : sc.get((diag ? A * d * Bm * C : A * Bm * C) * es);
+    if (diag) launch_diag(es, A, d, Bm, C, cur, dst, s);
+    else launch_trace(dt, A, d, Bm, C, cur, dst, s);
+    shape.erase(shape.begin() + aj);
+    if (!diag) shape.erase(shape.begin() + ai);
+    cur = dst;
+  }
+}
+
+void stage_prep_adjoint(const Plan& p, size_t i, int dt, const void* g, void* gin, hipStream_t s) {
+  if (i >= p.cp.n_inputs) throw Error("input index out of range");
+  const InputPrep& q = p.cp.preps[i];
+  int es = (int)dtype_size(dt);
+  Scratch sc(s);
+  std::vector<std::vector<size_t>> shapes{q.input_shape};
+  for (auto& t : q.traces) {
+    auto sh = shapes.back();
+    sh.erase(sh.begin() + std::get<1>(t));
+    if (!std::get<2>(t)) sh.erase(sh.begin() + std::get<0>(t));
+    shapes.push_back(sh);
+  }
+  if (q.traces.empty()) {
+    launch_copy(prod(q.input_shape) * es, g, gin, s);
+    return;
+  }
+  const void* cur = g;
+  for (size_t t = q.traces.size(); t-- > 0;) {
+    auto& full = shapes[t];
+    size_t ai = std::get<0>(q.traces[t]), aj = std::get<1>(q.traces[t]);
+    bool diag = std::get<2>(q.traces[t]);
+    size_t A = 1, d = full[ai], Bm = 1, C = 1;
+    for (size_t k = 0; k < ai; ++k) A *= full[k];
+    for (size_t k = ai + 1; k < aj; ++k) Bm *= full[k];
+    for (size_t k = aj + 1; k < full.size(); ++k) C *= full[k];
+    void* dst = t == 0 ? gin : sc.get(prod(full) * es);
+    launch_fill_zero(dst, prod(full) * es, s);
+    launch_diag_scatter(es, !diag, A, d, Bm, C, cur, dst, s);
+    cur = dst;
+  }
+}
+
+void stage_step_forward(const Plan& p, size_t step, int dt, int prec, const void* l, const void* r, void* out, hipStream_t s) {
+  if (step >= p.cp.steps.size()) throw Error("step index out of range");
+  const PairStep& st = p.cp.steps[step];
+  Scratch sc(s);
+  const void* a = operand(sc, dt, l, st.left_shape, st.sum_a, st.a_perm);
+  const void* b = operand(sc, dt, r, st.right_shape, st.sum_b, st.b_perm);
+  GemmArgs g{};
+  g.a = a, g.b = b, g.c = out;
+  g.m = st.m, g.k = st.k, g.n = st.n;
+  g.lda = g.k, g.ldb = g.n, g.ldc = g.n;
+  g.nbo = 1, g.nbi = st.bt;
+  g.sai = g.m * g.k, g.sbi = g.k * g.n, g.sci = g.m * g.n;
+  launch_gemm(dt, prec, GEMM_AUTO, g, s);
+}
+
+void stage_step_adjoint(const Plan& p, size_t step, int dt, const void* l, const void* r, const void* gout, void* gl, void* gr,
+                        hipStream_t s) {
+  if (step >= p.cp.steps.size()) throw Error("step index out of range");
+  const PairStep& st = p.cp.steps[step];
+  int es = (int)dtype_size(dt);
+  Scratch sc(s);
+  const void* a2 = operand(sc, dt, l, st.left_shape, st.sum_a, st.a_perm);
+  const void* b2 = operand(sc, dt, r, st.right_shape, st.sum_b, st.b_perm);
+  int64_t bt = st.bt, m = st.m, k = st.k, n = st.n;
+  if (gl) {
+    // gA2[bt, m, k] = G[bt, m, n] . B2^H  (B2 stored [k, n])
+    void* ga2 = sc.get(bt * m * k * es);
+    GemmArgs g{};
+    g.a = gout, g.b = b2, g.c = ga2;
+    g.m = m, g.k = n, g.n = k;
+    g.lda = n, g.ldb = n, g.ldc = k;
+    g.nbo = 1, g.nbi = bt;
+    g.sai = m * n, g.sbi = k * n, g.sci = m * k;
+    launch_gemm_adjoint(dt, 0, 1, g, s);
+    operand_adjoint(sc, dt, ga2, st.left_shape, st.sum_a, st.a_perm, gl);
+  }
+  if (gr) {
+    // gB2[bt, k, n] = A2^H . G  (A2 stored [m, k])
+    void* gb2 = sc.get(bt * k * n * es);
+    GemmArgs g{};
+    g.a = a2, g.b = gout, g.c = gb2;
+    g.m = k, g.k = m, g.n = n;
+    g.lda = k, g.ldb = n, g.ldc = n;
+    g.nbo = 1, g.nbi = bt;
+    g.sai = m * k, g.sbi = m * n, g.sci = k * n;
+    launch_gemm_adjoint(dt, 1, 0, g, s);
+    operand_adjoint(sc, dt, gb2, st.right_shape, st.sum_b, st.b_perm, gr);
+  }
+}
+
+void stage_finalize_forward(const Plan& p, int dt, const void* in, void* out, hipStream_t s) {
+  const Finalize& f = p.cp.fin;
+  Scratch sc(s);
+  int es = (int)dtype_size(dt);
+  const void* x = operand(sc, dt, in, f.input_shape, f.sum, f.perm);
+  launch_copy(prod(f.output_shape) * es, x, out, s);
+}
+
+void stage_finalize_adjoint(const Plan& p, int dt, const void* g, void* gin, hipStream_t s) {
+  const Finalize& f = p.cp.fin;
+  Scratch sc(s);
+  operand_adjoint(sc, dt, g, f.input_shape, f.sum, f.perm, gin);
+}
+
+// --------------------------------------------------------------- grouped -----
+void grouped_execute(const std::vector<std::shared_ptr<const Plan>>& plans, int dt, bool strip,
+                     const std::vector<const void* const*>& inputs, const std::vector<void*>& outputs, double* log10_exp,
+                     hipStream_t s, Report* rep) {
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (rep) {
+    ATN_HIP(hipEventCreate(&e0));
+    ATN_HIP(hipEventCreate(&e1));
+    ATN_HIP(hipEventRecord(e0, s));
+  }
+  double flops = 0;
+  size_t l0 = g_launch_count;
+  for (size_t p = 0; p < plans.size(); ++p) {
+    ExecOptions o;
+    o.dtype = dt;
+    o.strip = strip;
+    Executor ex(plans[p], o);
+    Report r;
+    ex.run(inputs[p], outputs[p], s, strip ? &r : nullptr);
+    if (log10_exp) log10_exp[p] = strip ? r.log10_exponent : 0.0;
+    flops += (dtype_complex(dt) ? 8.0 : 2.0) * plans[p]->flops_per_slice * (double)plans[p]->n_slices;
+    ATN_HIP(hipStreamSynchronize(s));
+  }
+  if (rep) {
+    ATN_HIP(hipEventRecord(e1, s));
+    ATN_HIP(hipEventSynchronize(e1));
+    float ms = 0;
+    ATN_HIP(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    rep->elapsed_ms = ms;
+    rep->flops = flops;
+    rep->tflops = ms > 0 ? flops / (ms * 1e-3) / 1e12 : 0;
+    rep->n_slices = plans.size();
+    rep->n_launches = g_launch_count - l0;
+  }
+}
+
+}  // namespace atn

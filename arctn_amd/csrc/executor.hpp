@@ -1,0 +1,127 @@
+// Executors of a Plan on one GPU.
+#pragma once
+
+#include <functional>
+#include <memory>
+#include <mutex>
+
+#include "kernels.hpp"
+#include "plan.hpp"
+
+namespace atn {
+
+struct ExecOptions {
+  int dtype = C64;
+  int precision = NATIVE;
+  bool batched = false;
+  bool use_graph = false;
+  bool autotune = false;
+  bool strip = false;
+  size_t max_batch = 0;
+  size_t workspace_limit = 0;
+};
+
+struct Report {
+  double elapsed_ms = 0, flops = 0, tflops = 0, slices_per_s = 0, log10_exponent = 0, max_abs_result = 0;
+  size_t n_slices = 0, batch = 0, n_launches = 0, workspace_bytes = 0;
+};
+
+// Where an operand lives at run time.
+struct Ref {
+  enum Kind { NONE, WS, INPUT, OUTPUT } kind = NONE;
+  int id = 0;  // workspace buffer id or input index
+};
+
+struct Ctx {
+  char* ws = nullptr;
+  const std::vector<size_t>* offs = nullptr;
+  const void* const* inputs = nullptr;
+  void* out = nullptr;
+  hipStream_t s = nullptr;
+  uint64_t g0 = 0;
+  void* ptr(const Ref& r) const {
+    switch (r.kind) {
+      case Ref::WS: return ws + (*offs)[r.id];
+      case Ref::INPUT: return const_cast<void*>(inputs[r.id]);
+      case Ref::OUTPUT: return out;
+      default: return nullptr;
+    }
+  }
+};
+
+using Op = std::function<void(const Ctx&)>;
+
+class Executor {
+ public:
+  Executor(std::shared_ptr<const Plan> plan, const ExecOptions& o);
+  ~Executor();
+  void run(const void* const* inputs, void* out, hipStream_t s, Report* rep);
+  size_t batch() const { return B_; }
+  size_t workspace_bytes() const { return ws_bytes_; }
+
+ private:
+  struct Buf {
+    size_t bytes;
+    int first, last;
+  };
+  void build(size_t B);
+  int newbuf(size_t bytes);
+  void use(const Ref& r);
+  void emit(Op op) { cur_->push_back(std::move(op)); ++t_; }
+  void gen_steps(bool body, size_t nb, std::vector<Ref>& slots, std::vector<std::vector<size_t>>& shapes);
+  void gen_inputs(bool body, size_t nb, std::vector<Ref>& slots, std::vector<std::vector<size_t>>& shapes);
+  Ref gen_finalize(size_t nb, const Ref& src, bool to_output);
+  void gen_gemm(const PairStep& st, size_t nb_outer, bool va, bool vb, const Ref& a, const Ref& b, const Ref& c);
+  void run_ops(const std::vector<Op>& ops, const Ctx& c);
+  void enqueue(const Ctx& c);
+  void autotune(const Ctx& c, hipStream_t s);
+
+  struct GemmSite {
+    GemmArgs g;
+    Ref a, b, c;
+  };
+
+  std::shared_ptr<const Plan> P_;
+  ExecOptions o_;
+  int es_;
+  size_t B_ = 1;
+  std::vector<Buf> bufs_;
+  std::vector<size_t> offs_;
+  std::vector<Op> pre_, body_, tail_, post_;
+  std::vector<Op>* cur_ = nullptr;
+  int t_ = 0;
+  int body_begin_ = 0, body_end_ = 0;
+  size_t tail_nb_ = 0;
+  char* ws_ = nullptr;
+  size_t ws_bytes_ = 0;
+  int acc_buf_ = -1, expo_buf_ = -1, maxbits_buf_ = -1;
+  hipGraphExec_t gexec_ = nullptr;
+  hipGraph_t graph_ = nullptr;
+  hipStream_t cap_stream_ = nullptr;
+  std::vector<const void*> graph_inputs_;
+  void* graph_out_ = nullptr;
+  size_t launches_per_run_ = 0;
+  // gemm kernel choice per GEMM op (autotune)
+  std::vector<int> gemm_choice_;
+  std::vector<GemmSite> gemm_sites_;
+  bool tuned_ = false;
+  // tail batch generation replays the body's buffer ids
+  std::vector<int> body_ids_;
+  size_t replay_pos_ = 0;
+  bool replay_ = false, record_ = false;
+};
+
+// Stage execution with stream-ordered scratch (src/compiled.rs:60-63, 143-153, 206-209).
+void stage_prep_forward(const Plan& p, size_t i, int dt, const void* in, void* out, hipStream_t s);
+void stage_prep_adjoint(const Plan& p, size_t i, int dt, const void* g, void* gin, hipStream_t s);
+void stage_step_forward(const Plan& p, size_t step, int dt, int prec, const void* l, const void* r, void* out, hipStream_t s);
+void stage_step_adjoint(const Plan& p, size_t step, int dt, const void* l, const void* r, const void* g, void* gl, void* gr,
+                        hipStream_t s);
+void stage_finalize_forward(const Plan& p, int dt, const void* in, void* out, hipStream_t s);
+void stage_finalize_adjoint(const Plan& p, int dt, const void* g, void* gin, hipStream_t s);
+
+void grouped_execute(const std::vector<std::shared_ptr<const Plan>>& plans, int dt, bool strip,
+                     const std::vector<const void* const*>& inputs, const std::vector<void*>& outputs, double* log10_exp,
+                     hipStream_t s, Report* rep);
+
+}  // namespace atn

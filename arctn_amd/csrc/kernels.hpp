@@ -1,0 +1,111 @@
+// Host-side interface of the gfx950 kernels.
+#pragma once
+
+#include "common.hpp"
+
+namespace atn {
+
+constexpr int MAXR = 40;   // max (merged) rank handled by the generic kernels
+constexpr int MAXT = 16;   // max axes per tile group of the LDS permute
+
+// ---- permute (src/tensor.rs:453-492), pure copy, bit-exact ----------------
+struct PermTile {
+  // Outer (grid) axes: one tile per index.
+  int n_r;
+  uint32_t r_dims[MAXR];
+  int64_t r_in[MAXR], r_out[MAXR];
+  // Tile U = G_in u G_out. LDS holds U in output-order row-major layout.
+  int n_hin, n_gin, n_hout;
+  uint32_t hin_dims[MAXT], gin_dims[MAXT], hout_dims[MAXT];
+  int64_t hin_in[MAXT], hout_out[MAXT];
+  int32_t hin_lds[MAXT], gin_lds[MAXT];
+  uint32_t u_size, lo_in, lo_out, n_hi_in, n_hi_out;
+  uint64_t n_tiles;
+};
+struct PermGeneric {
+  int rank;
+  uint32_t dims[MAXR];   // output-order dims
+  int64_t in_str[MAXR];  // input stride of each output axis
+  uint64_t numel;
+};
+struct PermPlan {
+  int kind = 0;  // 0 identity (alias), 1 LDS tile, 2 generic
+  int es = 8;    // element bytes
+  size_t numel = 0;
+  PermTile tile;
+  PermGeneric gen;
+  unsigned threads = 256;
+  size_t lds = 0;
+};
+// Permute `shape` by `perm` (output axis d = input axis perm[d]) for `batch`
+// consecutive tensors.
+PermPlan plan_permute(const std::vector<size_t>& shape, const std::vector<size_t>& perm, size_t batch, int es);
+void launch_permute(const PermPlan& p, const void* in, void* out, hipStream_t s);
+
+// ---- reductions / gathers (src/tensor.rs:503-669) --------------------------
+// out[o, q] = sum_{i<d} in[o, i, q] (sequential order), view [outer, d, inner]
+void launch_sum_axis(int dt, size_t outer, size_t d, size_t inner, const void* in, void* out, hipStream_t s);
+// view [A, d, Bm, d, C]; trace -> [A, Bm, C]; diag -> [A, d, Bm, C]
+void launch_trace(int dt, size_t A, size_t d, size_t Bm, size_t C, const void* in, void* out, hipStream_t s);
+void launch_diag(int es, size_t A, size_t d, size_t Bm, size_t C, const void* in, void* out, hipStream_t s);
+
+// Slice gather: out[b, e] = in[base(g0 + b) + off(e)].
+struct GatherArgs {
+  int n_ax;
+  uint32_t dims[MAXR];
+  int64_t str[MAXR];
+  int n_sel;
+  uint64_t sel_div[MAXR], sel_mod[MAXR];
+  int64_t sel_str[MAXR];
+  uint64_t inner;
+};
+void launch_gather(int es, const GatherArgs& a, uint64_t g0, size_t nb, const void* in, void* out, hipStream_t s);
+
+// acc64[blk, e] += sum over slices g in [g0, g0+nb) of block blk of part[g-g0, e]
+void launch_reduce_f64(int dt, const void* part, void* acc64, uint64_t g0, size_t nb, size_t n_internal, size_t numel,
+                       hipStream_t s);
+// out[scatter(blk, e)] = (T) src[blk, e]; src_f64 selects a double accumulator as the source.
+struct ScatterArgs {
+  int n_blk;
+  uint64_t blk_div[MAXR], blk_mod[MAXR];
+  int64_t blk_str[MAXR];
+  int n_kept;
+  uint32_t kept_dims[MAXR];
+  int64_t kept_str[MAXR];
+};
+void launch_scatter(int dt, bool src_f64, const ScatterArgs& a, uint64_t blk0, size_t n_blk, size_t numel,
+                    const void* src, void* out, hipStream_t s);
+// acc[e] (+)= part[e] in T (first = copy)
+void launch_accum(int dt, bool first, const void* part, void* acc, size_t numel, hipStream_t s);
+void launch_fill_zero(void* p, size_t bytes, hipStream_t s);
+void launch_copy(size_t bytes, const void* in, void* out, hipStream_t s);
+// Adjoint helpers: out[o, i, q] = in[o, q] (broadcast of a summed axis), and
+// the transpose of trace / diagonal extraction: out (pre-zeroed) [A, d, Bm, d, C]
+// receives g on its diagonal; g is [A, Bm, C] (trace) or [A, d, Bm, C] (diag).
+void launch_broadcast_axis(int es, size_t outer, size_t d, size_t inner, const void* in, void* out, hipStream_t s);
+void launch_diag_scatter(int es, bool trace, size_t A, size_t d, size_t Bm, size_t C, const void* g, void* out, hipStream_t s);
+
+// ---- strip (src/compiled.rs:412-430), power-of-two variant ----------------
+// maxbits: one uint64 slot (non-negative double bits); expo: int64 running exponent.
+void launch_maxabs(int dt, const void* x, size_t numel, uint64_t* maxbits, hipStream_t s);
+void launch_scale_pow2(int dt, void* x, size_t numel, const uint64_t* maxbits, int64_t* expo, hipStream_t s);
+
+// ---- GEMM -------------------------------------------------------------------
+struct GemmArgs {
+  const void* a;
+  const void* b;
+  void* c;
+  int64_t m, k, n;
+  int64_t lda, ldb, ldc;       // row strides in elements
+  int64_t sao, sai, sbo, sbi, sco, sci;  // batch strides (outer slice batch, inner plan batch)
+  int64_t nbo, nbi;
+};
+enum GemmKernel { GEMM_AUTO = -1, GEMM_SIMPLE = 0, GEMM_MFMA = 1, GEMM_MFMA_SMALL = 2 };
+// Returns the kernel used.
+int launch_gemm(int dt, int prec, int kernel, const GemmArgs& g, hipStream_t s);
+// Conjugating variants for the adjoint: opA/opB: 0 = N, 1 = conj-transpose (as stored).
+void launch_gemm_adjoint(int dt, int opa, int opb, const GemmArgs& g, hipStream_t s);
+
+extern thread_local size_t g_launch_count;
+
+}  // namespace atn

@@ -1,0 +1,693 @@
+// Data-movement kernels: LDS-tiled permute, sums, traces, diagonals, slice
+// gathers, cross-slice reductions and power-of-two strip. Pure copies and
+// sequential-order sums, so results are bit-identical to the reference's
+// src/tensor.rs index loops.
+#include <algorithm>
+#include <numeric>
+
+#include "kernels.hpp"
+
+namespace atn {
+
+thread_local size_t g_launch_count = 0;
+
+template <int ES> struct Bits;
+template <> struct Bits<4> { using T = uint32_t; };
+template <> struct Bits<8> { using T = uint64_t; };
+template <> struct Bits<16> { using T = uint4; };
+
+template <class T> struct Arith;
+template <> struct Arith<float> {
+  static __device__ float add(float a, float b) { return a + b; }
+  static __device__ double mag(float a) { return fabs((double)a); }
+  static __device__ float scale(float a, float s) { return a * s; }
+};
+template <> struct Arith<double> {
+  static __device__ double add(double a, double b) { return a + b; }
+  static __device__ double mag(double a) { return fabs(a); }
+  static __device__ double scale(double a, double s) { return a * s; }
+};
+template <> struct Arith<float2> {
+  static __device__ float2 add(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+  static __device__ double mag(float2 a) { return fmax(fabs((double)a.x), fabs((double)a.y)); }
+  static __device__ float2 scale(float2 a, float s) { return make_float2(a.x * s, a.y * s); }
+};
+template <> struct Arith<double2> {
+  static __device__ double2 add(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+  static __device__ double mag(double2 a) { return fmax(fabs(a.x), fabs(a.y)); }
+  static __device__ double2 scale(double2 a, double s) { return make_double2(a.x * s, a.y * s); }
+};
+
+template <class F>
+static void dispatch_dt(int dt, F&& f) {
+  switch (dt) {
+    case F32: f(float{}); break;
+    case F64: f(double{}); break;
+    case C64: f(float2{}); break;
+    case C128: f(double2{}); break;
+    default: throw Error("unsupported dtype code");
+  }
+}
+template <class F>
+static void dispatch_es(int es, F&& f) {
+  switch (es) {
+    case 4: f(std::integral_constant<int, 4>{}); break;
+    case 8: f(std::integral_constant<int, 8>{}); break;
+    case 16: f(std::integral_constant<int, 16>{}); break;
+    default: throw Error("unsupported element size");
+  }
+}
+
+static unsigned grid_for(uint64_t work, unsigned threads, unsigned cap = 65536u) {
+  uint64_t g = (work + threads - 1) / threads;
+  if (g < 1) g = 1;
+  return (unsigned)std::min<uint64_t>(g, cap);
+}
+
+#define LAUNCH(kernel, grid, block, lds, stream, ...)                            \
+  do {                                                                            \
+    hipLaunchKernelGGL(kernel, dim3(grid), dim3(block), lds, stream, __VA_ARGS__); \
+    ATN_HIP(hipGetLastError());                                                   \
+    ++g_launch_count;                                                             \
+  } while (0)
+
+// ------------------------------------------------------------------ permute --
+// One workgroup per tile of the axis set U. Phase 1 reads U with the input's
+// innermost axes (G_in) contiguous across lanes and scatters into LDS laid out
+// in output order; phase 2 streams LDS out with the output's innermost axes
+// (G_out) contiguous. Offset tables for the non-contiguous part of each phase
+// are built once per workgroup in LDS.
+template <int ES>
+__global__ __launch_bounds__(256) void permute_tile_kernel(PermTile a, const typename Bits<ES>::T* __restrict__ in,
+                                                           typename Bits<ES>::T* __restrict__ out) {
+  using E = typename Bits<ES>::T;
+  extern __shared__ __align__(16) char smem[];
+  int64_t* in_hi = reinterpret_cast<int64_t*>(smem);
+  int64_t* out_hi = in_hi + a.n_hi_in;
+  int32_t* lds_hi = reinterpret_cast<int32_t*>(out_hi + a.n_hi_out);
+  int32_t* lds_lo = lds_hi + a.n_hi_in;
+  size_t tab_bytes = (size_t)(a.n_hi_in + a.n_hi_out) * 8 + (size_t)(a.n_hi_in + a.lo_in) * 4;
+  E* tile = reinterpret_cast<E*>(smem + ((tab_bytes + 15) & ~(size_t)15));
+
+  for (uint32_t h = threadIdx.x; h < a.n_hi_in; h += blockDim.x) {
+    uint32_t r = h;
+    int64_t o = 0;
+    int32_t l = 0;
+    for (int d = a.n_hin - 1; d >= 0; --d) {
+      uint32_t dim = a.hin_dims[d], q = r / dim, i = r - q * dim;
+      r = q;
+      o += (int64_t)i * a.hin_in[d];
+      l += (int32_t)i * a.hin_lds[d];
+    }
+    in_hi[h] = o;
+    lds_hi[h] = l;
+  }
+  for (uint32_t h = threadIdx.x; h < a.lo_in; h += blockDim.x) {
+    uint32_t r = h;
+    int32_t l = 0;
+    for (int d = a.n_gin - 1; d >= 0; --d) {
+      uint32_t dim = a.gin_dims[d], q = r / dim, i = r - q * dim;
+      r = q;
+      l += (int32_t)i * a.gin_lds[d];
+    }
+    lds_lo[h] = l;
+  }
+  for (uint32_t h = threadIdx.x; h < a.n_hi_out; h += blockDim.x) {
+    uint32_t r = h;
+    int64_t o = 0;
+    for (int d = a.n_hout - 1; d >= 0; --d) {
+      uint32_t dim = a.hout_dims[d], q = r / dim, i = r - q * dim;
+      r = q;
+      o += (int64_t)i * a.hout_out[d];
+    }
+    out_hi[h] = o;
+  }
+  __syncthreads();
+
+  for (uint64_t t = blockIdx.x; t < a.n_tiles; t += gridDim.x) {
+    uint64_t r = t;
+    int64_t bin = 0, bout = 0;
+    for (int d = a.n_r - 1; d >= 0; --d) {
+      uint64_t dim = a.r_dims[d], q = r / dim, i = r - q * dim;
+      r = q;
+      bin += (int64_t)i * a.r_in[d];
+      bout += (int64_t)i * a.r_out[d];
+    }
+    const E* src = in + bin;
+    E* dst = out + bout;
+    for (uint32_t j = threadIdx.x; j < a.u_size; j += blockDim.x) {
+      uint32_t hi = j / a.lo_in, lo = j - hi * a.lo_in;
+      tile[lds_lo[lo] + lds_hi[hi]] = src[in_hi[hi] + lo];
+    }
+    __syncthreads();
+    for (uint32_t j = threadIdx.x; j < a.u_size; j += blockDim.x) {
+      uint32_t hi = j / a.lo_out, lo = j - hi * a.lo_out;
+      dst[out_hi[hi] + lo] = tile[j];
+    }
+    __syncthreads();
+  }
+}
+
+template <int ES>
+__global__ __launch_bounds__(256) void permute_generic_kernel(PermGeneric a, const typename Bits<ES>::T* __restrict__ in,
+                                                              typename Bits<ES>::T* __restrict__ out) {
+  for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e < a.numel; e += (uint64_t)gridDim.x * blockDim.x) {
+    uint64_t r = e;
+    int64_t off = 0;
+    for (int d = a.rank - 1; d >= 0; --d) {
+      uint64_t dim = a.dims[d], q = r / dim;
+      off += (int64_t)(r - q * dim) * a.in_str[d];
+      r = q;
+    }
+    out[e] = in[off];
+  }
+}
+
+PermPlan plan_permute(const std::vector<size_t>& shape0, const std::vector<size_t>& perm0, size_t batch, int es) {
+  PermPlan p;
+  p.es = es;
+  p.numel = batch * prod(shape0);
+  // Drop unit axes, then merge input-adjacent axes that stay adjacent in the output.
+  std::vector<size_t> keep;  // input axes kept
+  for (size_t a = 0; a < shape0.size(); ++a)
+    if (shape0[a] != 1) keep.push_back(a);
+  std::vector<size_t> perm;  // over kept axes, in output order
+  for (size_t d : perm0)
+    if (shape0[d] != 1) perm.push_back((size_t)(std::find(keep.begin(), keep.end(), d) - keep.begin()));
+  std::vector<size_t> shape;
+  for (size_t a : keep) shape.push_back(shape0[a]);
+  // groups in output order: runs of consecutive input axes
+  std::vector<std::vector<size_t>> groups;
+  for (size_t d = 0; d < perm.size(); ++d) {
+    if (!groups.empty() && groups.back().back() + 1 == perm[d]) groups.back().push_back(perm[d]);
+    else groups.push_back({perm[d]});
+  }
+  // merged axes, input order: (dim, in_first_axis); output order = order of groups
+  size_t ng = groups.size();
+  std::vector<size_t> gdim(ng), gfirst(ng);
+  for (size_t g = 0; g < ng; ++g) {
+    size_t d = 1;
+    for (size_t a : groups[g]) d *= shape[a];
+    gdim[g] = d;
+    gfirst[g] = groups[g][0];
+  }
+  std::vector<size_t> in_order(ng);  // merged axes sorted by input position
+  std::iota(in_order.begin(), in_order.end(), 0);
+  std::sort(in_order.begin(), in_order.end(), [&](size_t x, size_t y) { return gfirst[x] < gfirst[y]; });
+  bool ident = true;
+  for (size_t i = 0; i < ng; ++i) ident = ident && in_order[i] == i;
+  if (ident) {
+    p.kind = 0;
+    return p;
+  }
+  // Split large merged axes into factors <= 64 so tiles stay small (a reshape
+  // of a contiguous group is valid on both sides).
+  struct Ax { size_t dim; size_t out_pos; size_t in_pos; };
+  std::vector<Ax> axes;  // in input order, each with its output position (fractional order via pairs)
+  {
+    std::vector<std::vector<size_t>> pieces(ng);  // factors of each merged axis, outer to inner
+    for (size_t g = 0; g < ng; ++g) {
+      size_t d = gdim[g];
+      std::vector<size_t> f;
+      while (d > 64) {
+        size_t best = 0;
+        for (size_t c = 64; c >= 8; --c)
+          if (d % c == 0) {
+            best = c;
+            break;
+          }
+        if (!best) break;
+        f.push_back(best);
+        d /= best;
+      }
+      f.push_back(d);
+      std::reverse(f.begin(), f.end());  // outer factor first
+      pieces[g] = f;
+    }
+    // Output order enumerates groups in output order, pieces outer->inner.
+    size_t opos = 0;
+    std::vector<std::vector<size_t>> out_pos(ng);
+    for (size_t g = 0; g < ng; ++g)
+      for (size_t q = 0; q < pieces[g].size(); ++q) out_pos[g].push_back(opos++);
+    size_t ipos = 0;
+    for (size_t gi : in_order)
+      for (size_t q = 0; q < pieces[gi].size(); ++q) axes.push_back(Ax{pieces[gi][q], out_pos[gi][q], ipos++});
+  }
+  size_t r = axes.size();
+  // strides
+  std::vector<int64_t> in_str(r), out_str(r);
+  {
+    int64_t s = 1;
+    for (size_t a = r; a-- > 0;) {
+      in_str[a] = s;
+      s *= (int64_t)axes[a].dim;
+    }
+    std::vector<size_t> by_out(r);
+    for (size_t a = 0; a < r; ++a) by_out[axes[a].out_pos] = a;
+    s = 1;
+    for (size_t d = r; d-- > 0;) {
+      out_str[by_out[d]] = s;
+      s *= (int64_t)axes[by_out[d]].dim;
+    }
+  }
+  std::vector<size_t> by_out(r);
+  for (size_t a = 0; a < r; ++a) by_out[axes[a].out_pos] = a;
+
+  auto generic = [&]() {
+    p.kind = 2;
+    PermGeneric& g = p.gen;
+    if (r + 1 > (size_t)MAXR) throw Error("permute rank exceeds the generic kernel limit");
+    g.rank = 0;
+    if (batch > 1) {
+      g.dims[g.rank] = (uint32_t)batch;
+      g.in_str[g.rank] = (int64_t)prod(shape0);
+      g.rank++;
+    }
+    for (size_t d = 0; d < r; ++d) {
+      g.dims[g.rank] = (uint32_t)axes[by_out[d]].dim;
+      g.in_str[g.rank] = in_str[by_out[d]];
+      g.rank++;
+    }
+    g.numel = p.numel;
+    p.threads = 256;
+  };
+
+  for (size_t tmin : {64, 32, 16, 8}) {
+    // G_in: innermost input axes with product >= tmin; G_out likewise in output order.
+    std::vector<bool> in_gin(r, false), in_gout(r, false);
+    size_t prodin = 1, prodout = 1;
+    for (size_t a = r; a-- > 0 && prodin < tmin;) in_gin[a] = true, prodin *= axes[a].dim;
+    for (size_t d = r; d-- > 0 && prodout < tmin;) in_gout[by_out[d]] = true, prodout *= axes[by_out[d]].dim;
+    size_t u = 1;
+    int nu = 0;
+    for (size_t a = 0; a < r; ++a)
+      if (in_gin[a] || in_gout[a]) u *= axes[a].dim, ++nu;
+    size_t lo_in = prodin, lo_out = prodout;
+    size_t n_hi_in = u / lo_in, n_hi_out = u / lo_out;
+    size_t tab = (n_hi_in + n_hi_out) * 8 + (n_hi_in + lo_in) * 4;
+    size_t lds = ((tab + 15) & ~(size_t)15) + u * (size_t)es;
+    int n_gin = 0, n_gout = 0;
+    for (size_t a = 0; a < r; ++a) n_gin += in_gin[a], n_gout += in_gout[a];
+    if (lds > 64 * 1024 || nu - n_gin > MAXT || n_gin > MAXT || nu - n_gout > MAXT || r - nu + 1 > (size_t)MAXR) continue;
+    PermTile& t = p.tile;
+    // LDS strides: row-major over U in output order.
+    std::vector<int64_t> lds_str(r, 0);
+    {
+      int64_t s = 1;
+      for (size_t d = r; d-- > 0;) {
+        size_t a = by_out[d];
+        if (in_gin[a] || in_gout[a]) lds_str[a] = s, s *= (int64_t)axes[a].dim;
+      }
+    }
+    t.n_hin = t.n_gin = t.n_hout = 0;
+    for (size_t a = 0; a < r; ++a) {  // input order
+      if (in_gin[a]) {
+        t.gin_dims[t.n_gin] = (uint32_t)axes[a].dim;
+        t.gin_lds[t.n_gin] = (int32_t)lds_str[a];
+        t.n_gin++;
+      } else if (in_gout[a]) {
+        t.hin_dims[t.n_hin] = (uint32_t)axes[a].dim;
+        t.hin_in[t.n_hin] = in_str[a];
+        t.hin_lds[t.n_hin] = (int32_t)lds_str[a];
+        t.n_hin++;
+      }
+    }
+    for (size_t d = 0; d < r; ++d) {  // output order
+      size_t a = by_out[d];
+      if ((in_gin[a] || in_gout[a]) && !in_gout[a]) {
+        t.hout_dims[t.n_hout] = (uint32_t)axes[a].dim;
+        t.hout_out[t.n_hout] = out_str[a];
+        t.n_hout++;
+      }
+    }
+    // Grid axes: batch (outermost) then the remaining axes in output order.
+    t.n_r = 0;
+    if (batch > 1) {
+      t.r_dims[t.n_r] = (uint32_t)batch;
+      t.r_in[t.n_r] = (int64_t)prod(shape0);
+      t.r_out[t.n_r] = (int64_t)prod(shape0);
+      t.n_r++;
+    }
+    for (size_t d = 0; d < r; ++d) {
+      size_t a = by_out[d];
+      if (in_gin[a] || in_gout[a]) continue;
+      t.r_dims[t.n_r] = (uint32_t)axes[a].dim;
+      t.r_in[t.n_r] = in_str[a];
+      t.r_out[t.n_r] = out_str[a];
+      t.n_r++;
+    }
+    t.u_size = (uint32_t)u;
+    t.lo_in = (uint32_t)lo_in;
+    t.lo_out = (uint32_t)lo_out;
+    t.n_hi_in = (uint32_t)n_hi_in;
+    t.n_hi_out = (uint32_t)n_hi_out;
+    t.n_tiles = p.numel / u;
+    p.kind = 1;
+    p.lds = lds;
+    p.threads = (unsigned)std::min<size_t>(256, std::max<size_t>(64, (u + 63) / 64 * 64));
+    return p;
+  }
+  generic();
+  return p;
+}
+
+void launch_permute(const PermPlan& p, const void* in, void* out, hipStream_t s) {
+  if (p.kind == 0) {
+    if (in != out) ATN_HIP(hipMemcpyAsync(out, in, p.numel * p.es, hipMemcpyDeviceToDevice, s));
+    return;
+  }
+  dispatch_es(p.es, [&](auto esc) {
+    constexpr int ES = decltype(esc)::value;
+    using E = typename Bits<ES>::T;
+    if (p.kind == 1) {
+      unsigned grid = (unsigned)std::min<uint64_t>(p.tile.n_tiles, 2048u * 4u);
+      LAUNCH(permute_tile_kernel<ES>, grid, p.threads, p.lds, s, p.tile, (const E*)in, (E*)out);
+    } else {
+      LAUNCH(permute_generic_kernel<ES>, grid_for(p.gen.numel, 256), 256, 0, s, p.gen, (const E*)in, (E*)out);
+    }
+  });
+}
+
+// ------------------------------------------------------------- sum / trace ---
+template <class T>
+__global__ void sum_axis_exact_kernel(const T* __restrict__ in, T* __restrict__ out, uint64_t outer, uint64_t d,
+                                      uint64_t inner);
+
+void launch_sum_axis(int dt, size_t outer, size_t d, size_t inner, const void* in, void* out, hipStream_t s) {
+  dispatch_dt(dt, [&](auto tag) {
+    using T = decltype(tag);
+    LAUNCH(sum_axis_exact_kernel<T>, grid_for(outer * inner, 256), 256, 0, s, (const T*)in, (T*)out, (uint64_t)outer,
+           (uint64_t)d, (uint64_t)inner);
+  });
+}
+
+// The reference starts from zero and adds every term (acc = 0; acc += x);
+// starting from the first term is identical for IEEE addition since 0 + x == x
+// for every x except -0.0, where 0 + (-0.0) = +0.0. Keep the exact form.
+template <class T>
+__device__ inline T zero_of();
+template <> __device__ inline float zero_of<float>() { return 0.f; }
+template <> __device__ inline double zero_of<double>() { return 0.0; }
+template <> __device__ inline float2 zero_of<float2>() { return make_float2(0.f, 0.f); }
+template <> __device__ inline double2 zero_of<double2>() { return make_double2(0.0, 0.0); }
+
+template <class T>
+__global__ void sum_axis_exact_kernel(const T* __restrict__ in, T* __restrict__ out, uint64_t outer, uint64_t d,
+                                      uint64_t inner) {
+  uint64_t n = outer * inner;
+  for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (uint64_t)gridDim.x * blockDim.x) {
+    uint64_t o = e / inner, q = e - o * inner;
+    const T* p = in + o * d * inner + q;
+    T acc = zero_of<T>();
+    for (uint64_t i = 0; i < d; ++i) acc = Arith<T>::add(acc, p[i * inner]);
+    out[e] = acc;
+  }
+}
+
+template <class T>
+__global__ void trace_kernel(const T* __restrict__ in, T* __restrict__ out, uint64_t A, uint64_t d, uint64_t Bm, uint64_t C) {
+  uint64_t n = A * Bm * C;
+  for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (uint64_t)gridDim.x * blockDim.x) {
+    uint64_t c = e % C, t = e / C, b = t % Bm, a = t / Bm;
+    uint64_t base = ((a * d * Bm + b) * d) * C + c;
+    uint64_t step = (Bm * d + 1) * C;
+    T acc = zero_of<T>();
+    for (uint64_t x = 0; x < d; ++x) acc = Arith<T>::add(acc, in[base + x * step]);
+    out[e] = acc;
+  }
+}
+
+template <int ES>
+__global__ void diag_kernel(const typename Bits<ES>::T* __restrict__ in, typename Bits<ES>::T* __restrict__ out, uint64_t A,
+                            uint64_t d, uint64_t Bm, uint64_t C) {
+  uint64_t n = A * d * Bm * C;
+  for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (uint64_t)gridDim.x * blockDim.x) {
+    uint64_t c = e % C, t = e / C, b = t % Bm;
+    t /= Bm;
+    uint64_t x = t % d, a = t / d;
+    out[e] = in[(((a * d + x) * Bm + b) * d + x) * C + c];
+  }
+}
+
+void launch_trace(int dt, size_t A, size_t d, size_t Bm, size_t C, const void* in, void* out, hipStream_t s) {
+  dispatch_dt(dt, [&](auto tag) {
+    using T = decltype(tag);
+    LAUNCH(trace_kernel<T>, grid_for(A * Bm * C, 256), 256, 0, s, (const T*)in, (T*)out, (uint64_t)A, (uint64_t)d,
+           (uint64_t)Bm, (uint64_t)C);
+  });
+}
+void launch_diag(int es, size_t A, size_t d, size_t Bm, size_t C, const void* in, void* out, hipStream_t s) {
+  dispatch_es(es, [&](auto esc) {
+    constexpr int ES = decltype(esc)::value;
+    using E = typename Bits<ES>::T;
+    LAUNCH(diag_kernel<ES>, grid_for(A * d * Bm * C, 256), 256, 0, s, (const E*)in, (E*)out, (uint64_t)A, (uint64_t)d,
+           (uint64_t)Bm, (uint64_t)C);
+  });
+}
+
+// ------------------------------------------------------------------ gather ---
+template <int ES>
+__global__ void gather_kernel(GatherArgs a, uint64_t g0, uint64_t nb, const typename Bits<ES>::T* __restrict__ in,
+                              typename Bits<ES>::T* __restrict__ out) {
+  uint64_t n = nb * a.inner;
+  for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (uint64_t)gridDim.x * blockDim.x) {
+    uint64_t b = e / a.inner, r = e - b * a.inner, g = g0 + b;
+    int64_t off = 0;
+    for (int q = 0; q < a.n_sel; ++q) off += (int64_t)((g / a.sel_div[q]) % a.sel_mod[q]) * a.sel_str[q];
+    for (int d = a.n_ax - 1; d >= 0; --d) {
+      uint64_t dim = a.dims[d], q = r / dim;
+      off += (int64_t)(r - q * dim) * a.str[d];
+      r = q;
+    }
+    out[e] = in[off];
+  }
+}
+
+void launch_gather(int es, const GatherArgs& a, uint64_t g0, size_t nb, const void* in, void* out, hipStream_t s) {
+  dispatch_es(es, [&](auto esc) {
+    constexpr int ES = decltype(esc)::value;
+    using E = typename Bits<ES>::T;
+    LAUNCH(gather_kernel<ES>, grid_for(nb * a.inner, 256), 256, 0, s, a, g0, (uint64_t)nb, (const E*)in, (E*)out);
+  });
+}
+
+// --------------------------------------------------------------- reductions --
+template <class T> struct Wide;
+template <> struct Wide<float> { using W = double; static __device__ W up(float x) { return x; } static __device__ float down(double x) { return (float)x; } };
+template <> struct Wide<double> { using W = double; static __device__ W up(double x) { return x; } static __device__ double down(double x) { return x; } };
+template <> struct Wide<float2> { using W = double2; static __device__ W up(float2 x) { return make_double2(x.x, x.y); } static __device__ float2 down(double2 x) { return make_float2((float)x.x, (float)x.y); } };
+template <> struct Wide<double2> { using W = double2; static __device__ W up(double2 x) { return x; } static __device__ double2 down(double2 x) { return x; } };
+
+// Deterministic: every accumulator element adds the batch's slices of its
+// block in slice order.
+template <class T>
+__global__ void reduce_f64_kernel(const T* __restrict__ part, typename Wide<T>::W* __restrict__ acc, uint64_t g0, uint64_t nb,
+                                  uint64_t n_int, uint64_t numel) {
+  using W = typename Wide<T>::W;
+  uint64_t blk0 = g0 / n_int, blk1 = (g0 + nb - 1) / n_int;
+  uint64_t n = (blk1 - blk0 + 1) * numel;
+  for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (uint64_t)gridDim.x * blockDim.x) {
+    uint64_t bl = blk0 + t / numel, e = t % numel;
+    uint64_t lo = bl * n_int > g0 ? bl * n_int : g0, hi = (bl + 1) * n_int < g0 + nb ? (bl + 1) * n_int : g0 + nb;
+    W a = acc[bl * numel + e];
+    for (uint64_t g = lo; g < hi; ++g) a = Arith<W>::add(a, Wide<T>::up(part[(g - g0) * numel + e]));
+    acc[bl * numel + e] = a;
+  }
+}
+
+void launch_reduce_f64(int dt, const void* part, void* acc64, uint64_t g0, size_t nb, size_t n_internal, size_t numel,
+                       hipStream_t s) {
+  dispatch_dt(dt, [&](auto tag) {
+    using T = decltype(tag);
+    uint64_t blocks = (g0 + nb - 1) / n_internal - g0 / n_internal + 1;
+    LAUNCH(reduce_f64_kernel<T>, grid_for(blocks * numel, 256), 256, 0, s, (const T*)part, (typename Wide<T>::W*)acc64, g0,
+           (uint64_t)nb, (uint64_t)n_internal, (uint64_t)numel);
+  });
+}
+
+template <class T, class S>
+__global__ void scatter_kernel(ScatterArgs a, uint64_t blk0, uint64_t n_blk, uint64_t numel, const S* __restrict__ src,
+                               T* __restrict__ out) {
+  uint64_t n = n_blk * numel;
+  for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (uint64_t)gridDim.x * blockDim.x) {
+    uint64_t bl = blk0 + t / numel, e = t % numel;
+    int64_t off = 0;
+    for (int q = 0; q < a.n_blk; ++q) off += (int64_t)((bl / a.blk_div[q]) % a.blk_mod[q]) * a.blk_str[q];
+    uint64_t r = e;
+    for (int d = a.n_kept - 1; d >= 0; --d) {
+      uint64_t dim = a.kept_dims[d], q = r / dim;
+      off += (int64_t)(r - q * dim) * a.kept_str[d];
+      r = q;
+    }
+    if constexpr (std::is_same<S, T>::value) out[off] = src[t];
+    else out[off] = Wide<T>::down(src[t]);
+  }
+}
+
+void launch_scatter(int dt, bool src_f64, const ScatterArgs& a, uint64_t blk0, size_t n_blk, size_t numel, const void* src,
+                    void* out, hipStream_t s) {
+  dispatch_dt(dt, [&](auto tag) {
+    using T = decltype(tag);
+    using W = typename Wide<T>::W;
+    if (src_f64)
+      LAUNCH((scatter_kernel<T, W>), grid_for(n_blk * numel, 256), 256, 0, s, a, blk0, (uint64_t)n_blk, (uint64_t)numel,
+             (const W*)src, (T*)out);
+    else
+      LAUNCH((scatter_kernel<T, T>), grid_for(n_blk * numel, 256), 256, 0, s, a, blk0, (uint64_t)n_blk, (uint64_t)numel,
+             (const T*)src, (T*)out);
+  });
+}
+
+template <class T>
+__global__ void accum_kernel(const T* __restrict__ part, T* __restrict__ acc, uint64_t n, int first) {
+  for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (uint64_t)gridDim.x * blockDim.x)
+    acc[e] = first ? part[e] : Arith<T>::add(acc[e], part[e]);
+}
+
+void launch_accum(int dt, bool first, const void* part, void* acc, size_t numel, hipStream_t s) {
+  dispatch_dt(dt, [&](auto tag) {
+    using T = decltype(tag);
+    LAUNCH(accum_kernel<T>, grid_for(numel, 256), 256, 0, s, (const T*)part, (T*)acc, (uint64_t)numel, first ? 1 : 0);
+  });
+}
+
+__global__ void fill_zero_kernel(uint32_t* p, uint64_t n) {
+  for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (uint64_t)gridDim.x * blockDim.x) p[e] = 0u;
+}
+
+void launch_fill_zero(void* p, size_t bytes, hipStream_t s) {
+  if (bytes % 4) throw Error("fill_zero needs a multiple of 4 bytes");
+  LAUNCH(fill_zero_kernel, grid_for(bytes / 4, 256), 256, 0, s, (uint32_t*)p, (uint64_t)(bytes / 4));
+}
+
+// -------------------------------------------------------------------- strip --
+// Power-of-two normalisation: multiplying by 2^-e is exact, so the mantissa
+// times 2^exponent reproduces the unstripped value bit for bit (barring
+// underflow); the reference divides by the exact max magnitude instead.
+template <class T>
+__global__ void maxabs_kernel(const T* __restrict__ x, uint64_t n, unsigned long long* maxbits) {
+  double m = 0.0;
+  for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (uint64_t)gridDim.x * blockDim.x) {
+    double v = Arith<T>::mag(x[e]);
+    m = v > m ? v : m;  // NaN never wins; a NaN result stays unscaled
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    double t = __shfl_down(m, o, 64);
+    m = t > m ? t : m;
+  }
+  if ((threadIdx.x & 63) == 0 && m > 0.0) atomicMax(maxbits, (unsigned long long)__double_as_longlong(m));
+}
+
+template <class T>
+__global__ void scale_pow2_kernel(T* __restrict__ x, uint64_t n, const unsigned long long* maxbits, long long* expo);
+template <>
+__global__ void scale_pow2_kernel<float2>(float2* __restrict__ x, uint64_t n, const unsigned long long* maxbits, long long* expo) {
+  double m = __longlong_as_double((long long)*maxbits);
+  if (!(m > 0.0) || !isfinite(m)) return;
+  int e;
+  frexp(m, &e);
+  if (blockIdx.x == 0 && threadIdx.x == 0) *expo += e;
+  float s = (float)ldexp(1.0, -e);
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+    x[i] = make_float2(x[i].x * s, x[i].y * s);
+}
+template <>
+__global__ void scale_pow2_kernel<double2>(double2* __restrict__ x, uint64_t n, const unsigned long long* maxbits, long long* expo) {
+  double m = __longlong_as_double((long long)*maxbits);
+  if (!(m > 0.0) || !isfinite(m)) return;
+  int e;
+  frexp(m, &e);
+  if (blockIdx.x == 0 && threadIdx.x == 0) *expo += e;
+  double s = ldexp(1.0, -e);
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+    x[i] = make_double2(x[i].x * s, x[i].y * s);
+}
+
+template <>
+__global__ void scale_pow2_kernel<float>(float* __restrict__ x, uint64_t n, const unsigned long long* maxbits, long long* expo) {
+  double m = __longlong_as_double((long long)*maxbits);
+  if (!(m > 0.0) || !isfinite(m)) return;
+  int e;
+  frexp(m, &e);
+  if (blockIdx.x == 0 && threadIdx.x == 0) *expo += e;
+  float s = (float)ldexp(1.0, -e);
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) x[i] *= s;
+}
+template <>
+__global__ void scale_pow2_kernel<double>(double* __restrict__ x, uint64_t n, const unsigned long long* maxbits, long long* expo) {
+  double m = __longlong_as_double((long long)*maxbits);
+  if (!(m > 0.0) || !isfinite(m)) return;
+  int e;
+  frexp(m, &e);
+  if (blockIdx.x == 0 && threadIdx.x == 0) *expo += e;
+  double s = ldexp(1.0, -e);
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) x[i] *= s;
+}
+
+void launch_maxabs(int dt, const void* x, size_t numel, uint64_t* maxbits, hipStream_t s) {
+  dispatch_dt(dt, [&](auto tag) {
+    using T = decltype(tag);
+    LAUNCH(maxabs_kernel<T>, grid_for(numel, 256, 2048), 256, 0, s, (const T*)x, (uint64_t)numel, (unsigned long long*)maxbits);
+  });
+}
+void launch_scale_pow2(int dt, void* x, size_t numel, const uint64_t* maxbits, int64_t* expo, hipStream_t s) {
+  dispatch_dt(dt, [&](auto tag) {
+    using T = decltype(tag);
+    LAUNCH(scale_pow2_kernel<T>, grid_for(numel, 256), 256, 0, s, (T*)x, (uint64_t)numel, (const unsigned long long*)maxbits,
+           (long long*)expo);
+  });
+}
+
+// ------------------------------------------------------------ copies/adjoint --
+template <int ES>
+__global__ void copy_kernel(const typename Bits<ES>::T* __restrict__ in, typename Bits<ES>::T* __restrict__ out, uint64_t n) {
+  for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (uint64_t)gridDim.x * blockDim.x) out[e] = in[e];
+}
+
+void launch_copy(size_t bytes, const void* in, void* out, hipStream_t s) {
+  if (bytes % 4) throw Error("copy needs a multiple of 4 bytes");
+  LAUNCH(copy_kernel<4>, grid_for(bytes / 4, 256), 256, 0, s, (const uint32_t*)in, (uint32_t*)out, (uint64_t)(bytes / 4));
+}
+
+template <int ES>
+__global__ void broadcast_axis_kernel(const typename Bits<ES>::T* __restrict__ in, typename Bits<ES>::T* __restrict__ out,
+                                      uint64_t outer, uint64_t d, uint64_t inner) {
+  uint64_t n = outer * d * inner;
+  for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (uint64_t)gridDim.x * blockDim.x) {
+    uint64_t q = e % inner, o = e / (d * inner);
+    out[e] = in[o * inner + q];
+  }
+}
+
+void launch_broadcast_axis(int es, size_t outer, size_t d, size_t inner, const void* in, void* out, hipStream_t s) {
+  dispatch_es(es, [&](auto esc) {
+    constexpr int ES = decltype(esc)::value;
+    using E = typename Bits<ES>::T;
+    LAUNCH(broadcast_axis_kernel<ES>, grid_for(outer * d * inner, 256), 256, 0, s, (const E*)in, (E*)out, (uint64_t)outer,
+           (uint64_t)d, (uint64_t)inner);
+  });
+}
+
+template <int ES>
+__global__ void diag_scatter_kernel(const typename Bits<ES>::T* __restrict__ g, typename Bits<ES>::T* __restrict__ out, int trace,
+                                    uint64_t A, uint64_t d, uint64_t Bm, uint64_t C) {
+  uint64_t n = A * d * Bm * C;
+  for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (uint64_t)gridDim.x * blockDim.x) {
+    uint64_t c = e % C, t = e / C, b = t % Bm;
+    t /= Bm;
+    uint64_t x = t % d, a = t / d;
+    uint64_t src = trace ? (a * Bm + b) * C + c : e;
+    out[(((a * d + x) * Bm + b) * d + x) * C + c] = g[src];
+  }
+}
+
+void launch_diag_scatter(int es, bool trace, size_t A, size_t d, size_t Bm, size_t C, const void* g, void* out, hipStream_t s) {
+  dispatch_es(es, [&](auto esc) {
+    constexpr int ES = decltype(esc)::value;
+    using E = typename Bits<ES>::T;
+    LAUNCH(diag_scatter_kernel<ES>, grid_for(A * d * Bm * C, 256), 256, 0, s, (const E*)g, (E*)out, trace ? 1 : 0, (uint64_t)A,
+           (uint64_t)d, (uint64_t)Bm, (uint64_t)C);
+  });
+}
+
+}  // namespace atn

@@ -1,0 +1,260 @@
+"""Synthetic tensor networks for the executor's workloads (seeded, deterministic).
+
+Every generator returns ``(net, arrays)`` where ``net`` is a dict
+``{"inputs", "output", "size_dict"}`` with integer legs (the layout of
+src/network.rs:135-143) and ``arrays`` is a list of NumPy arrays in
+``net["inputs"]`` order, row-major, axis ``i`` of input ``t`` carrying leg
+``net["inputs"][t][i]``.
+
+* ``random_circuit_amplitude``: grid random-circuit amplitude networks
+  (Haar single-qubit layer + fSim(pi/2, pi/6) on one of four alternating
+  nearest-neighbour edge patterns per cycle), kets |0>, bra a seeded
+  bitstring, optionally some qubits left open as output legs.
+* ``brickwork_expectation``: <psi| Z_0 Z_1 |psi> of a 1D RY.RZ + CZ
+  brickwork circuit as a doubled bra-ket network, parameterised by angles.
+* ``random_regular``: configuration-model d-regular graph networks without
+  self-loops (the rule of src/network.rs:432-458), per-edge dims.
+* ``random_connected``: small random connected networks with open legs (the
+  shape of src/network.rs:492-540), for parity tests.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+
+
+def haar_unitary_2x2(rng):
+    z = (rng.standard_normal((2, 2)) + 1j * rng.standard_normal((2, 2))) / np.sqrt(2.0)
+    q, r = np.linalg.qr(z)
+    d = np.diagonal(r)
+    return q * (d / np.abs(d))[None, :]
+
+
+def fsim(theta, phi):
+    c, s = np.cos(theta), np.sin(theta)
+    return np.array(
+        [[1, 0, 0, 0], [0, c, -1j * s, 0], [0, -1j * s, c, 0], [0, 0, 0, np.exp(-1j * phi)]],
+        dtype=np.complex128,
+    )
+
+
+def grid_edge_pattern(rows, cols, pattern):
+    """Edges of pattern 0..3: horizontal even/odd columns, vertical even/odd rows."""
+    edges = []
+    q = lambda r, c: r * cols + c
+    if pattern in (0, 1):
+        for r in range(rows):
+            for c in range(pattern, cols - 1, 2):
+                edges.append((q(r, c), q(r, c + 1)))
+    else:
+        p = pattern - 2
+        for r in range(p, rows - 1, 2):
+            for c in range(cols):
+                edges.append((q(r, c), q(r + 1, c)))
+    return edges
+
+
+class _Builder:
+    def __init__(self):
+        self.inputs, self.arrays, self.size_dict = [], [], {}
+        self.next_leg = 0
+
+    def leg(self, dim=2):
+        l = self.next_leg
+        self.next_leg += 1
+        self.size_dict[l] = dim
+        return l
+
+    def add(self, legs, array):
+        assert tuple(array.shape) == tuple(self.size_dict[l] for l in legs)
+        self.inputs.append(list(legs))
+        self.arrays.append(array)
+
+
+def random_circuit_amplitude(rows, cols, cycles, seed, open_qubits=(), dtype=np.complex64):
+    """Random-circuit amplitude network <x| C |0...0> on a rows x cols grid.
+
+    Gate tensors are stored as U[out, in] (single qubit) and
+    G[out_a, out_b, in_a, in_b] (two qubit). Qubits in ``open_qubits`` keep
+    their final leg open; output legs are ordered by qubit index.
+    Returns ``(net, arrays, bitstring)``.
+    """
+    rng = np.random.default_rng(seed)
+    bits = np.random.default_rng([seed, 1]).integers(0, 2, size=rows * cols)
+    n = rows * cols
+    b = _Builder()
+    cur = []
+    ket0 = np.array([1.0, 0.0], dtype=np.complex128)
+    for _ in range(n):
+        l = b.leg()
+        b.add([l], ket0)
+        cur.append(l)
+    g2 = fsim(np.pi / 2, np.pi / 6).reshape(2, 2, 2, 2)
+    for cyc in range(cycles):
+        for qb in range(n):
+            u = haar_unitary_2x2(rng)
+            nl = b.leg()
+            b.add([nl, cur[qb]], u)
+            cur[qb] = nl
+        for qa, qb in grid_edge_pattern(rows, cols, cyc % 4):
+            na, nb = b.leg(), b.leg()
+            b.add([na, nb, cur[qa], cur[qb]], g2)
+            cur[qa], cur[qb] = na, nb
+    output = []
+    open_set = set(open_qubits)
+    for qb in range(n):
+        if qb in open_set:
+            output.append(cur[qb])
+        else:
+            e = np.zeros(2, dtype=np.complex128)
+            e[bits[qb]] = 1.0
+            b.add([cur[qb]], e)
+    net = {"inputs": b.inputs, "output": output, "size_dict": b.size_dict}
+    arrays = [np.ascontiguousarray(a.astype(dtype)) for a in b.arrays]
+    return net, arrays, bits
+
+
+def ry(theta):
+    c, s = np.cos(theta / 2), np.sin(theta / 2)
+    return np.array([[c, -s], [s, c]], dtype=np.complex128)
+
+
+def rz(phi):
+    return np.array([[np.exp(-0.5j * phi), 0], [0, np.exp(0.5j * phi)]], dtype=np.complex128)
+
+
+def brickwork_structure(n_qubits, layers):
+    """Doubled bra-ket network structure for <psi|Z_0 Z_1|psi>.
+
+    Returns ``(net, gate_slots)`` where ``gate_slots`` describes every input:
+    ``("ket0",)``, ``("u", layer, qubit, conj)``, ``("cz", conj)``, ``("z",)``,
+    ``("id",)``. The ket half uses U[out,in]; the bra half uses conj(U) with
+    the same leg roles, and the two halves meet through Z (qubits 0, 1) or an
+    identity on every other qubit.
+    """
+    b = _Builder()
+    slots = []
+    dummy = lambda legs: np.zeros([2] * len(legs), dtype=np.complex128)
+    ends = []
+    for conj in (False, True):
+        cur = []
+        for _ in range(n_qubits):
+            l = b.leg()
+            b.add([l], dummy([l]))
+            slots.append(("ket0",))
+            cur.append(l)
+        for layer in range(layers):
+            for q in range(n_qubits):
+                nl = b.leg()
+                b.add([nl, cur[q]], dummy([0, 0]))
+                slots.append(("u", layer, q, conj))
+                cur[q] = nl
+            for q in range(layer % 2, n_qubits - 1, 2):
+                na, nb = b.leg(), b.leg()
+                b.add([na, nb, cur[q], cur[q + 1]], dummy([0, 0, 0, 0]))
+                slots.append(("cz", conj))
+                cur[q], cur[q + 1] = na, nb
+        ends.append(cur)
+    for q in range(n_qubits):
+        b.add([ends[1][q], ends[0][q]], dummy([0, 0]))
+        slots.append(("z",) if q < 2 else ("id",))
+    net = {"inputs": b.inputs, "output": [], "size_dict": b.size_dict}
+    return net, slots
+
+
+def brickwork_angles(n_qubits, layers, seed):
+    rng = np.random.default_rng(seed)
+    return rng.uniform(-np.pi, np.pi, size=(layers, n_qubits, 2))
+
+
+def brickwork_arrays(slots, angles, dtype=np.complex128):
+    """Input tensors of the doubled network for the given angles (NumPy)."""
+    cz = np.diag([1, 1, 1, -1]).astype(np.complex128).reshape(2, 2, 2, 2)
+    z = np.diag([1.0, -1.0]).astype(np.complex128)
+    out = []
+    for s in slots:
+        if s[0] == "ket0":
+            out.append(np.array([1.0, 0.0], dtype=np.complex128))
+        elif s[0] == "u":
+            _, layer, q, conj = s
+            u = rz(angles[layer, q, 1]) @ ry(angles[layer, q, 0])
+            out.append(np.conj(u) if conj else u)
+        elif s[0] == "cz":
+            out.append(cz)
+        elif s[0] == "z":
+            out.append(z)
+        else:
+            out.append(np.eye(2, dtype=np.complex128))
+    return [np.ascontiguousarray(a.astype(dtype)) for a in out]
+
+
+def random_regular(n, degree, dims, seed, dtype=np.float32):
+    """d-regular multigraph network without self-loops, per-edge dims from ``dims``.
+
+    Stub shuffling is resampled until no pair is a self-loop (the rule of
+    src/network.rs:432-444). Entries are N(0, 1).
+    """
+    assert (n * degree) % 2 == 0
+    rng = np.random.default_rng(seed)
+    stubs = np.repeat(np.arange(n), degree)
+    while True:
+        rng.shuffle(stubs)
+        pairs = stubs.reshape(-1, 2)
+        if np.all(pairs[:, 0] != pairs[:, 1]):
+            break
+    inputs = [[] for _ in range(n)]
+    size_dict = {}
+    for leg, (a, b) in enumerate(pairs.tolist()):
+        inputs[a].append(leg)
+        inputs[b].append(leg)
+        size_dict[leg] = int(dims[rng.integers(0, len(dims))])
+    net = {"inputs": inputs, "output": [], "size_dict": size_dict}
+    arrays = [
+        np.ascontiguousarray(rng.standard_normal([size_dict[l] for l in legs]).astype(dtype))
+        for legs in inputs
+    ]
+    return net, arrays
+
+
+def random_connected(n, extra, dims, n_open, seed):
+    """Small connected random network with open legs (shape of src/network.rs:492-540)."""
+    rng = np.random.default_rng(seed)
+    inputs = [[] for _ in range(n)]
+    size_dict = {}
+    nxt = [0]
+
+    def add_edge(a, b):
+        inputs[a].append(nxt[0])
+        inputs[b].append(nxt[0])
+        size_dict[nxt[0]] = int(dims[rng.integers(0, len(dims))])
+        nxt[0] += 1
+
+    for i in range(1, n):
+        add_edge(i, int(rng.integers(0, i)))
+    for _ in range(extra):
+        a = int(rng.integers(0, n))
+        b = int(rng.integers(0, n))
+        while b == a:
+            b = int(rng.integers(0, n))
+        add_edge(a, b)
+    output = []
+    for _ in range(n_open):
+        t = int(rng.integers(0, n))
+        inputs[t].append(nxt[0])
+        output.append(nxt[0])
+        size_dict[nxt[0]] = int(dims[rng.integers(0, len(dims))])
+        nxt[0] += 1
+    return {"inputs": inputs, "output": output, "size_dict": size_dict}
+
+
+def random_arrays(net, dtype, seed):
+    """Uniform [-1, 1) entries (real and imaginary parts), like src/tensor.rs:117-119, 233-238."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for legs in net["inputs"]:
+        shape = [net["size_dict"][l] for l in legs]
+        a = rng.uniform(-1.0, 1.0, size=shape)
+        if np.issubdtype(np.dtype(dtype), np.complexfloating):
+            a = a + 1j * rng.uniform(-1.0, 1.0, size=shape)
+        out.append(np.array(a, dtype=dtype, order="C"))
+    return out

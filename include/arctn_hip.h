@@ -1,0 +1,183 @@
+/*
+ * arctn_hip.h - C ABI of the MI355X (gfx950) executor for ArcTN contraction plans.
+ *
+ * The executor takes the same plan input as the reference's CPU executor
+ * (network legs/dims, SSA path, sliced legs) and runs it on one GPU with
+ * hand-written HIP kernels. All tensors handed across this ABI are DEVICE
+ * pointers, row-major (C order) in net.inputs leg order, complex values as
+ * interleaved (re, im) pairs - the layout of src/tensor.rs:340-344 and of
+ * NumPy complex64/complex128.
+ *
+ * Conventions (the reference engine-interface rule "must not unwind across the
+ * C boundary", docs/engine-interface.md):
+ *   - every entry point returns ATN_OK (0) or a negative status; the message of
+ *     the last failure on the calling thread is atn_last_error();
+ *   - all validation happens before any kernel is launched;
+ *   - strings returned through char** are released with atn_free();
+ *   - plans are immutable after creation and may be shared between threads;
+ *     executors are single-threaded objects (one per stream).
+ */
+#ifndef ARCTN_HIP_H
+#define ARCTN_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ATN_ABI_VERSION 1
+
+enum {
+  ATN_OK = 0,
+  ATN_ERR_INVALID = -1,  /* malformed network, path, slice set, shapes, options */
+  ATN_ERR_DEVICE = -2,   /* HIP runtime failure (allocation, launch, sync) */
+  ATN_ERR_INTERNAL = -3
+};
+
+/* Element types: the four dtypes of the reference (pybind/src/lib.rs:234-239). */
+typedef enum { ATN_F32 = 0, ATN_F64 = 1, ATN_C64 = 2, ATN_C128 = 3 } atn_dtype;
+
+/* Arithmetic of the contraction GEMMs.
+ *   NATIVE: FP32 MFMA for f32/complex64 (complex as 4M: a real GEMM against the
+ *           2x2 real expansion of B), FP64 MFMA for f64/complex128.
+ *   C3M:    complex 3M (Gauss) on FP32 MFMA; complex64 only.
+ *   BF16X3: complex64/f32 products as hi*hi + hi*lo + lo*hi in BF16 MFMA, FP32 accumulate.
+ *   FP16X2: same with FP16 hi/lo halves and per-step power-of-two operand rescaling. */
+typedef enum { ATN_PREC_NATIVE = 0, ATN_PREC_C3M = 1, ATN_PREC_BF16X3 = 2, ATN_PREC_FP16X2 = 3 } atn_precision;
+
+/* Network + plan, as flat arrays (the fields of the execution-plan v2 artifact,
+ * src/execution_plan.rs:39-43). Legs are u32 ids (src/network.rs:13). */
+typedef struct {
+  size_t n_inputs;
+  const size_t* input_lens;    /* n_inputs entries: rank of each input */
+  const uint32_t* input_legs;  /* concatenated legs, sum(input_lens) entries */
+  size_t n_output;
+  const uint32_t* output;      /* open legs in output order */
+  size_t n_dims;
+  const uint32_t* dim_legs;    /* size_dict keys */
+  const size_t* dim_sizes;     /* size_dict values */
+  size_t n_steps;
+  const size_t* ssa_path;      /* 2 * n_steps entries: (left, right) SSA ids */
+  size_t n_sliced;
+  const uint32_t* sliced_legs; /* may include output legs (output-index slicing) */
+} atn_network_desc;
+
+typedef struct atn_plan atn_plan;
+typedef struct atn_executor atn_executor;
+
+typedef struct {
+  size_t n_inputs;
+  size_t n_steps;          /* pair steps of the per-slice plan */
+  size_t n_slices;         /* product of all sliced dims (internal x output blocks) */
+  size_t n_blocks;         /* output-index blocks (1 without output slicing) */
+  size_t n_varying_steps;  /* steps whose operands depend on the slice index */
+  double log10_flops;      /* reference convention: sum over steps of prod(union dims), whole network */
+  double log2_peak_size;   /* live-set peak of the whole network (src/path.rs:396-399) */
+  double flops_per_slice;  /* real FLOPs of one slice: sum c*bt*m*k*n, c = 8 complex / 2 real */
+  double mults_per_slice;  /* scalar multiplications of one slice (reference convention) */
+} atn_plan_info;
+
+typedef struct {
+  int dtype;            /* atn_dtype */
+  int precision;        /* atn_precision */
+  int batched;          /* 0: plain per-slice path (one slice, one step at a time);
+                           1: all slices of a step in one strided-batched launch,
+                              slice-invariant steps hoisted, FP64 cross-slice sum */
+  int use_graph;        /* batched only: capture the whole plan in one hipGraph */
+  int autotune;         /* time candidate GEMM kernels per step on first use */
+  int strip;            /* unsliced plans: power-of-two normalisation after every
+                           step, exponent kept in FP64 (execute_stripped) */
+  size_t max_batch;     /* slices per batched launch; 0 = largest that fits workspace_limit */
+  size_t workspace_limit;  /* bytes; 0 = 60% of free device memory */
+} atn_exec_options;
+
+typedef struct {
+  double elapsed_ms;        /* device time of the execution (hipEvents on the stream) */
+  double flops;             /* algorithmic real FLOPs executed (all slices) */
+  double tflops;
+  double slices_per_s;
+  double log10_exponent;    /* strip: result = output * 10^log10_exponent */
+  double max_abs_result;
+  size_t n_slices;
+  size_t batch;             /* slices per batched launch actually used */
+  size_t n_launches;        /* kernel launches of one execution */
+  size_t workspace_bytes;
+} atn_report;
+
+int atn_abi_version(void);
+const char* atn_last_error(void);
+void atn_free(void* p);
+
+/* Replaces CompiledContraction::compile (src/compiled.rs:254-366) together with
+ * the slice bookkeeping of contract_network_sliced (src/slice.rs:960-1016,
+ * 1070-1137). Validates the network (src/network.rs:171-229), the path
+ * (src/path.rs:239-434) and the slice set (src/slice.rs:146-172). */
+int atn_plan_create(const atn_network_desc* desc, atn_plan** out);
+void atn_plan_destroy(atn_plan* plan);
+int atn_plan_stats(const atn_plan* plan, atn_plan_info* out);
+/* JSON mirror of input_preps()/pair_steps()/finalize() of the per-slice plan
+ * (src/compiled.rs:241-250, 102-153): slots, legs, shapes, sum axes, perms,
+ * (bt, m, k, n). */
+int atn_plan_describe(const atn_plan* plan, char** json_out);
+
+/* Executors own device workspace, per-step kernel choices and the captured
+ * graph; create one per (plan, options) and reuse it. */
+int atn_executor_create(const atn_plan* plan, const atn_exec_options* opts, atn_executor** out);
+void atn_executor_destroy(atn_executor* ex);
+/* Replaces contract_network_sliced (src/slice.rs:960-1065) and
+ * CompiledContraction::execute / execute_stripped (src/compiled.rs:369-381):
+ * d_inputs[i] is input i (full, unsliced shape), d_out receives the full output
+ * in net.output order. stream may be NULL (default stream). Returns after the
+ * work is enqueued and, when report != NULL, completed and timed. */
+int atn_executor_run(atn_executor* ex, const void* const* d_inputs, void* d_out, void* stream, atn_report* report);
+
+/* One-shot convenience: create an executor, run, destroy. */
+int atn_execute(const atn_plan* plan, const atn_exec_options* opts, const void* const* d_inputs, void* d_out,
+                void* stream, atn_report* report);
+
+/* Stage execution of the per-slice plan, the hook the reference exposes for
+ * autograd tapes (src/compiled.rs:7-10, 60-63, 143-153, 206-209):
+ *   atn_prep_forward:     input i (slot i) <- prep(input)
+ *   atn_step_forward:     slot n_inputs+s <- step s (left slot, right slot)
+ *   atn_finalize_forward: output <- finalize(slot fin)
+ * and their adjoints (no reference counterpart): given dL/d(conj result) they
+ * produce dL/d(conj operand) with conjugate-transposed GEMMs, inverse
+ * permutations and broadcasts over summed legs. Tensors have the per-slice
+ * shapes reported by atn_plan_describe. Scratch memory is stream-ordered. */
+int atn_prep_forward(const atn_plan* plan, size_t input, int dtype, const void* in, void* out, void* stream);
+int atn_prep_adjoint(const atn_plan* plan, size_t input, int dtype, const void* grad_out, void* grad_in, void* stream);
+int atn_step_forward(const atn_plan* plan, size_t step, int dtype, int precision, const void* left, const void* right,
+                     void* out, void* stream);
+int atn_step_adjoint(const atn_plan* plan, size_t step, int dtype, const void* left, const void* right,
+                     const void* grad_out, void* grad_left, void* grad_right, void* stream);
+int atn_finalize_forward(const atn_plan* plan, int dtype, const void* in, void* out, void* stream);
+int atn_finalize_adjoint(const atn_plan* plan, int dtype, const void* grad_out, void* grad_in, void* stream);
+
+/* Element-level kernels, exported for parity tests of the data-movement path
+ * (src/tensor.rs:453-669): op 0 permute(arg = axes), 1 sum_axis(arg[0]),
+ * 2 trace_pair(arg[0], arg[1]), 3 diag_pair(arg[0], arg[1]),
+ * 4 select_axis(arg[0], arg[1]). Device pointers; output shape implied. */
+int atn_tensor_op(int dtype, int op, size_t rank, const size_t* shape, const size_t* arg, size_t n_arg,
+                  const void* d_in, void* d_out, void* stream);
+
+/* Batched strided GEMM used by the executor, exported for kernel tests:
+ * C[b] = A[b] . B[b], row-major, batch strides in elements (0 = broadcast).
+ * kernel: -1 automatic choice, 0 one-thread-per-element kernel, 1 MFMA kernel. */
+int atn_gemm(int dtype, int precision, int kernel, size_t batch, size_t m, size_t k, size_t n, const void* d_a,
+             size_t stride_a, const void* d_b, size_t stride_b, void* d_c, size_t stride_c, void* stream);
+
+/* Grouped execution of many independent unsliced plans (uneven instances):
+ * every plan's steps run in lockstep rounds, each round one grouped launch per
+ * kernel kind. inputs[p] is the d_inputs array of plan p; outputs[p] its output.
+ * With strip != 0 every step result is renormalised by a power of two and
+ * log10_exponents[p] receives the exponent (result = output * 10^e). */
+int atn_grouped_execute(const atn_plan* const* plans, size_t n_plans, int dtype, int strip,
+                        const void* const* const* inputs, void* const* outputs, double* log10_exponents,
+                        void* stream, atn_report* report);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* ARCTN_HIP_H */

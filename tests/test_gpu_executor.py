@@ -1,0 +1,186 @@
+"""Executor parity on the GPU: plain and batched paths against the oracle,
+the reference's known answers and NumPy einsum."""
+import numpy as np
+import pytest
+
+from arctn_amd import CompiledContraction, Plan, contract_sliced
+from arctn_amd import planner as P
+from arctn_amd import workloads as W
+from oracle import pyoracle as O
+from tests.golden import fixtures as F
+
+pytestmark = pytest.mark.gpu
+
+DTYPES = [np.float32, np.float64, np.complex64, np.complex128]
+FX = F.load()
+MODES = [dict(), dict(batched=True), dict(batched=True, use_graph=True), dict(batched=True, max_batch=3)]
+
+
+def _vals(fx, dtype):
+    v = fx["values"] if "values" in fx else None
+    if np.issubdtype(np.dtype(dtype), np.complexfloating):
+        return np.array([complex(a, b) for a, b in v], dtype=dtype)
+    return np.array([a for a, _ in v], dtype=dtype)
+
+
+@pytest.mark.parametrize("name", ["mpi_matrix_product", "mpi_vector_dot"])
+@pytest.mark.parametrize("dtype", DTYPES)
+@pytest.mark.parametrize("mode", range(len(MODES)))
+def test_mpi_fixtures_exact(cuda, name, dtype, mode):
+    """tests/mpi_cli.rs:72-148: exact results under every slicing."""
+    fx = FX[name]
+    net = F.net_of(fx)
+    vals = _vals(fx, dtype)
+    half = len(vals) // 2
+    shape = [net["size_dict"][l] for l in net["inputs"][0]]
+    arrays = [vals[:half].reshape(shape), vals[half:].reshape(shape)]
+    cplx = np.issubdtype(np.dtype(dtype), np.complexfloating)
+    exp = np.array([complex(a, b) for a, b in fx["complex_expected"]] if cplx else fx["real_expected"], dtype=dtype)
+    for sl in fx["slicings"]:
+        got = contract_sliced(net["inputs"], net["output"], net["size_dict"], arrays, sl, ssa_path=fx["ssa_path"],
+                              **MODES[mode])
+        assert got.dtype == np.dtype(dtype)
+        assert np.array_equal(got.reshape(-1), exp), (sl, got)
+
+
+@pytest.mark.parametrize("mode", range(len(MODES)))
+def test_unary_output_blocks_known_answer(cuda, mode):
+    """tests/slice_output.rs:52-70."""
+    fx = FX["unary_output_blocks"]
+    net = F.net_of(fx)
+    a = np.array([[[100 * x + 10 * y + z for z in range(4)] for y in range(3)] for x in range(2)], dtype=np.float64)
+    for sl in fx["slicings"]:
+        got = contract_sliced(net["inputs"], net["output"], net["size_dict"], [a], sl, ssa_path=[], **MODES[mode])
+        assert got.shape == (4, 2)
+        assert np.array_equal(got.reshape(-1), np.array(fx["expected"])), sl
+
+
+def test_slice_validation_dot(cuda):
+    fx = FX["slice_validation_dot"]
+    net = F.net_of(fx)
+    a, b = np.array(fx["a"]), np.array(fx["b"])
+    for sl in fx["slicings"]:
+        got = contract_sliced(net["inputs"], net["output"], net["size_dict"], [a, b], sl, ssa_path=fx["ssa_path"])
+        assert got.reshape(()) == fx["expected"]
+
+
+def test_scalar32_matmul(cuda):
+    """tests/scalar32.rs:99-148 tolerances against naive einsum."""
+    fx = FX["scalar32_matmul"]
+    net = F.net_of(fx)
+    cc = CompiledContraction.compile(net["inputs"], net["output"], net["size_dict"], ssa_path=fx["ssa_path"])
+    ra, rb = np.array(fx["real_a"], np.float32).reshape(2, 3), np.array(fx["real_b"], np.float32).reshape(3, 2)
+    want = O.contract(net, [ra, rb], mode="naive")
+    assert np.abs(cc.execute([ra, rb]) - want).max() < 2e-6
+    ca = np.array([complex(*v) for v in fx["complex_a"]], np.complex64).reshape(2, 3)
+    cb = np.array([complex(*v) for v in fx["complex_b"]], np.complex64).reshape(3, 2)
+    want = O.contract(net, [ca, cb], mode="naive")
+    assert np.abs(cc.execute([ca, cb]) - want).max() < 1e-5
+
+
+@pytest.mark.parametrize("case", range(len(FX["output_slicing_cases"]["cases"])))
+@pytest.mark.parametrize("dtype", DTYPES)
+def test_output_slicing_cases_vs_einsum(cuda, case, dtype):
+    """pybind/tests/test_output_slicing.py:49-65, 113-143."""
+    name, eq, net, sliced, path = F.einsum_case(FX["output_slicing_cases"]["cases"][case])
+    arrays = F.case_arrays(net, dtype)
+    ref = np.einsum(eq, *arrays, optimize=False)
+    single = np.dtype(dtype) in (np.dtype(np.float32), np.dtype(np.complex64))
+    tol = 2e-4 if single else 1e-11
+    for mode in MODES:
+        got = contract_sliced(net["inputs"], net["output"], net["size_dict"], arrays, sliced, ssa_path=path, **mode)
+        assert got.shape == ref.shape and got.dtype == np.dtype(dtype)
+        np.testing.assert_allclose(got, ref, rtol=tol, atol=tol, err_msg=f"{name} {mode}")
+
+
+def _random_case(seed, dims=(2, 3, 4)):
+    rng = np.random.default_rng(seed)
+    n = int(rng.integers(3, 8))
+    net = W.random_connected(n, int(rng.integers(0, n)), list(dims), int(rng.integers(0, 3)), seed)
+    return net, P.greedy(net)
+
+
+@pytest.mark.parametrize("dtype", DTYPES)
+def test_random_networks_plain_and_batched_vs_oracle(cuda, dtype):
+    """Random connected networks (correctness.rs:43-103 style), every leg sliced in turn."""
+    single = np.dtype(dtype) in (np.dtype(np.float32), np.dtype(np.complex64))
+    tol = 1e-4 if single else 1e-10
+    for seed in range(12):
+        net, path = _random_case(seed)
+        arrays = W.random_arrays(net, dtype, seed)
+        want = O.contract(net, [a.astype(np.complex128) for a in arrays], path, "naive")
+        scale = max(1.0, np.abs(want).max())
+        got = CompiledContraction.compile(net["inputs"], net["output"], net["size_dict"], ssa_path=path).execute(arrays)
+        assert np.abs(got - want).max() / scale < tol, seed
+        legs = sorted(net["size_dict"])
+        for sl in ([legs[0]], legs[: min(3, len(legs))], legs[-2:]):
+            for mode in MODES:
+                got = contract_sliced(net["inputs"], net["output"], net["size_dict"], arrays, sl, ssa_path=path, **mode)
+                assert np.abs(got - want).max() / scale < tol, (seed, sl, mode)
+
+
+def test_mixed_network_traces_and_diagonals(cuda):
+    """compiled_pair_steps.rs:68-122: diagonal, trace, dangling, batch and hyperedge legs."""
+    fx = FX["mixed_network"]
+    net = F.net_of(fx)
+    for path in (P.greedy(net), [tuple(p) for p in fx["hand_path"]]):
+        for dtype in (np.float64, np.complex128):
+            arrays = W.random_arrays(net, dtype, 7)
+            want = O.contract(net, arrays, path, "naive")
+            got = CompiledContraction.compile(net["inputs"], net["output"], net["size_dict"], ssa_path=path).execute(arrays)
+            assert np.abs(got - want).max() < 1e-9 * max(1, np.abs(want).max())
+
+
+def test_plan_description_matches_oracle(cuda):
+    fx = FX["mixed_network"]
+    net = F.net_of(fx)
+    path = [tuple(p) for p in fx["hand_path"]]
+    mine = Plan(net["inputs"], net["output"], net["size_dict"], path).describe()
+    ref = O.describe(net, path)
+    for s_m, s_r in zip(mine["steps"], ref["steps"]):
+        s_m = dict(s_m)
+        s_m.pop("varying")
+        assert s_m == s_r
+    assert mine["finalize"] == ref["finalize"]
+
+
+def test_circuit_4x4_amplitude_vs_complex128_oracle(cuda):
+    """16-qubit 4x4 random circuit, 8 cycles: one amplitude, rel. err <= 1e-5."""
+    net, arrays, _ = W.random_circuit_amplitude(4, 4, 8, 0)
+    path = P.greedy(net)
+    want = O.contract(net, [a.astype(np.complex128) for a in arrays], path, "compiled")
+    cc = CompiledContraction.compile(net["inputs"], net["output"], net["size_dict"], ssa_path=path)
+    for mode in MODES[:2]:
+        got = cc.execute(arrays, **mode)
+        assert abs(complex(got) - complex(want)) / abs(complex(want)) < 1e-5
+
+
+def test_stripped_reconstructs_plain(cuda):
+    net, arrays, _ = W.random_circuit_amplitude(4, 4, 8, 0)
+    path = P.greedy(net)
+    cc = CompiledContraction.compile(net["inputs"], net["output"], net["size_dict"], ssa_path=path)
+    plain = complex(cc.execute(arrays))
+    mant, e = cc.execute_stripped(arrays)
+    assert abs(complex(mant) * 10.0 ** e - plain) / abs(plain) < 1e-5
+    fx = FX["strip_extreme"]
+    net = F.net_of(fx)
+    cc = CompiledContraction.compile(net["inputs"], net["output"], net["size_dict"], ssa_path=fx["ssa_path"])
+    m, e = cc.execute_stripped([np.array(1e200), np.array(1.0)])
+    assert e > 199 and abs(float(m) * 10.0 ** e - 1e200) / 1e200 < 1e-12
+    m, e = cc.execute_stripped([np.array(1e200 - 1e200j), np.array(1.0 + 0j)])
+    assert abs(complex(m) * 10.0 ** e - (1e200 - 1e200j)) / abs(1e200 - 1e200j) < 1e-12
+
+
+def test_sliced_circuit_plain_batched_agree(cuda):
+    """A sliced 4x4 circuit: plain (T sums in reference chunk order) vs batched (FP64 sums)."""
+    net, arrays, _ = W.random_circuit_amplitude(4, 4, 8, 0)
+    path = P.greedy(net)
+    sliced = P.find_slices_to_size(net, path, 2 ** 7)
+    assert sliced and len(sliced) >= 3
+    want = O.contract(net, [a.astype(np.complex128) for a in arrays], path, "compiled")
+    outs = []
+    for mode in MODES:
+        got = contract_sliced(net["inputs"], net["output"], net["size_dict"], arrays, sliced, ssa_path=path, **mode)
+        assert abs(complex(got) - complex(want)) / abs(complex(want)) < 1e-5, mode
+        outs.append(complex(got))
+    assert outs[1] == outs[2]  # graph replay == eager batched, bit for bit
