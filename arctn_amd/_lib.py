@@ -49,6 +49,11 @@ class Report(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class KernelStat(C.Structure):
+    _fields_ = [("name", C.c_char * 32), ("launches", C.c_size_t), ("ms", C.c_double), ("flops", C.c_double),
+                ("bytes", C.c_double)]
+
+
 # name -> (restype, argtypes)
 _VP = C.c_void_p
 _SIGS = {
@@ -62,6 +67,8 @@ _SIGS = {
     "atn_executor_create": (C.c_int, [_VP, C.POINTER(ExecOptions), C.POINTER(_VP)]),
     "atn_executor_destroy": (None, [_VP]),
     "atn_executor_run": (C.c_int, [_VP, C.POINTER(_VP), _VP, _VP, C.POINTER(Report)]),
+    "atn_executor_profile": (C.c_int, [_VP, C.POINTER(_VP), _VP, _VP, C.POINTER(KernelStat), C.c_size_t,
+                                       C.POINTER(C.c_size_t)]),
     "atn_execute": (C.c_int, [_VP, C.POINTER(ExecOptions), C.POINTER(_VP), _VP, _VP, C.POINTER(Report)]),
     "atn_prep_forward": (C.c_int, [_VP, C.c_size_t, C.c_int, _VP, _VP, _VP]),
     "atn_prep_adjoint": (C.c_int, [_VP, C.c_size_t, C.c_int, _VP, _VP, _VP]),

@@ -184,6 +184,22 @@ class Plan:
         return out, (rep.as_dict() if report else None)
 
 
+def profile(plan, dev_inputs, dtype_code, out=None, stream=None, **opts):
+    """Per-kernel-kind device time and algorithmic work of one eager execution."""
+    torch = _torch()
+    ex = plan.executor(dtype_code, **opts)
+    if out is None:
+        shape = [plan.size_dict[l] for l in plan.output]
+        out = torch.empty(shape, dtype=_torch_dtype(dtype_code), device=dev_inputs[0].device)
+    ptrs = _lib.pointer_array([t.data_ptr() for t in dev_inputs])
+    s = stream if stream is not None else torch.cuda.current_stream(dev_inputs[0].device)
+    stats = (_lib.KernelStat * 32)()
+    n = _lib.C.c_size_t(0)
+    _lib.check(_lib.lib().atn_executor_profile(ex.h, ptrs, out.data_ptr(), s.cuda_stream, stats, 32, _lib.C.byref(n)))
+    return {stats[i].name.decode(): {"launches": stats[i].launches, "ms": stats[i].ms, "flops": stats[i].flops,
+                                     "bytes": stats[i].bytes} for i in range(n.value)}
+
+
 def _finish(out, numpy_in):
     if numpy_in:
         return out.cpu().numpy()

@@ -152,6 +152,27 @@ int atn_executor_run(atn_executor* ex, const void* const* d_inputs, void* d_out,
   });
 }
 
+int atn_executor_profile(atn_executor* ex, const void* const* d_inputs, void* d_out, void* stream,
+                         atn_kernel_stat* stats, size_t max_stats, size_t* n_stats) {
+  return guard([&] {
+    if (!ex || !d_out || !stats || !n_stats) throw Error("NULL argument");
+    std::vector<KernelStat> ks;
+    ex->ex->profile(d_inputs, d_out, static_cast<hipStream_t>(stream), ks);
+    size_t n = 0;
+    for (size_t k = 0; k < ks.size() && n < max_stats; ++k) {
+      if (!ks[k].launches) continue;
+      atn_kernel_stat& o = stats[n++];
+      std::memset(o.name, 0, sizeof o.name);
+      std::strncpy(o.name, kernel_kind_name((int)k), sizeof o.name - 1);
+      o.launches = ks[k].launches;
+      o.ms = ks[k].ms;
+      o.flops = ks[k].flops;
+      o.bytes = ks[k].bytes;
+    }
+    *n_stats = n;
+  });
+}
+
 int atn_execute(const atn_plan* plan, const atn_exec_options* opts, const void* const* d_inputs, void* d_out, void* stream,
                 atn_report* report) {
   return guard([&] {

@@ -110,7 +110,8 @@ void Executor::gen_inputs(bool body, size_t nb, std::vector<Ref>& slots, std::ve
       ref = Ref{Ref::WS, newbuf(nbI * ga.inner * es)};
       Ref src{Ref::INPUT, (int)i};
       use(ref);
-      emit([=](const Ctx& c) { launch_gather(es, ga, c.g0, nbI, c.ptr(src), c.ptr(ref), c.s); });
+      emit([=](const Ctx& c) { launch_gather(es, ga, c.g0, nbI, c.ptr(src), c.ptr(ref), c.s); }, 0,
+           2.0 * nbI * ga.inner * es);
     } else {
       ref = Ref{Ref::INPUT, (int)i};
     }
@@ -126,10 +127,11 @@ void Executor::gen_inputs(bool body, size_t nb, std::vector<Ref>& slots, std::ve
       Ref src = ref;
       use(src);
       use(dst);
+      double by = (double)(A * d * Bm * C + out_numel) * es;
       if (diag)
-        emit([=](const Ctx& c) { launch_diag(es, A, d, Bm, C, c.ptr(src), c.ptr(dst), c.s); });
+        emit([=](const Ctx& c) { launch_diag(es, A, d, Bm, C, c.ptr(src), c.ptr(dst), c.s); }, 0, by);
       else
-        emit([=](const Ctx& c) { launch_trace(dt, A, d, Bm, C, c.ptr(src), c.ptr(dst), c.s); });
+        emit([=](const Ctx& c) { launch_trace(dt, A, d, Bm, C, c.ptr(src), c.ptr(dst), c.s); }, 0, by);
       shape.erase(shape.begin() + aj);
       if (!diag) shape.erase(shape.begin() + ai);
       ref = dst;
@@ -162,11 +164,17 @@ void Executor::gen_gemm(const PairStep& st, size_t nbo, bool va, bool vb, const 
   use(a);
   use(b);
   use(c);
-  emit([=](const Ctx& cx) {
-    GemmArgs q = g;
-    q.a = cx.ptr(a), q.b = cx.ptr(b), q.c = cx.ptr(c);
-    launch_gemm(dt, prec, gemm_choice_[idx], q, cx.s);
-  });
+  double nz = (double)g.nbo * (double)g.nbi, es = (double)es_;
+  double flops = (dtype_complex(dt) ? 8.0 : 2.0) * nz * (double)g.m * (double)g.k * (double)g.n;
+  double bytes = es * (((g.sao || g.nbo == 1) ? nz : (double)g.nbi) * (double)(g.m * g.k) +
+                       ((g.sbo || g.nbo == 1) ? nz : (double)g.nbi) * (double)(g.k * g.n) + nz * (double)(g.m * g.n));
+  emit(
+      [=](const Ctx& cx) {
+        GemmArgs q = g;
+        q.a = cx.ptr(a), q.b = cx.ptr(b), q.c = cx.ptr(c);
+        launch_gemm(dt, prec, gemm_choice_[idx], q, cx.s);
+      },
+      flops, bytes);
 }
 
 void Executor::gen_steps(bool body, size_t nb, std::vector<Ref>& slots, std::vector<std::vector<size_t>>& shapes) {
@@ -189,7 +197,8 @@ void Executor::gen_steps(bool body, size_t nb, std::vector<Ref>& slots, std::vec
         Ref src = cur;
         use(src);
         use(dst);
-        emit([=](const Ctx& c) { launch_sum_axis(dt, outer, d, inner, c.ptr(src), c.ptr(dst), c.s); });
+        emit([=](const Ctx& c) { launch_sum_axis(dt, outer, d, inner, c.ptr(src), c.ptr(dst), c.s); }, 0,
+             (double)(outer * inner * (d + 1)) * es);
         temps.push_back(dst);
         shape.erase(shape.begin() + ax);
         cur = dst;
@@ -200,7 +209,7 @@ void Executor::gen_steps(bool body, size_t nb, std::vector<Ref>& slots, std::vec
         Ref src = cur;
         use(src);
         use(dst);
-        emit([=](const Ctx& c) { launch_permute(pp, c.ptr(src), c.ptr(dst), c.s); });
+        emit([=](const Ctx& c) { launch_permute(pp, c.ptr(src), c.ptr(dst), c.s); }, 0, 2.0 * pp.numel * es);
         temps.push_back(dst);
         cur = dst;
       }
@@ -244,7 +253,8 @@ Ref Executor::gen_finalize(size_t nb, const Ref& src0, bool to_output) {
     Ref src = cur;
     use(src);
     use(dst);
-    emit([=](const Ctx& c) { launch_sum_axis(dt, outer, d, inner, c.ptr(src), c.ptr(dst), c.s); });
+    emit([=](const Ctx& c) { launch_sum_axis(dt, outer, d, inner, c.ptr(src), c.ptr(dst), c.s); }, 0,
+         (double)(outer * inner * (d + 1)) * es);
     shape.erase(shape.begin() + ax);
     cur = dst;
   }
@@ -254,12 +264,12 @@ Ref Executor::gen_finalize(size_t nb, const Ref& src0, bool to_output) {
     Ref src = cur;
     use(src);
     use(dst);
-    emit([=](const Ctx& c) { launch_permute(pp, c.ptr(src), c.ptr(dst), c.s); });
+    emit([=](const Ctx& c) { launch_permute(pp, c.ptr(src), c.ptr(dst), c.s); }, 0, 2.0 * numel * es);
     cur = dst;
   } else if (to_output && cur.kind != Ref::OUTPUT) {
     Ref src = cur;
     use(src);
-    emit([=](const Ctx& c) { launch_copy(numel * es, c.ptr(src), c.out, c.s); });
+    emit([=](const Ctx& c) { launch_copy(numel * es, c.ptr(src), c.out, c.s); }, 0, 2.0 * numel * es);
     cur = Ref{Ref::OUTPUT, 0};
   }
   return cur;
@@ -314,7 +324,8 @@ void Executor::build(size_t B) {
         size_t n_int = P.n_internal;
         use(res);
         use(acc);
-        emit([=](const Ctx& c) { launch_reduce_f64(dt, c.ptr(res), c.ptr(acc), c.g0, nb, n_int, out_numel, c.s); });
+        emit([=](const Ctx& c) { launch_reduce_f64(dt, c.ptr(res), c.ptr(acc), c.g0, nb, n_int, out_numel, c.s); }, 0,
+             (double)(nb * out_numel * es) + 4.0 * out_numel * es);
       }
     };
     cur_ = &body_;
@@ -447,28 +458,85 @@ void Executor::run_ops(const std::vector<Op>& ops, const Ctx& c) {
   for (auto& op : ops) op(c);
 }
 
-void Executor::enqueue(const Ctx& c0) {
+void Executor::enqueue(const Ctx& c0, const std::function<void(const Op&, const Ctx&)>& visit) {
   const Plan& P = *P_;
   Ctx c = c0;
-  run_ops(pre_, c);
+  auto run = [&](const std::vector<Op>& ops) {
+    if (visit)
+      for (auto& op : ops) visit(op, c);
+    else
+      run_ops(ops, c);
+  };
+  run(pre_);
   if (o_.batched) {
     size_t nfull = P.n_slices / B_;
     for (size_t q = 0; q < nfull; ++q) {
       c.g0 = q * B_;
-      run_ops(body_, c);
+      run(body_);
     }
     if (tail_nb_) {
       c.g0 = nfull * B_;
-      run_ops(tail_, c);
+      run(tail_);
     }
   } else {
     for (size_t g = 0; g < P.n_slices; ++g) {
       c.g0 = g;
-      run_ops(body_, c);
+      run(body_);
     }
   }
   c.g0 = 0;
-  run_ops(post_, c);
+  run(post_);
+}
+
+void Executor::profile(const void* const* inputs, void* out, hipStream_t s, std::vector<KernelStat>& stats) {
+  Ctx c;
+  c.ws = ws_;
+  c.offs = &offs_;
+  c.inputs = inputs;
+  c.out = out;
+  c.s = s;
+  if (o_.autotune && !tuned_) autotune(c, s);
+  struct Rec {
+    hipEvent_t a, b;
+    int kind;
+    size_t launches;
+    double flops, bytes;
+  };
+  std::vector<Rec> recs;
+  std::vector<hipEvent_t> pool;
+  auto ev = [&]() {
+    hipEvent_t e;
+    ATN_HIP(hipEventCreate(&e));
+    pool.push_back(e);
+    return e;
+  };
+  try {
+    enqueue(c, [&](const Op& op, const Ctx& cx) {
+      Rec r{ev(), ev(), K_OTHER, 0, op.flops, op.bytes};
+      size_t l0 = g_launch_count;
+      ATN_HIP(hipEventRecord(r.a, cx.s));
+      op(cx);
+      ATN_HIP(hipEventRecord(r.b, cx.s));
+      r.kind = g_last_kind;
+      r.launches = g_launch_count - l0;
+      recs.push_back(r);
+    });
+    ATN_HIP(hipStreamSynchronize(s));
+    stats.assign(K_NKINDS, KernelStat{});
+    for (auto& r : recs) {
+      float ms = 0;
+      ATN_HIP(hipEventElapsedTime(&ms, r.a, r.b));
+      auto& k = stats[r.kind];
+      k.launches += r.launches;
+      k.ms += ms;
+      k.flops += r.flops;
+      k.bytes += r.bytes;
+    }
+  } catch (...) {
+    for (auto e : pool) (void)hipEventDestroy(e);
+    throw;
+  }
+  for (auto e : pool) (void)hipEventDestroy(e);
 }
 
 void Executor::autotune(const Ctx& c, hipStream_t s) {

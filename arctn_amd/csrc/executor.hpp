@@ -49,13 +49,27 @@ struct Ctx {
   }
 };
 
-using Op = std::function<void(const Ctx&)>;
+// One enqueued operation and its algorithmic work (FLOPs and HBM bytes of the
+// tensors it must read and write), used for per-kernel roofline accounting.
+struct Op {
+  std::function<void(const Ctx&)> fn;
+  double flops = 0, bytes = 0;
+  void operator()(const Ctx& c) const { fn(c); }
+};
+
+struct KernelStat {
+  size_t launches = 0;
+  double ms = 0, flops = 0, bytes = 0;
+};
 
 class Executor {
  public:
   Executor(std::shared_ptr<const Plan> plan, const ExecOptions& o);
   ~Executor();
   void run(const void* const* inputs, void* out, hipStream_t s, Report* rep);
+  // Eager run with a pair of HIP events around every operation; accumulates
+  // per-kernel-kind launches, device time and algorithmic FLOPs/bytes.
+  void profile(const void* const* inputs, void* out, hipStream_t s, std::vector<KernelStat>& stats);
   size_t batch() const { return B_; }
   size_t workspace_bytes() const { return ws_bytes_; }
 
@@ -67,13 +81,16 @@ class Executor {
   void build(size_t B);
   int newbuf(size_t bytes);
   void use(const Ref& r);
-  void emit(Op op) { cur_->push_back(std::move(op)); ++t_; }
+  void emit(std::function<void(const Ctx&)> fn, double flops = 0, double bytes = 0) {
+    cur_->push_back(Op{std::move(fn), flops, bytes});
+    ++t_;
+  }
   void gen_steps(bool body, size_t nb, std::vector<Ref>& slots, std::vector<std::vector<size_t>>& shapes);
   void gen_inputs(bool body, size_t nb, std::vector<Ref>& slots, std::vector<std::vector<size_t>>& shapes);
   Ref gen_finalize(size_t nb, const Ref& src, bool to_output);
   void gen_gemm(const PairStep& st, size_t nb_outer, bool va, bool vb, const Ref& a, const Ref& b, const Ref& c);
   void run_ops(const std::vector<Op>& ops, const Ctx& c);
-  void enqueue(const Ctx& c);
+  void enqueue(const Ctx& c, const std::function<void(const Op&, const Ctx&)>& visit = nullptr);
   void autotune(const Ctx& c, hipStream_t s);
 
   struct GemmSite {

@@ -249,6 +249,7 @@ static void run_simple(int dt, const GemmArgs& g, hipStream_t s, int opa = 0, in
     // keep the total grid bounded for huge batches of tiny problems
     dim3 grid(gx, zgrid(g));
     if (nz * gx > (1u << 22)) grid.y = std::max<unsigned>(1, (1u << 22) / gx);
+    g_last_kind = K_GEMM_SIMPLE;
     if (opa == 0 && opb == 0) LAUNCH((gemm_simple_kernel<T, 0, 0>), grid, dim3(256), 0, s, g);
     else if (opa == 1 && opb == 0) LAUNCH((gemm_simple_kernel<T, 1, 0>), grid, dim3(256), 0, s, g);
     else if (opa == 0 && opb == 1) LAUNCH((gemm_simple_kernel<T, 0, 1>), grid, dim3(256), 0, s, g);
@@ -261,6 +262,7 @@ static void run_mfma(const GemmArgs& g, hipStream_t s) {
   int64_t N = CPLX ? 2 * g.n : g.n;
   uint64_t tiles = (uint64_t)((g.m + BM - 1) / BM) * (uint64_t)((N + BN - 1) / BN);
   dim3 grid((unsigned)std::min<uint64_t>(tiles, 1u << 20), zgrid(g));
+  g_last_kind = sizeof(F) == 4 ? K_GEMM_MFMA_F32 : K_GEMM_MFMA_F64;
   LAUNCH((gemm_mfma_kernel<F, CPLX, BM, BN, BK>), grid, dim3(256), 0, s, g);
 }
 

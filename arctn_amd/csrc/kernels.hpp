@@ -107,5 +107,11 @@ int launch_gemm(int dt, int prec, int kernel, const GemmArgs& g, hipStream_t s);
 void launch_gemm_adjoint(int dt, int opa, int opb, const GemmArgs& g, hipStream_t s);
 
 extern thread_local size_t g_launch_count;
+// Kind of the last kernel launched on this thread (profiling labels).
+enum KernelKind { K_OTHER = 0, K_PERMUTE_TILE, K_PERMUTE_GENERIC, K_SUM, K_TRACE_DIAG, K_GATHER, K_REDUCE, K_SCATTER,
+                  K_GEMM_SIMPLE, K_GEMM_MFMA_F32, K_GEMM_MFMA_F64, K_GEMM_MFMA_BF16X3, K_GEMM_MFMA_FP16X2, K_GEMM_MFMA_C3M,
+                  K_STRIP, K_COPY, K_NKINDS };
+extern thread_local int g_last_kind;
+const char* kernel_kind_name(int k);
 
 }  // namespace atn

@@ -10,6 +10,15 @@
 namespace atn {
 
 thread_local size_t g_launch_count = 0;
+thread_local int g_last_kind = K_OTHER;
+
+const char* kernel_kind_name(int k) {
+  static const char* names[K_NKINDS] = {"other", "permute_tile", "permute_generic", "sum_axis", "trace_diag", "slice_gather",
+                                        "slice_reduce_f64", "output_scatter", "gemm_simple", "gemm_mfma_f32",
+                                        "gemm_mfma_f64", "gemm_mfma_bf16x3", "gemm_mfma_fp16x2", "gemm_mfma_c3m",
+                                        "strip", "copy"};
+  return (k >= 0 && k < K_NKINDS) ? names[k] : "other";
+}
 
 template <int ES> struct Bits;
 template <> struct Bits<4> { using T = uint32_t; };
@@ -361,8 +370,10 @@ void launch_permute(const PermPlan& p, const void* in, void* out, hipStream_t s)
     using E = typename Bits<ES>::T;
     if (p.kind == 1) {
       unsigned grid = (unsigned)std::min<uint64_t>(p.tile.n_tiles, 2048u * 4u);
+      g_last_kind = K_PERMUTE_TILE;
       LAUNCH(permute_tile_kernel<ES>, grid, p.threads, p.lds, s, p.tile, (const E*)in, (E*)out);
     } else {
+      g_last_kind = K_PERMUTE_GENERIC;
       LAUNCH(permute_generic_kernel<ES>, grid_for(p.gen.numel, 256), 256, 0, s, p.gen, (const E*)in, (E*)out);
     }
   });
@@ -376,6 +387,7 @@ __global__ void sum_axis_exact_kernel(const T* __restrict__ in, T* __restrict__ 
 void launch_sum_axis(int dt, size_t outer, size_t d, size_t inner, const void* in, void* out, hipStream_t s) {
   dispatch_dt(dt, [&](auto tag) {
     using T = decltype(tag);
+    g_last_kind = K_SUM;
     LAUNCH(sum_axis_exact_kernel<T>, grid_for(outer * inner, 256), 256, 0, s, (const T*)in, (T*)out, (uint64_t)outer,
            (uint64_t)d, (uint64_t)inner);
   });
@@ -432,6 +444,7 @@ __global__ void diag_kernel(const typename Bits<ES>::T* __restrict__ in, typenam
 void launch_trace(int dt, size_t A, size_t d, size_t Bm, size_t C, const void* in, void* out, hipStream_t s) {
   dispatch_dt(dt, [&](auto tag) {
     using T = decltype(tag);
+    g_last_kind = K_TRACE_DIAG;
     LAUNCH(trace_kernel<T>, grid_for(A * Bm * C, 256), 256, 0, s, (const T*)in, (T*)out, (uint64_t)A, (uint64_t)d,
            (uint64_t)Bm, (uint64_t)C);
   });
@@ -440,6 +453,7 @@ void launch_diag(int es, size_t A, size_t d, size_t Bm, size_t C, const void* in
   dispatch_es(es, [&](auto esc) {
     constexpr int ES = decltype(esc)::value;
     using E = typename Bits<ES>::T;
+    g_last_kind = K_TRACE_DIAG;
     LAUNCH(diag_kernel<ES>, grid_for(A * d * Bm * C, 256), 256, 0, s, (const E*)in, (E*)out, (uint64_t)A, (uint64_t)d,
            (uint64_t)Bm, (uint64_t)C);
   });
@@ -467,6 +481,7 @@ void launch_gather(int es, const GatherArgs& a, uint64_t g0, size_t nb, const vo
   dispatch_es(es, [&](auto esc) {
     constexpr int ES = decltype(esc)::value;
     using E = typename Bits<ES>::T;
+    g_last_kind = K_GATHER;
     LAUNCH(gather_kernel<ES>, grid_for(nb * a.inner, 256), 256, 0, s, a, g0, (uint64_t)nb, (const E*)in, (E*)out);
   });
 }
@@ -500,6 +515,7 @@ void launch_reduce_f64(int dt, const void* part, void* acc64, uint64_t g0, size_
   dispatch_dt(dt, [&](auto tag) {
     using T = decltype(tag);
     uint64_t blocks = (g0 + nb - 1) / n_internal - g0 / n_internal + 1;
+    g_last_kind = K_REDUCE;
     LAUNCH(reduce_f64_kernel<T>, grid_for(blocks * numel, 256), 256, 0, s, (const T*)part, (typename Wide<T>::W*)acc64, g0,
            (uint64_t)nb, (uint64_t)n_internal, (uint64_t)numel);
   });
@@ -529,6 +545,7 @@ void launch_scatter(int dt, bool src_f64, const ScatterArgs& a, uint64_t blk0, s
   dispatch_dt(dt, [&](auto tag) {
     using T = decltype(tag);
     using W = typename Wide<T>::W;
+    g_last_kind = K_SCATTER;
     if (src_f64)
       LAUNCH((scatter_kernel<T, W>), grid_for(n_blk * numel, 256), 256, 0, s, a, blk0, (uint64_t)n_blk, (uint64_t)numel,
              (const W*)src, (T*)out);
@@ -547,6 +564,7 @@ __global__ void accum_kernel(const T* __restrict__ part, T* __restrict__ acc, ui
 void launch_accum(int dt, bool first, const void* part, void* acc, size_t numel, hipStream_t s) {
   dispatch_dt(dt, [&](auto tag) {
     using T = decltype(tag);
+    g_last_kind = K_REDUCE;
     LAUNCH(accum_kernel<T>, grid_for(numel, 256), 256, 0, s, (const T*)part, (T*)acc, (uint64_t)numel, first ? 1 : 0);
   });
 }
@@ -627,12 +645,14 @@ __global__ void scale_pow2_kernel<double>(double* __restrict__ x, uint64_t n, co
 void launch_maxabs(int dt, const void* x, size_t numel, uint64_t* maxbits, hipStream_t s) {
   dispatch_dt(dt, [&](auto tag) {
     using T = decltype(tag);
+    g_last_kind = K_STRIP;
     LAUNCH(maxabs_kernel<T>, grid_for(numel, 256, 2048), 256, 0, s, (const T*)x, (uint64_t)numel, (unsigned long long*)maxbits);
   });
 }
 void launch_scale_pow2(int dt, void* x, size_t numel, const uint64_t* maxbits, int64_t* expo, hipStream_t s) {
   dispatch_dt(dt, [&](auto tag) {
     using T = decltype(tag);
+    g_last_kind = K_STRIP;
     LAUNCH(scale_pow2_kernel<T>, grid_for(numel, 256), 256, 0, s, (T*)x, (uint64_t)numel, (const unsigned long long*)maxbits,
            (long long*)expo);
   });
@@ -646,6 +666,7 @@ __global__ void copy_kernel(const typename Bits<ES>::T* __restrict__ in, typenam
 
 void launch_copy(size_t bytes, const void* in, void* out, hipStream_t s) {
   if (bytes % 4) throw Error("copy needs a multiple of 4 bytes");
+  g_last_kind = K_COPY;
   LAUNCH(copy_kernel<4>, grid_for(bytes / 4, 256), 256, 0, s, (const uint32_t*)in, (uint32_t*)out, (uint64_t)(bytes / 4));
 }
 

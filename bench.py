@@ -1,0 +1,167 @@
+#!/usr/bin/env python3
+"""Benchmark of the MI355X contraction executor on the sliced 30-qubit circuit.
+
+Workload: 30-qubit 5x6-grid random-circuit amplitude network, 14 cycles
+(Haar single-qubit gates + fSim(pi/2, pi/6) on alternating edge patterns),
+complex64, one amplitude, plan = random-greedy path sliced to 2^24 elements
+(256 internal slices), synthetic seeded data. A "step" is one complete
+contraction of all 256 slices.
+
+Printed JSON line (one, on stdout):
+  value          whole-job contraction TFLOP/s of the measured form
+                 (algorithmic 8*m*k*n per complex step, all slices)
+  slices_per_s   slices contracted per second by that form
+  plain_*        the plain per-slice FP32 GPU path (one slice, one step at a time)
+  rel_err        |x - ref| / |ref| against the complex128 GPU path (FP64 MFMA,
+                 FP64 sums), itself checked against the CPU oracle in tests
+  roofline       the dominant kernel's achieved rate vs the MI355X peak
+  cpu_baseline   the oracle (CPU restatement of the reference path) on a
+                 bounded sample of slices, on this host's cores
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "contraction TFLOP/s + slices/s vs plain FP32 GPU path at rel err ≤1e-5"
+PEAKS = {"gemm_mfma_f32": 157.3e12, "gemm_mfma_f64": 78.6e12, "gemm_mfma_c3m": 157.3e12,
+         "gemm_mfma_bf16x3": 2.5e15, "gemm_mfma_fp16x2": 2.5e15}
+HBM_PEAK = 8.0e12
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="circuit_5x6_c14_t24")
+    ap.add_argument("--precision", default="native")
+    ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--skip-plain", action="store_true")
+    ap.add_argument("--skip-cpu", action="store_true")
+    ap.add_argument("--skip-profile", action="store_true")
+    ap.add_argument("--cpu-slices", type=int, default=0, help="CPU sample size (0 = one per core)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the slice range of each GPU "
+              "would be one contiguous share of the slices, reduced once at the end.", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    from arctn_amd import Plan
+    from arctn_amd.api import profile
+    from arctn_amd.plans.make_plans import load
+
+    log(f"[bench] loading workload {args.workload}")
+    net, arrays, doc = load(args.workload)
+    plan = Plan(net["inputs"], net["output"], net["size_dict"], doc["ssa_path"], doc["sliced_legs"])
+    info = plan.info()
+    dev = torch.device("cuda", 0)
+    x64 = [torch.from_numpy(a).to(dev) for a in arrays]
+    x128 = [t.to(torch.complex128) for t in x64]
+    stream = torch.cuda.current_stream(dev)
+    form = dict(batched=True, use_graph=not args.no_graph, precision=args.precision)
+    flops = 8.0 * info["flops_per_slice"] * info["n_slices"]
+
+    log(f"[bench] plan: {info['n_steps']} steps, {info['n_slices']} slices, "
+        f"{info['n_varying_steps']} slice-dependent steps, {flops:.3e} FLOP per contraction")
+    out = torch.empty([], dtype=torch.complex64, device=dev)
+    for w in range(args.warmup):
+        plan.run(x64, 2, out=out, **form)
+        torch.cuda.synchronize()
+        log(f"[bench] warmup {w + 1}/{args.warmup} done")
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    e0.record(stream)
+    for _ in range(args.steps):
+        plan.run(x64, 2, out=out, **form)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    wall_ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    ms = e0.elapsed_time(e1) / args.steps
+    value = flops / (ms * 1e-3) / 1e12
+    log(f"[bench] {args.steps} steps: {ms:.2f} ms/step (wall {wall_ms:.2f}), {value:.2f} TFLOP/s")
+    got = complex(out.cpu())
+
+    ref_out, _ = plan.run(x128, 3, batched=True)
+    ref = complex(ref_out.cpu())
+    rel_err = abs(got - ref) / abs(ref)
+    log(f"[bench] amplitude {got:.6e} vs complex128 {ref:.6e}: rel err {rel_err:.2e}")
+
+    plain = {}
+    if not args.skip_plain:
+        plan.run(x64, 2, batched=False)
+        torch.cuda.synchronize()
+        p_out, rep = plan.run(x64, 2, batched=False, report=True)
+        plain = {"plain_fp32_ms": rep["elapsed_ms"], "plain_fp32_tflops": rep["tflops"],
+                 "plain_slices_per_s": rep["slices_per_s"], "plain_launches": rep["n_launches"],
+                 "plain_rel_err": abs(complex(p_out.cpu()) - ref) / abs(ref),
+                 "speedup_vs_plain": rep["elapsed_ms"] / ms}
+        log(f"[bench] plain per-slice FP32 path: {rep['elapsed_ms']:.1f} ms -> speedup {rep['elapsed_ms'] / ms:.2f}x")
+
+    roofline = None
+    kernels = {}
+    if not args.skip_profile:
+        kernels = profile(plan, x64, 2, batched=True, precision=args.precision)
+        dom = max(kernels, key=lambda k: kernels[k]["ms"])
+        k = kernels[dom]
+        if dom in PEAKS:
+            ach = k["flops"] / (k["ms"] * 1e-3)
+            roofline = {"bound": "mfma", "achieved": ach / 1e12, "peak": PEAKS[dom] / 1e12, "unit": "TFLOP/s",
+                        "frac": ach / PEAKS[dom], "traffic": None, "kernel": dom,
+                        "avg_launch_ms": k["ms"] / k["launches"], "launches": k["launches"]}
+        else:
+            ach = k["bytes"] / (k["ms"] * 1e-3)
+            roofline = {"bound": "hbm", "achieved": ach / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                        "frac": ach / HBM_PEAK, "traffic": None, "kernel": dom,
+                        "avg_launch_ms": k["ms"] / k["launches"], "launches": k["launches"]}
+        log(f"[bench] dominant kernel {dom}: {roofline['achieved']:.1f} {roofline['unit']} "
+            f"({100 * roofline['frac']:.1f}% of peak)")
+
+    cpu = None
+    if not args.skip_cpu:
+        from oracle import pyoracle as O
+
+        cores = min(16, os.cpu_count() or 1)
+        n = args.cpu_slices or cores
+        log(f"[bench] CPU oracle sample: {n} slices on {cores} threads")
+        t0 = time.perf_counter()
+        O.contract_slice_range(net, arrays, [tuple(p) for p in doc["ssa_path"]], doc["sliced_legs"], 0, n,
+                               threads=cores)
+        dt = time.perf_counter() - t0
+        cpu = {"value": n / dt, "unit": "slices/s", "cores": cores, "kind": "port",
+               "sample": f"slices 0..{n - 1} of {info['n_slices']} (complex64), oracle CPU restatement, "
+                         f"{dt:.1f} s", "tflops": 8.0 * info["flops_per_slice"] * n / dt / 1e12}
+        log(f"[bench] CPU: {n / dt:.3f} slices/s")
+
+    line = {
+        "metric": METRIC, "value": round(value, 3), "unit": "TFLOP/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(ms, 3), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "complex64", "data": "synthetic (seeded random circuit, random-init gates)",
+        "config": {"workload": args.workload, "qubits": 30, "grid": "5x6", "cycles": 14,
+                   "n_slices": info["n_slices"], "target_size": 2 ** 24, "form": "batched" + ("" if args.no_graph else "+hipgraph"),
+                   "precision": args.precision, "flops_per_contraction": flops},
+        "slices_per_s": round(info["n_slices"] / (ms * 1e-3), 2),
+        "rel_err": rel_err,
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "kernels": kernels,
+    }
+    line.update(plain)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -133,6 +133,20 @@ void atn_executor_destroy(atn_executor* ex);
  * work is enqueued and, when report != NULL, completed and timed. */
 int atn_executor_run(atn_executor* ex, const void* const* d_inputs, void* d_out, void* stream, atn_report* report);
 
+/* Per-kernel accounting of one eager execution with HIP events around every
+ * launch on `stream` (used for roofline reporting; no reference counterpart).
+ * stats[k] for kernel kind k: name, launches, device ms, algorithmic FLOPs and
+ * HBM bytes (operands read + results written). *n_stats <= max_stats kinds. */
+typedef struct {
+  char name[32];
+  size_t launches;
+  double ms;
+  double flops;
+  double bytes;
+} atn_kernel_stat;
+int atn_executor_profile(atn_executor* ex, const void* const* d_inputs, void* d_out, void* stream,
+                         atn_kernel_stat* stats, size_t max_stats, size_t* n_stats);
+
 /* One-shot convenience: create an executor, run, destroy. */
 int atn_execute(const atn_plan* plan, const atn_exec_options* opts, const void* const* d_inputs, void* d_out,
                 void* stream, atn_report* report);

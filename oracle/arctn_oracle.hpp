@@ -971,6 +971,66 @@ Tensor<T> contract_sliced(const Network& net, const std::vector<Tensor<T>>& ts, 
   return acc;
 }
 
+// Bounded sample of the sliced loop: internal slices [lo, hi) only, summed with
+// the reference's chunking applied to the sample (src/slice.rs:1018-1064).
+template <class T>
+Tensor<T> contract_slice_range(const Network& net, const std::vector<Tensor<T>>& ts, const Path& path,
+                               const std::vector<Leg>& sliced, size_t lo, size_t hi, int threads) {
+  validate_shapes(net, ts);
+  validate_slice_legs(net, sliced);
+  for (Leg l : sliced)
+    if (std::count(net.output.begin(), net.output.end(), l)) throw Error("slice range supports internal slices only");
+  std::map<Leg, size_t> spos;
+  for (size_t i = 0; i < sliced.size(); ++i) spos[sliced[i]] = i;
+  Network sub = net;
+  for (auto& legs : sub.inputs) legs.erase(std::remove_if(legs.begin(), legs.end(), [&](Leg l) { return spos.count(l) > 0; }), legs.end());
+  for (Leg l : sliced) sub.size_dict.erase(l);
+  auto dims = net.shape_of(sliced);
+  size_t n_slices = 1;
+  for (size_t d : dims) n_slices *= d;
+  if (!(lo < hi && hi <= n_slices)) throw Error("slice range out of bounds");
+  auto run_one = [&](size_t s) {
+    std::vector<size_t> idx(sliced.size());
+    decode_slice_index(s, dims, idx);
+    std::vector<Tensor<T>> st;
+    for (size_t i = 0; i < ts.size(); ++i) {
+      Tensor<T> t = ts[i];
+      for (size_t ax = net.inputs[i].size(); ax-- > 0;) {
+        auto it = spos.find(net.inputs[i][ax]);
+        if (it != spos.end()) t = t.select_axis(ax, idx[it->second]);
+      }
+      st.push_back(std::move(t));
+    }
+    return contract_network(sub, std::move(st), path);
+  };
+  size_t n = hi - lo;
+  size_t out_numel = 1;
+  for (Leg l : net.output) out_numel *= net.dim(l);
+  size_t n_chunks = slice_parallel_chunks(n, out_numel), chunk = (n + n_chunks - 1) / n_chunks;
+  std::vector<Tensor<T>> partial(n_chunks);
+  std::vector<std::string> errs(n_chunks);
+#pragma omp parallel for schedule(dynamic, 1) num_threads(threads > 0 ? threads : 1) if (n_chunks > 1 && threads > 1)
+  for (long c = 0; c < (long)n_chunks; ++c) {
+    try {
+      size_t a = lo + c * chunk, b = std::min(lo + (c + 1) * chunk, hi);
+      Tensor<T> acc = run_one(a);
+      for (size_t s = a + 1; s < b; ++s) {
+        Tensor<T> part = run_one(s);
+        for (size_t e = 0; e < acc.data.size(); ++e) acc.data[e] = add(acc.data[e], part.data[e]);
+      }
+      partial[c] = std::move(acc);
+    } catch (const std::exception& ex) {
+      errs[c] = ex.what();
+    }
+  }
+  for (auto& e : errs)
+    if (!e.empty()) throw Error(e);
+  Tensor<T> acc = std::move(partial[0]);
+  for (size_t c = 1; c < n_chunks; ++c)
+    for (size_t e = 0; e < acc.data.size(); ++e) acc.data[e] = add(acc.data[e], partial[c].data[e]);
+  return acc;
+}
+
 // ---------------------------------------------------------------- naive -----
 // src/naive.rs:10-76: brute force over the joint index space (<= 2^28).
 template <class T>

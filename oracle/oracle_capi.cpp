@@ -178,6 +178,19 @@ int orc_contract(const orc_net* d, int dtype, int mode, const void* const* input
   });
 }
 
+// Internal slices [lo, hi) of a sliced contraction, summed (bounded CPU sample).
+int orc_contract_slice_range(const orc_net* d, int dtype, size_t lo, size_t hi, const void* const* inputs, void* out,
+                             int threads) {
+  return guard([&] {
+    Network net = to_net(d);
+    dispatch(dtype, [&](auto tag) {
+      using T = decltype(tag);
+      auto ts = load_inputs<T>(net, inputs);
+      store(contract_slice_range(net, ts, to_path(d), to_sliced(d), lo, hi, threads), out);
+    });
+  });
+}
+
 // Gradients of the compiled contraction w.r.t. every input, given
 // dL/d(conj output). grads[i] receives input i's gradient.
 int orc_gradient(const orc_net* d, int dtype, const void* const* inputs, const void* grad_out, void* const* grads) {

@@ -51,6 +51,8 @@ def lib():
         L.orc_simulate_path.argtypes = [C.POINTER(_Net), C.POINTER(C.c_double)]
         L.orc_contract.argtypes = [C.POINTER(_Net), C.c_int, C.c_int, C.POINTER(C.c_void_p), C.c_void_p,
                                    C.POINTER(C.c_double), C.c_int]
+        L.orc_contract_slice_range.argtypes = [C.POINTER(_Net), C.c_int, C.c_size_t, C.c_size_t,
+                                               C.POINTER(C.c_void_p), C.c_void_p, C.c_int]
         L.orc_gradient.argtypes = [C.POINTER(_Net), C.c_int, C.POINTER(C.c_void_p), C.c_void_p, C.POINTER(C.c_void_p)]
         L.orc_tensor_op.argtypes = [C.c_int, C.c_int, C.c_size_t, C.POINTER(C.c_size_t), C.POINTER(C.c_size_t),
                                     C.c_size_t, C.c_void_p, C.c_void_p]
@@ -106,6 +108,16 @@ def contract(net, arrays, path=(), mode="compiled", sliced=(), threads=1):
     _check(lib().orc_contract(C.byref(na.s), DTYPE_CODE[dt], MODES[mode], _ptrs(arrays),
                               out.ctypes.data, C.byref(e), threads))
     return (out, e.value) if mode == "stripped" else out
+
+
+def contract_slice_range(net, arrays, path, sliced, lo, hi, threads=1):
+    """Sum of internal slices [lo, hi) of a sliced contraction."""
+    arrays, dt = _prep(arrays)
+    na = NetArgs(net, path, sliced)
+    out = np.zeros([net["size_dict"][l] for l in net["output"]], dtype=dt)
+    _check(lib().orc_contract_slice_range(C.byref(na.s), DTYPE_CODE[dt], lo, hi, _ptrs(arrays), out.ctypes.data,
+                                          threads))
+    return out
 
 
 def gradient(net, arrays, path, grad_out):
