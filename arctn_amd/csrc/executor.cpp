@@ -168,10 +168,12 @@ void Executor::gen_gemm(const PairStep& st, size_t nbo, bool va, bool vb, const 
   double flops = (dtype_complex(dt) ? 8.0 : 2.0) * nz * (double)g.m * (double)g.k * (double)g.n;
   double bytes = es * (((g.sao || g.nbo == 1) ? nz : (double)g.nbi) * (double)(g.m * g.k) +
                        ((g.sbo || g.nbo == 1) ? nz : (double)g.nbi) * (double)(g.k * g.n) + nz * (double)(g.m * g.n));
+  Ref scr{Ref::WS, scratch_buf_};
   emit(
       [=](const Ctx& cx) {
         GemmArgs q = g;
         q.a = cx.ptr(a), q.b = cx.ptr(b), q.c = cx.ptr(c);
+        q.scratch = cx.ptr(scr);
         launch_gemm(dt, prec, gemm_choice_[idx], q, cx.s);
       },
       flops, bytes);
@@ -290,6 +292,7 @@ void Executor::build(size_t B) {
   const bool sliced = P.n_slices > 1;
   const size_t out_numel = P.sub_out_numel;
   cur_ = &pre_;
+  scratch_buf_ = newbuf(256);  // FP16X2 operand scales; lives for the whole run
   if (o_.strip) {
     maxbits_buf_ = newbuf(8);
     expo_buf_ = newbuf(8);
@@ -382,9 +385,10 @@ void Executor::build(size_t B) {
     }
     body_end_ = t_;
   }
-  // strip bookkeeping lives for the whole run
+  // strip bookkeeping and GEMM scratch live for the whole run
   for (int id : {maxbits_buf_, expo_buf_})
     if (o_.strip && id >= 0) bufs_[id].first = 0, bufs_[id].last = t_ + 1;
+  bufs_[scratch_buf_].first = 0, bufs_[scratch_buf_].last = t_ + 1;
   if (acc_buf_ >= 0 && o_.batched && sliced) bufs_[acc_buf_].last = t_ + 1;
   // Offline first-fit placement of buffers with overlapping lifetimes.
   std::vector<int> order(bufs_.size());
@@ -557,9 +561,10 @@ void Executor::autotune(const Ctx& c, hipStream_t s) {
     }
     GemmArgs q = site.g;
     q.a = c.ptr(site.a), q.b = c.ptr(site.b), q.c = c.ptr(site.c);
+    q.scratch = c.ws + offs_[scratch_buf_];
     float best_ms = 1e30f;
     int choice = GEMM_SIMPLE;
-    for (int cand : {GEMM_SIMPLE, GEMM_MFMA}) {
+    for (int cand : {GEMM_SIMPLE, GEMM_SKINNY, GEMM_MFMA}) {
       launch_gemm(o_.dtype, o_.precision, cand, q, s);
       ATN_HIP(hipEventRecord(e0, s));
       for (int r = 0; r < 3; ++r) launch_gemm(o_.dtype, o_.precision, cand, q, s);
@@ -792,6 +797,7 @@ void stage_step_forward(const Plan& p, size_t step, int dt, int prec, const void
   g.lda = g.k, g.ldb = g.n, g.ldc = g.n;
   g.nbo = 1, g.nbi = st.bt;
   g.sai = g.m * g.k, g.sbi = g.k * g.n, g.sci = g.m * g.n;
+  g.scratch = sc.get(256);
   launch_gemm(dt, prec, GEMM_AUTO, g, s);
 }
 

@@ -111,7 +111,7 @@ class Executor {
   size_t tail_nb_ = 0;
   char* ws_ = nullptr;
   size_t ws_bytes_ = 0;
-  int acc_buf_ = -1, expo_buf_ = -1, maxbits_buf_ = -1;
+  int acc_buf_ = -1, expo_buf_ = -1, maxbits_buf_ = -1, scratch_buf_ = -1;
   hipGraphExec_t gexec_ = nullptr;
   hipGraph_t graph_ = nullptr;
   hipStream_t cap_stream_ = nullptr;

@@ -17,6 +17,8 @@ namespace atn {
 
 typedef float v4f __attribute__((ext_vector_type(4)));
 typedef double v4d __attribute__((ext_vector_type(4)));
+typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
+typedef _Float16 v8h __attribute__((ext_vector_type(8)));
 
 #define LAUNCH(kernel, grid, block, lds, stream, ...)                            \
   do {                                                                            \
@@ -73,6 +75,88 @@ __global__ __launch_bounds__(256) void gemm_simple_kernel(GemmArgs g) {
         acc = CM<T>::mac(acc, a, b);
       }
       C[i * g.ldc + j] = acc;
+    }
+  }
+}
+
+// ----------------------------------------------------------- skinny GEMMs --
+// Small m (m <= MM): one thread per output column j keeps all m accumulators,
+// so B and C are streamed exactly once with coalesced accesses; A (m x k) is
+// wave-uniform and served from the scalar cache. Loads of UN consecutive k
+// values are issued before the multiply-adds.
+template <class T, int MM>
+__global__ __launch_bounds__(256) void gemm_smallm_kernel(GemmArgs g) {
+  constexpr int UN = 4;
+  const uint64_t nz = (uint64_t)g.nbo * g.nbi;
+  for (uint64_t z = blockIdx.y; z < nz; z += gridDim.y) {
+    uint64_t zo = z / g.nbi, zi = z - zo * g.nbi;
+    const T* A = (const T*)g.a + zo * g.sao + zi * g.sai;
+    const T* B = (const T*)g.b + zo * g.sbo + zi * g.sbi;
+    T* C = (T*)g.c + zo * g.sco + zi * g.sci;
+    for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < g.n; j += (int64_t)gridDim.x * blockDim.x) {
+      T acc[MM];
+#pragma unroll
+      for (int i = 0; i < MM; ++i) acc[i] = CM<T>::zero();
+      int64_t p = 0;
+      for (; p + UN <= g.k; p += UN) {
+        T b[UN];
+#pragma unroll
+        for (int u = 0; u < UN; ++u) b[u] = B[(p + u) * g.ldb + j];
+#pragma unroll
+        for (int u = 0; u < UN; ++u)
+#pragma unroll
+          for (int i = 0; i < MM; ++i)
+            if (i < g.m) acc[i] = CM<T>::mac(acc[i], A[i * g.lda + p + u], b[u]);
+      }
+      for (; p < g.k; ++p) {
+        T b = B[p * g.ldb + j];
+#pragma unroll
+        for (int i = 0; i < MM; ++i)
+          if (i < g.m) acc[i] = CM<T>::mac(acc[i], A[i * g.lda + p], b);
+      }
+#pragma unroll
+      for (int i = 0; i < MM; ++i)
+        if (i < g.m) C[i * g.ldc + j] = acc[i];
+    }
+  }
+}
+
+// Small n (n <= NN): one thread per output row i keeps all n accumulators;
+// B (k x n) is wave-uniform.
+template <class T, int NN>
+__global__ __launch_bounds__(256) void gemm_smalln_kernel(GemmArgs g) {
+  constexpr int UN = 4;
+  const uint64_t nz = (uint64_t)g.nbo * g.nbi;
+  for (uint64_t z = blockIdx.y; z < nz; z += gridDim.y) {
+    uint64_t zo = z / g.nbi, zi = z - zo * g.nbi;
+    const T* A = (const T*)g.a + zo * g.sao + zi * g.sai;
+    const T* B = (const T*)g.b + zo * g.sbo + zi * g.sbi;
+    T* C = (T*)g.c + zo * g.sco + zi * g.sci;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < g.m; i += (int64_t)gridDim.x * blockDim.x) {
+      T acc[NN];
+#pragma unroll
+      for (int j = 0; j < NN; ++j) acc[j] = CM<T>::zero();
+      const T* Ar = A + i * g.lda;
+      int64_t p = 0;
+      for (; p + UN <= g.k; p += UN) {
+        T a[UN];
+#pragma unroll
+        for (int u = 0; u < UN; ++u) a[u] = Ar[p + u];
+#pragma unroll
+        for (int u = 0; u < UN; ++u)
+#pragma unroll
+          for (int j = 0; j < NN; ++j)
+            if (j < g.n) acc[j] = CM<T>::mac(acc[j], a[u], B[(p + u) * g.ldb + j]);
+      }
+      for (; p < g.k; ++p) {
+        T a = Ar[p];
+#pragma unroll
+        for (int j = 0; j < NN; ++j)
+          if (j < g.n) acc[j] = CM<T>::mac(acc[j], a, B[p * g.ldb + j]);
+      }
+#pragma unroll
+      for (int j = 0; j < NN; ++j)
+        if (j < g.n) C[i * g.ldc + j] = acc[j];
     }
   }
 }
@@ -226,6 +310,212 @@ __global__ __launch_bounds__(256) void gemm_mfma_kernel(GemmArgs g) {
   }
 }
 
+// ------------------------------------------------------ split-precision GEMM --
+// FP32 (or complex64 via the 2x2 expansion of B) products on the 16-bit MFMA:
+// every operand value x is split into hi = round16(x * 2^s) and
+// lo = round16(x * 2^s - hi) while staging into LDS, and
+//   A.B ~= (Ahi.Bhi + Ahi.Blo + Alo.Bhi) * 2^-(sa+sb)
+// accumulated in FP32 (each 16-bit product is exact in FP32; the dropped
+// lo.lo term is ~2^-22 relative for FP16 halves, ~2^-16 for BF16 halves).
+// BF16 keeps FP32's exponent range, so s = 0; FP16 uses a per-GEMM power-of-
+// two scale from the operands' max magnitudes (2^s * max < 2^15).
+//   16x16x32 layout: A frag lane l holds A[l%16][8*(l/16) .. +7], B frag lane l
+//   holds B[8*(l/16) .. +7][l%16]; output register r of lane l is D[4*(l/16)+r][l%16].
+template <bool F16> struct Half;
+template <> struct Half<false> {
+  using H = __bf16;
+  using V = v8bf;
+  static __device__ v4f mfma(V a, V b, v4f c) { return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0); }
+};
+template <> struct Half<true> {
+  using H = _Float16;
+  using V = v8h;
+  static __device__ v4f mfma(V a, V b, v4f c) { return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0); }
+};
+
+template <bool F16>
+__device__ inline void split16(float x, typename Half<F16>::H& hi, typename Half<F16>::H& lo) {
+  using H = typename Half<F16>::H;
+  hi = (H)x;
+  lo = (H)(x - (float)hi);
+}
+
+__device__ inline int pow2_shift(unsigned long long bits) {
+  double m = __longlong_as_double((long long)bits);
+  if (!(m > 0.0) || !isfinite(m)) return 0;
+  int e;
+  frexp(m, &e);  // m = f * 2^e, f in [0.5, 1)
+  return 15 - e;
+}
+
+// Staging: A rows are k-contiguous in LDS (ahi/alo[m][k]); B is stored
+// transposed (bhi/blo[n][k]) so both MFMA fragments are one 16-byte LDS read.
+// VEC: A rows are 16-byte aligned (float4 loads of 4 consecutive k).
+template <bool CPLX, bool F16, bool VEC>
+__global__ __launch_bounds__(256, 2) void gemm_split_kernel(GemmArgs g) {
+  using H = typename Half<F16>::H;
+  using V = typename Half<F16>::V;
+  constexpr int BM = 128, BN = 128, BK = 64;
+  constexpr int WM = 64, WN = 64, TM = 4, TN = 4;
+  constexpr int SK = BK + 8;  // row stride in halves (144 B, 16-byte aligned rows)
+  extern __shared__ __align__(16) char smem_split[];
+  H* ahi = reinterpret_cast<H*>(smem_split);
+  H* alo = ahi + BM * SK;
+  H* bhi = alo + BM * SK;
+  H* blo = bhi + BN * SK;
+
+  float sa = 1.f, sb = 1.f, unscale = 1.f;
+  if constexpr (F16) {
+    const unsigned long long* mb = (const unsigned long long*)g.scratch;
+    int ea = pow2_shift(mb[0]), eb = pow2_shift(mb[1]);
+    sa = ldexpf(1.f, ea), sb = ldexpf(1.f, eb), unscale = ldexpf(1.f, -(ea + eb));
+  }
+  const int64_t M = g.m, N = CPLX ? 2 * g.n : g.n, K = CPLX ? 2 * g.k : g.k;
+  const int64_t lda = CPLX ? 2 * g.lda : g.lda, ldc = CPLX ? 2 * g.ldc : g.ldc, ldb = g.ldb;
+  const int tiles_n = (int)((N + BN - 1) / BN), tiles_m = (int)((M + BM - 1) / BM);
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm0 = (wave >> 1) * WM, wn0 = (wave & 1) * WN;
+  const uint64_t nz = (uint64_t)g.nbo * g.nbi, ntile = (uint64_t)tiles_m * tiles_n;
+  constexpr int A_Q = BM * BK / 4 / 256;  // float4 groups of A per thread (8)
+  // B items per thread: complex -> (complex row pair, complex column); real -> (4 k rows, column)
+  constexpr int B_Q = CPLX ? (BK / 4) * (BN / 2) / 256 : (BK / 4) * BN / 256;  // 4 / 8
+
+  for (uint64_t z = blockIdx.y; z < nz; z += gridDim.y) {
+    uint64_t zo = z / g.nbi, zi = z - zo * g.nbi;
+    const float* A = (const float*)g.a + (zo * g.sao + zi * g.sai) * (CPLX ? 2 : 1);
+    const float* B = (const float*)g.b + (zo * g.sbo + zi * g.sbi) * (CPLX ? 2 : 1);
+    float* C = (float*)g.c + (zo * g.sco + zi * g.sci) * (CPLX ? 2 : 1);
+    for (uint64_t t = blockIdx.x; t < ntile; t += gridDim.x) {
+      const int64_t m0 = (int64_t)(t / tiles_n) * BM, n0 = (int64_t)(t % tiles_n) * BN;
+      v4f acc[TM][TN];
+#pragma unroll
+      for (int i = 0; i < TM; ++i)
+#pragma unroll
+        for (int j = 0; j < TN; ++j) acc[i][j] = v4f{0, 0, 0, 0};
+      float4 ra[A_Q];
+      float4 rb[B_Q];
+      auto load = [&](int64_t k0) {
+#pragma unroll
+        for (int u = 0; u < A_Q; ++u) {
+          int q = tid + 256 * u, row = q / (BK / 4), c4 = (q % (BK / 4)) * 4;
+          int64_t gr = m0 + row, gc = k0 + c4;
+          const float* src = A + gr * lda + gc;
+          if (VEC && gr < M && gc + 3 < K) {
+            ra[u] = *reinterpret_cast<const float4*>(src);
+          } else {
+            ra[u].x = (gr < M && gc < K) ? src[0] : 0.f;
+            ra[u].y = (gr < M && gc + 1 < K) ? src[1] : 0.f;
+            ra[u].z = (gr < M && gc + 2 < K) ? src[2] : 0.f;
+            ra[u].w = (gr < M && gc + 3 < K) ? src[3] : 0.f;
+          }
+        }
+        if constexpr (CPLX) {
+#pragma unroll
+          for (int u = 0; u < B_Q; ++u) {  // complex rows p, p+1 of one complex column j
+            int q = tid + 256 * u, j = q % (BN / 2), pp = q / (BN / 2);
+            int64_t gp = k0 / 2 + 2 * pp, gj = n0 / 2 + j;
+            bool ok0 = gj < g.n && gp < g.k, ok1 = gj < g.n && gp + 1 < g.k;
+            const float* b0 = B + (gp * ldb + gj) * 2;
+            rb[u].x = ok0 ? b0[0] : 0.f;
+            rb[u].y = ok0 ? b0[1] : 0.f;
+            rb[u].z = ok1 ? b0[2 * ldb] : 0.f;
+            rb[u].w = ok1 ? b0[2 * ldb + 1] : 0.f;
+          }
+        } else {
+#pragma unroll
+          for (int u = 0; u < B_Q; ++u) {  // 4 consecutive k rows of one column
+            int q = tid + 256 * u, col = q % BN, r4 = (q / BN) * 4;
+            int64_t gr = k0 + r4, gc = n0 + col;
+            const float* b0 = B + gr * ldb + gc;
+            bool okc = gc < N;
+            rb[u].x = (okc && gr < K) ? b0[0] : 0.f;
+            rb[u].y = (okc && gr + 1 < K) ? b0[ldb] : 0.f;
+            rb[u].z = (okc && gr + 2 < K) ? b0[2 * ldb] : 0.f;
+            rb[u].w = (okc && gr + 3 < K) ? b0[3 * ldb] : 0.f;
+          }
+        }
+      };
+      auto put4 = [&](H* hi, H* lo, int off, float x0, float x1, float x2, float x3) {
+        H h[4], l[4];
+        split16<F16>(x0, h[0], l[0]);
+        split16<F16>(x1, h[1], l[1]);
+        split16<F16>(x2, h[2], l[2]);
+        split16<F16>(x3, h[3], l[3]);
+        *reinterpret_cast<uint2*>(hi + off) = *reinterpret_cast<const uint2*>(h);
+        *reinterpret_cast<uint2*>(lo + off) = *reinterpret_cast<const uint2*>(l);
+      };
+      auto store = [&]() {
+#pragma unroll
+        for (int u = 0; u < A_Q; ++u) {
+          int q = tid + 256 * u, row = q / (BK / 4), c4 = (q % (BK / 4)) * 4;
+          put4(ahi, alo, row * SK + c4, ra[u].x * sa, ra[u].y * sa, ra[u].z * sa, ra[u].w * sa);
+        }
+        if constexpr (CPLX) {
+#pragma unroll
+          for (int u = 0; u < B_Q; ++u) {
+            // Bx^T rows: n' = 2j holds (br, -bi) per complex k, n' = 2j+1 holds (bi, br).
+            int q = tid + 256 * u, j = q % (BN / 2), pp = q / (BN / 2);
+            float br0 = rb[u].x * sb, bi0 = rb[u].y * sb, br1 = rb[u].z * sb, bi1 = rb[u].w * sb;
+            put4(bhi, blo, (2 * j) * SK + 4 * pp, br0, -bi0, br1, -bi1);
+            put4(bhi, blo, (2 * j + 1) * SK + 4 * pp, bi0, br0, bi1, br1);
+          }
+        } else {
+#pragma unroll
+          for (int u = 0; u < B_Q; ++u) {
+            int q = tid + 256 * u, col = q % BN, r4 = (q / BN) * 4;
+            put4(bhi, blo, col * SK + r4, rb[u].x * sb, rb[u].y * sb, rb[u].z * sb, rb[u].w * sb);
+          }
+        }
+      };
+      const int64_t nk = (K + BK - 1) / BK;
+      load(0);
+      store();
+      __syncthreads();
+      const int cl = lane & 15;
+      for (int64_t kt = 0; kt < nk; ++kt) {
+        if (kt + 1 < nk) load((kt + 1) * BK);
+#pragma unroll
+        for (int ks = 0; ks < BK; ks += 32) {
+          const int kg = ks + (lane >> 4) * 8;
+          V ah[TM], al[TM];
+#pragma unroll
+          for (int i = 0; i < TM; ++i) {
+            ah[i] = *reinterpret_cast<const V*>(&ahi[(wm0 + i * 16 + cl) * SK + kg]);
+            al[i] = *reinterpret_cast<const V*>(&alo[(wm0 + i * 16 + cl) * SK + kg]);
+          }
+#pragma unroll
+          for (int j = 0; j < TN; ++j) {
+            V bh = *reinterpret_cast<const V*>(&bhi[(wn0 + j * 16 + cl) * SK + kg]);
+            V bl = *reinterpret_cast<const V*>(&blo[(wn0 + j * 16 + cl) * SK + kg]);
+#pragma unroll
+            for (int i = 0; i < TM; ++i) {
+              acc[i][j] = Half<F16>::mfma(al[i], bh, acc[i][j]);
+              acc[i][j] = Half<F16>::mfma(ah[i], bl, acc[i][j]);
+              acc[i][j] = Half<F16>::mfma(ah[i], bh, acc[i][j]);
+            }
+          }
+        }
+        __syncthreads();
+        if (kt + 1 < nk) {
+          store();
+          __syncthreads();
+        }
+      }
+#pragma unroll
+      for (int i = 0; i < TM; ++i)
+#pragma unroll
+        for (int j = 0; j < TN; ++j)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            int64_t row = m0 + wm0 + i * 16 + 4 * (lane >> 4) + r, col = n0 + wn0 + j * 16 + cl;
+            if (row < M && col < N) C[row * ldc + col] = acc[i][j][r] * unscale;
+          }
+    }
+  }
+}
+
+constexpr size_t SPLIT_LDS = 4 * 128 * (64 + 8) * 2;
+
 // ------------------------------------------------------------------ dispatch --
 template <class F>
 static void dispatch_dt(int dt, F&& f) {
@@ -272,13 +562,99 @@ static bool mfma_worth(int dt, const GemmArgs& g) {
   return g.m >= 32 && N >= 32 && K >= 16;
 }
 
+template <bool SMALL_M, int X>
+static void run_skinny_t(int dt, const GemmArgs& g, hipStream_t s) {
+  dispatch_dt(dt, [&](auto tag) {
+    using T = decltype(tag);
+    int64_t width = SMALL_M ? g.n : g.m;
+    unsigned gx = (unsigned)std::min<int64_t>((width + 255) / 256, 65536);
+    dim3 grid(gx, zgrid(g));
+    if ((uint64_t)grid.y * gx > (1u << 22)) grid.y = std::max<unsigned>(1, (1u << 22) / gx);
+    if constexpr (SMALL_M) LAUNCH((gemm_smallm_kernel<T, X>), grid, dim3(256), 0, s, g);
+    else LAUNCH((gemm_smalln_kernel<T, X>), grid, dim3(256), 0, s, g);
+  });
+}
+
+static bool run_skinny(int dt, const GemmArgs& g, hipStream_t s) {
+  if (g.m <= 32 && g.n >= 64) {
+    g_last_kind = K_GEMM_SMALL;
+    if (g.m <= 4) run_skinny_t<true, 4>(dt, g, s);
+    else if (g.m <= 8) run_skinny_t<true, 8>(dt, g, s);
+    else if (g.m <= 16) run_skinny_t<true, 16>(dt, g, s);
+    else run_skinny_t<true, 32>(dt, g, s);
+    return true;
+  }
+  if (g.n <= 32 && g.m >= 64) {
+    g_last_kind = K_GEMM_SMALL;
+    if (g.n <= 4) run_skinny_t<false, 4>(dt, g, s);
+    else if (g.n <= 8) run_skinny_t<false, 8>(dt, g, s);
+    else if (g.n <= 16) run_skinny_t<false, 16>(dt, g, s);
+    else run_skinny_t<false, 32>(dt, g, s);
+    return true;
+  }
+  return false;
+}
+
 int launch_gemm(int dt, int prec, int kernel, const GemmArgs& g, hipStream_t s) {
-  if (kernel == GEMM_AUTO) kernel = mfma_worth(dt, g) ? GEMM_MFMA : GEMM_SIMPLE;
+  if (kernel == GEMM_AUTO) {
+    if (mfma_worth(dt, g)) kernel = GEMM_MFMA;
+    else if ((g.m <= 32 && g.n >= 64) || (g.n <= 32 && g.m >= 64)) kernel = GEMM_SKINNY;
+    else kernel = GEMM_SIMPLE;
+  }
+  if (kernel == GEMM_SKINNY) {
+    if (run_skinny(dt, g, s)) return GEMM_SKINNY;
+    kernel = GEMM_SIMPLE;
+  }
   if (kernel == GEMM_SIMPLE) {
     run_simple(dt, g, s);
     return GEMM_SIMPLE;
   }
-  (void)prec;
+  if ((prec == BF16X3 || prec == FP16X2) && (dt == F32 || dt == C64)) {
+    int64_t N = dt == C64 ? 2 * g.n : g.n;
+    uint64_t tiles = (uint64_t)((g.m + 127) / 128) * (uint64_t)((N + 127) / 128);
+    dim3 grid((unsigned)std::min<uint64_t>(tiles, 1u << 20), zgrid(g));
+    if (prec == FP16X2) {
+      if (!g.scratch) throw Error("FP16X2 GEMM needs 16 bytes of device scratch");
+      // operand max magnitudes -> per-GEMM power-of-two scales
+      int64_t na = (g.sao ? g.nbo : 1) * (g.sai ? g.nbi : 1) * g.m * g.k;
+      int64_t nb = (g.sbo ? g.nbo : 1) * (g.sbi ? g.nbi : 1) * g.k * g.n;
+      launch_fill_zero(g.scratch, 16, s);
+      launch_maxabs(dt, g.a, (size_t)na, (uint64_t*)g.scratch, s);
+      launch_maxabs(dt, g.b, (size_t)nb, (uint64_t*)g.scratch + 1, s);
+      g_last_kind = K_GEMM_MFMA_FP16X2;
+    } else {
+      g_last_kind = K_GEMM_MFMA_BF16X3;
+    }
+    int64_t cf = dt == C64 ? 2 : 1;
+    bool vec = (cf * g.lda) % 4 == 0 && (uintptr_t)g.a % 16 == 0 && (cf * g.sai) % 4 == 0 && (cf * g.sao) % 4 == 0;
+    bool f16 = prec == FP16X2, cx = dt == C64;
+    // 72 KiB of dynamic LDS (gfx950 has 160 KiB per CU): raise the per-kernel limit once.
+#define SPLIT_LAUNCH(C_, F_, V_)                                                                          \
+  do {                                                                                                   \
+    static bool attr_set = false;                                                                        \
+    if (!attr_set) {                                                                                     \
+      ATN_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&gemm_split_kernel<C_, F_, V_>),         \
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)SPLIT_LDS));          \
+      attr_set = true;                                                                                   \
+    }                                                                                                    \
+    LAUNCH((gemm_split_kernel<C_, F_, V_>), grid, dim3(256), SPLIT_LDS, s, g);                            \
+  } while (0)
+    if (cx && f16) {
+      if (vec) SPLIT_LAUNCH(true, true, true);
+      else SPLIT_LAUNCH(true, true, false);
+    } else if (cx) {
+      if (vec) SPLIT_LAUNCH(true, false, true);
+      else SPLIT_LAUNCH(true, false, false);
+    } else if (f16) {
+      if (vec) SPLIT_LAUNCH(false, true, true);
+      else SPLIT_LAUNCH(false, true, false);
+    } else {
+      if (vec) SPLIT_LAUNCH(false, false, true);
+      else SPLIT_LAUNCH(false, false, false);
+    }
+#undef SPLIT_LAUNCH
+    return GEMM_MFMA;
+  }
   switch (dt) {
     case F32: run_mfma<float, false, 128, 128, 16>(g, s); break;
     case C64: run_mfma<float, true, 128, 128, 16>(g, s); break;

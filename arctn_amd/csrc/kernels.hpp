@@ -20,6 +20,7 @@ struct PermTile {
   int64_t hin_in[MAXT], hout_out[MAXT];
   int32_t hin_lds[MAXT], gin_lds[MAXT];
   uint32_t u_size, lo_in, lo_out, n_hi_in, n_hi_out;
+  int32_t sh_in, sh_out;  // log2 of lo_in / lo_out when powers of two, else -1
   uint64_t n_tiles;
 };
 struct PermGeneric {
@@ -99,8 +100,10 @@ struct GemmArgs {
   int64_t lda, ldb, ldc;       // row strides in elements
   int64_t sao, sai, sbo, sbi, sco, sci;  // batch strides (outer slice batch, inner plan batch)
   int64_t nbo, nbi;
+  // FP16X2 only: 16 bytes of device scratch for the operands' max-magnitude bits.
+  void* scratch = nullptr;
 };
-enum GemmKernel { GEMM_AUTO = -1, GEMM_SIMPLE = 0, GEMM_MFMA = 1, GEMM_MFMA_SMALL = 2 };
+enum GemmKernel { GEMM_AUTO = -1, GEMM_SIMPLE = 0, GEMM_MFMA = 1, GEMM_SKINNY = 2 };
 // Returns the kernel used.
 int launch_gemm(int dt, int prec, int kernel, const GemmArgs& g, hipStream_t s);
 // Conjugating variants for the adjoint: opA/opB: 0 = N, 1 = conj-transpose (as stored).
@@ -109,7 +112,7 @@ void launch_gemm_adjoint(int dt, int opa, int opb, const GemmArgs& g, hipStream_
 extern thread_local size_t g_launch_count;
 // Kind of the last kernel launched on this thread (profiling labels).
 enum KernelKind { K_OTHER = 0, K_PERMUTE_TILE, K_PERMUTE_GENERIC, K_SUM, K_TRACE_DIAG, K_GATHER, K_REDUCE, K_SCATTER,
-                  K_GEMM_SIMPLE, K_GEMM_MFMA_F32, K_GEMM_MFMA_F64, K_GEMM_MFMA_BF16X3, K_GEMM_MFMA_FP16X2, K_GEMM_MFMA_C3M,
+                  K_GEMM_SIMPLE, K_GEMM_SMALL, K_GEMM_MFMA_F32, K_GEMM_MFMA_F64, K_GEMM_MFMA_BF16X3, K_GEMM_MFMA_FP16X2, K_GEMM_MFMA_C3M,
                   K_STRIP, K_COPY, K_NKINDS };
 extern thread_local int g_last_kind;
 const char* kernel_kind_name(int k);

@@ -14,7 +14,7 @@ thread_local int g_last_kind = K_OTHER;
 
 const char* kernel_kind_name(int k) {
   static const char* names[K_NKINDS] = {"other", "permute_tile", "permute_generic", "sum_axis", "trace_diag", "slice_gather",
-                                        "slice_reduce_f64", "output_scatter", "gemm_simple", "gemm_mfma_f32",
+                                        "slice_reduce_f64", "output_scatter", "gemm_simple", "gemm_skinny", "gemm_mfma_f32",
                                         "gemm_mfma_f64", "gemm_mfma_bf16x3", "gemm_mfma_fp16x2", "gemm_mfma_c3m",
                                         "strip", "copy"};
   return (k >= 0 && k < K_NKINDS) ? names[k] : "other";
@@ -85,7 +85,9 @@ static unsigned grid_for(uint64_t work, unsigned threads, unsigned cap = 65536u)
 // innermost axes (G_in) contiguous across lanes and scatters into LDS laid out
 // in output order; phase 2 streams LDS out with the output's innermost axes
 // (G_out) contiguous. Offset tables for the non-contiguous part of each phase
-// are built once per workgroup in LDS.
+// are built once per workgroup in LDS. Every LDS row of lo_out elements is
+// padded by one element, so the transposed scatter of phase 1 (lanes writing
+// consecutive rows) spreads over the banks instead of hitting one.
 template <int ES>
 __global__ __launch_bounds__(256) void permute_tile_kernel(PermTile a, const typename Bits<ES>::T* __restrict__ in,
                                                            typename Bits<ES>::T* __restrict__ out) {
@@ -134,24 +136,57 @@ __global__ __launch_bounds__(256) void permute_tile_kernel(PermTile a, const typ
   __syncthreads();
 
   for (uint64_t t = blockIdx.x; t < a.n_tiles; t += gridDim.x) {
-    uint64_t r = t;
     int64_t bin = 0, bout = 0;
-    for (int d = a.n_r - 1; d >= 0; --d) {
-      uint64_t dim = a.r_dims[d], q = r / dim, i = r - q * dim;
-      r = q;
-      bin += (int64_t)i * a.r_in[d];
-      bout += (int64_t)i * a.r_out[d];
+    if (a.n_tiles <= 0xffffffffull) {
+      uint32_t r = (uint32_t)t;
+      for (int d = a.n_r - 1; d >= 0; --d) {
+        uint32_t dim = a.r_dims[d], q = r / dim, i = r - q * dim;
+        r = q;
+        bin += (int64_t)i * a.r_in[d];
+        bout += (int64_t)i * a.r_out[d];
+      }
+    } else {
+      uint64_t r = t;
+      for (int d = a.n_r - 1; d >= 0; --d) {
+        uint64_t dim = a.r_dims[d], q = r / dim, i = r - q * dim;
+        r = q;
+        bin += (int64_t)i * a.r_in[d];
+        bout += (int64_t)i * a.r_out[d];
+      }
     }
     const E* src = in + bin;
     E* dst = out + bout;
-    for (uint32_t j = threadIdx.x; j < a.u_size; j += blockDim.x) {
-      uint32_t hi = j / a.lo_in, lo = j - hi * a.lo_in;
-      tile[lds_lo[lo] + lds_hi[hi]] = src[in_hi[hi] + lo];
+    // Eight independent global loads in flight per lane before the LDS scatter.
+    constexpr int UN = 8;
+    for (uint32_t j0 = threadIdx.x; j0 < a.u_size; j0 += UN * blockDim.x) {
+      E v[UN];
+      int32_t d[UN];
+#pragma unroll
+      for (int q = 0; q < UN; ++q) {
+        uint32_t j = j0 + q * blockDim.x;
+        if (j < a.u_size) {
+          uint32_t hi = a.sh_in >= 0 ? j >> a.sh_in : j / a.lo_in;
+          uint32_t lo = j - hi * a.lo_in;
+          v[q] = src[in_hi[hi] + lo];
+          uint32_t jo = lds_lo[lo] + lds_hi[hi];  // output-order position in U
+          d[q] = jo + (a.sh_out >= 0 ? jo >> a.sh_out : jo / a.lo_out);
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < UN; ++q)
+        if (j0 + q * blockDim.x < a.u_size) tile[d[q]] = v[q];
     }
     __syncthreads();
-    for (uint32_t j = threadIdx.x; j < a.u_size; j += blockDim.x) {
-      uint32_t hi = j / a.lo_out, lo = j - hi * a.lo_out;
-      dst[out_hi[hi] + lo] = tile[j];
+    for (uint32_t j0 = threadIdx.x; j0 < a.u_size; j0 += UN * blockDim.x) {
+#pragma unroll
+      for (int q = 0; q < UN; ++q) {
+        uint32_t j = j0 + q * blockDim.x;
+        if (j < a.u_size) {
+          uint32_t hi = a.sh_out >= 0 ? j >> a.sh_out : j / a.lo_out;
+          uint32_t lo = j - hi * a.lo_out;
+          dst[out_hi[hi] + lo] = tile[j + hi];
+        }
+      }
     }
     __syncthreads();
   }
@@ -160,15 +195,28 @@ __global__ __launch_bounds__(256) void permute_tile_kernel(PermTile a, const typ
 template <int ES>
 __global__ __launch_bounds__(256) void permute_generic_kernel(PermGeneric a, const typename Bits<ES>::T* __restrict__ in,
                                                               typename Bits<ES>::T* __restrict__ out) {
-  for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e < a.numel; e += (uint64_t)gridDim.x * blockDim.x) {
-    uint64_t r = e;
-    int64_t off = 0;
-    for (int d = a.rank - 1; d >= 0; --d) {
-      uint64_t dim = a.dims[d], q = r / dim;
-      off += (int64_t)(r - q * dim) * a.in_str[d];
-      r = q;
+  using E = typename Bits<ES>::T;
+  constexpr int UN = 4;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t e0 = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e0 < a.numel; e0 += UN * stride) {
+    E v[UN];
+#pragma unroll
+    for (int q = 0; q < UN; ++q) {
+      uint64_t e = e0 + q * stride;
+      if (e < a.numel) {
+        uint64_t r = e;
+        int64_t off = 0;
+        for (int d = a.rank - 1; d >= 0; --d) {
+          uint64_t dim = a.dims[d], qq = r / dim;
+          off += (int64_t)(r - qq * dim) * a.in_str[d];
+          r = qq;
+        }
+        v[q] = in[off];
+      }
     }
-    out[e] = in[off];
+#pragma unroll
+    for (int q = 0; q < UN; ++q)
+      if (e0 + q * stride < a.numel) out[e0 + q * stride] = v[q];
   }
 }
 
@@ -281,20 +329,48 @@ PermPlan plan_permute(const std::vector<size_t>& shape0, const std::vector<size_
     p.threads = 256;
   };
 
+  // Tile budget: the LDS tile holds at most this many elements (32 KiB).
+  const size_t umax = (size_t)32768 / (size_t)es;
   for (size_t tmin : {64, 32, 16, 8}) {
     // G_in: innermost input axes with product >= tmin; G_out likewise in output order.
     std::vector<bool> in_gin(r, false), in_gout(r, false);
     size_t prodin = 1, prodout = 1;
-    for (size_t a = r; a-- > 0 && prodin < tmin;) in_gin[a] = true, prodin *= axes[a].dim;
-    for (size_t d = r; d-- > 0 && prodout < tmin;) in_gout[by_out[d]] = true, prodout *= axes[by_out[d]].dim;
-    size_t u = 1;
+    size_t ni = 0, no = 0;  // number of input-/output-innermost axes taken
+    for (size_t a = r; a-- > 0 && prodin < tmin;) in_gin[a] = true, prodin *= axes[a].dim, ++ni;
+    for (size_t d = r; d-- > 0 && prodout < tmin;) in_gout[by_out[d]] = true, prodout *= axes[by_out[d]].dim, ++no;
+    auto usize = [&]() {
+      size_t u = 1;
+      for (size_t a = 0; a < r; ++a)
+        if (in_gin[a] || in_gout[a]) u *= axes[a].dim;
+      return u;
+    };
+    // Grow both groups while the tile fits: longer contiguous runs on both
+    // sides and more work per tile (small tiles are dominated by per-tile
+    // index decoding and barriers).
+    for (bool grew = true; grew;) {
+      grew = false;
+      if (ni < r) {
+        size_t a = r - 1 - ni;
+        in_gin[a] = true;
+        if (usize() <= umax) prodin *= axes[a].dim, ++ni, grew = true;
+        else in_gin[a] = false;
+      }
+      if (no < r) {
+        size_t a = by_out[r - 1 - no];
+        bool was = in_gout[a];
+        in_gout[a] = true;
+        if (usize() <= umax) prodout *= axes[a].dim, ++no, grew = true;
+        else in_gout[a] = was;
+      }
+    }
+    size_t u = usize();
     int nu = 0;
     for (size_t a = 0; a < r; ++a)
-      if (in_gin[a] || in_gout[a]) u *= axes[a].dim, ++nu;
+      if (in_gin[a] || in_gout[a]) ++nu;
     size_t lo_in = prodin, lo_out = prodout;
     size_t n_hi_in = u / lo_in, n_hi_out = u / lo_out;
     size_t tab = (n_hi_in + n_hi_out) * 8 + (n_hi_in + lo_in) * 4;
-    size_t lds = ((tab + 15) & ~(size_t)15) + u * (size_t)es;
+    size_t lds = ((tab + 15) & ~(size_t)15) + (u + n_hi_out) * (size_t)es;
     int n_gin = 0, n_gout = 0;
     for (size_t a = 0; a < r; ++a) n_gin += in_gin[a], n_gout += in_gout[a];
     if (lds > 64 * 1024 || nu - n_gin > MAXT || n_gin > MAXT || nu - n_gout > MAXT || r - nu + 1 > (size_t)MAXR) continue;
@@ -350,6 +426,9 @@ PermPlan plan_permute(const std::vector<size_t>& shape0, const std::vector<size_
     t.lo_out = (uint32_t)lo_out;
     t.n_hi_in = (uint32_t)n_hi_in;
     t.n_hi_out = (uint32_t)n_hi_out;
+    auto lg = [](size_t v) { return (v & (v - 1)) == 0 ? (int32_t)__builtin_ctzll(v) : -1; };
+    t.sh_in = lg(lo_in);
+    t.sh_out = lg(lo_out);
     t.n_tiles = p.numel / u;
     p.kind = 1;
     p.lds = lds;
@@ -374,7 +453,7 @@ void launch_permute(const PermPlan& p, const void* in, void* out, hipStream_t s)
       LAUNCH(permute_tile_kernel<ES>, grid, p.threads, p.lds, s, p.tile, (const E*)in, (E*)out);
     } else {
       g_last_kind = K_PERMUTE_GENERIC;
-      LAUNCH(permute_generic_kernel<ES>, grid_for(p.gen.numel, 256), 256, 0, s, p.gen, (const E*)in, (E*)out);
+      LAUNCH(permute_generic_kernel<ES>, grid_for((p.gen.numel + 3) / 4, 256), 256, 0, s, p.gen, (const E*)in, (E*)out);
     }
   });
 }

@@ -81,17 +81,24 @@ def test_reductions_and_gathers_bit_exact(cuda, dtype):
 
 
 GEMM_SHAPES = [(1, 1, 1, 1), (1, 3, 4, 5), (2, 17, 33, 9), (1, 128, 64, 128), (3, 130, 70, 129), (1, 256, 256, 256),
-               (1, 1000, 48, 300), (1, 16, 4, 4096), (4, 64, 512, 64)]
+               (1, 1000, 48, 300), (1, 16, 4, 4096), (4, 64, 512, 64), (1, 8, 256, 1000), (2, 1000, 16, 8),
+               (1, 3, 5, 100), (1, 200, 7, 3), (1, 32, 128, 2000)]
+# (dtype, precision code, per-product relative error bound)
+PREC_CASES = [(np.float32, 0, 1e-6), (np.float64, 0, 1e-14), (np.complex64, 0, 1e-6), (np.complex128, 0, 1e-14),
+              (np.float32, 2, 1e-5), (np.complex64, 2, 1e-5), (np.float32, 3, 1e-6), (np.complex64, 3, 1e-6)]
 
 
-@pytest.mark.parametrize("dtype", DTYPES)
-@pytest.mark.parametrize("kernel", [0, 1])
-def test_gemm_against_numpy(cuda, dtype, kernel):
+@pytest.mark.parametrize("case", range(len(PREC_CASES)))
+@pytest.mark.parametrize("kernel", [0, 1, 2])
+def test_gemm_against_numpy(cuda, case, kernel):
+    """Simple (0), MFMA (1, native or split precision) and skinny (2) GEMMs vs complex128 NumPy."""
     import torch
     from arctn_amd import _lib
 
+    dtype, prec, eps = PREC_CASES[case]
+    if prec and kernel != 1:
+        pytest.skip("split precision applies to the MFMA kernel")
     code = {np.float32: 0, np.float64: 1, np.complex64: 2, np.complex128: 3}[dtype]
-    eps = {np.float32: 1e-6, np.complex64: 1e-6, np.float64: 1e-14, np.complex128: 1e-14}[dtype]
     rng = np.random.default_rng(5)
     for bt, m, k, n in GEMM_SHAPES:
         a = _rand((bt, m, k), dtype, rng)
@@ -99,7 +106,7 @@ def test_gemm_against_numpy(cuda, dtype, kernel):
         want = np.matmul(a.astype(np.complex128), b.astype(np.complex128))
         ta, tb = torch.from_numpy(a).cuda(), torch.from_numpy(b).cuda()
         tc = torch.empty((bt, m, n), dtype=ta.dtype, device="cuda")
-        _lib.check(_lib.lib().atn_gemm(code, 0, kernel, bt, m, k, n, ta.data_ptr(), m * k, tb.data_ptr(), k * n,
+        _lib.check(_lib.lib().atn_gemm(code, prec, kernel, bt, m, k, n, ta.data_ptr(), m * k, tb.data_ptr(), k * n,
                                        tc.data_ptr(), m * n, None))
         got = tc.cpu().numpy()
         scale = np.sqrt(k) * max(1.0, np.abs(want).max())
