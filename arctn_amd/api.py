@@ -140,6 +140,10 @@ class Plan:
     def handle(self):
         return self._h
 
+    def release(self):
+        """Destroy cached executors (frees their device workspaces)."""
+        self._executors.clear()
+
     def info(self):
         i = _lib.PlanInfo()
         _lib.check(_lib.lib().atn_plan_stats(self._h, _lib.C.byref(i)))

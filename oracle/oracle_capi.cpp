@@ -137,10 +137,13 @@ int orc_compile_describe(const orc_net* d, char** json_out) {
     s += "],\"finalize\":{\"slot\":" + std::to_string(f.slot) + ",\"input_legs\":" + legs_json(f.input_legs) +
          ",\"input_shape\":" + sz_json(f.input_shape) + ",\"sum\":" + sz_json(f.sum) + ",\"perm\":" + sz_json(f.perm) +
          ",\"output_shape\":" + sz_json(f.output_shape) + "},";
-    std::snprintf(buf, sizeof buf, "%.17g", c.log10_flops);
-    s += std::string("\"log10_flops\":") + buf + ",";
-    std::snprintf(buf, sizeof buf, "%.17g", c.log2_peak_size);
-    s += std::string("\"log2_peak_size\":") + buf + "}";
+    auto num = [&](double v) {  // JSON has no infinities: an empty path has log10 FLOPs -inf
+      if (!std::isfinite(v)) return std::string("null");
+      std::snprintf(buf, sizeof buf, "%.17g", v);
+      return std::string(buf);
+    };
+    s += std::string("\"log10_flops\":") + num(c.log10_flops) + ",";
+    s += std::string("\"log2_peak_size\":") + num(c.log2_peak_size) + "}";
     *json_out = static_cast<char*>(std::malloc(s.size() + 1));
     std::memcpy(*json_out, s.c_str(), s.size() + 1);
   });

@@ -1,0 +1,160 @@
+"""C-ABI boundary and host logic, without a GPU.
+
+The library must load and export every entry point include/arctn_hip.h
+declares; plan compilation (host-only) must reproduce the oracle's compiled
+plan exactly; malformed plans must be rejected with the reference's error
+behaviour before any device work.
+"""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+from arctn_amd import Plan
+from arctn_amd import _lib
+from arctn_amd import planner as P
+from arctn_amd import workloads as W
+from oracle import pyoracle as O
+from tests.golden import fixtures as F
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+FX = F.load()
+
+
+def _declared():
+    text = open(os.path.join(ROOT, "include", "arctn_hip.h")).read()
+    return sorted(set(re.findall(r"^\s*(?:int|void|const char\*)\s+(atn_\w+)\s*\(", text, re.M)))
+
+
+def test_library_exports_every_declared_symbol():
+    L = _lib.lib()
+    names = _declared()
+    assert len(names) >= 20
+    for name in names:
+        assert hasattr(L, name), name
+    assert L.atn_abi_version() == 1
+
+
+def test_ctypes_signatures_cover_the_header():
+    assert set(_declared()) == set(_lib._SIGS)
+
+
+def _strip(d):
+    d = dict(d)
+    d.pop("varying", None)
+    return d
+
+
+def test_plan_describe_matches_oracle_compile():
+    cases = [(F.net_of(FX["mixed_network"]), [tuple(p) for p in FX["mixed_network"]["hand_path"]]),
+             (F.net_of(FX["two_node_geometry"]), [(0, 1)]),
+             (F.net_of(FX["single_tensor"]), [])]
+    for seed in range(20):
+        rng = np.random.default_rng(seed)
+        n = int(rng.integers(2, 9))
+        net = W.random_connected(n, int(rng.integers(0, n)), [2, 3, 4], int(rng.integers(0, 3)), seed)
+        cases.append((net, P.greedy(net)))
+    net, _, _ = W.random_circuit_amplitude(4, 4, 8, 0)
+    cases.append((net, P.greedy(net)))
+    for net, path in cases:
+        mine = Plan(net["inputs"], net["output"], net["size_dict"], path).describe()
+        ref = O.describe(net, path)
+        assert mine["n_inputs"] == ref["n_inputs"]
+        assert mine["preps"] == ref["preps"]
+        assert [_strip(s) for s in mine["steps"]] == ref["steps"]
+        assert mine["finalize"] == ref["finalize"]
+
+
+def test_known_geometry_and_stats():
+    fx = FX["two_node_geometry"]
+    p = Plan(fx["inputs"], fx["output"], F.net_of(fx)["size_dict"], fx["ssa_path"])
+    st = p.describe()["steps"][0]
+    assert st["gemm"] == fx["gemm_dims"] and st["result_legs"] == fx["result_legs"]
+    fx = FX["liveness_peak"]
+    info = Plan(fx["inputs"], fx["output"], F.net_of(fx)["size_dict"], fx["ssa_path"]).info()
+    assert info["log2_peak_size"] == fx["log2_peak_size"]
+    fx = FX["chain_costs"]
+    info = Plan(fx["inputs"], fx["output"], F.net_of(fx)["size_dict"], fx["ssa_path"]).info()
+    assert abs(info["log10_flops"] - np.log10(fx["flops"])) < 1e-12
+
+
+def test_slice_bookkeeping_matches_reference_rules():
+    fx = FX["mixed_ragged_chunks"]
+    net = F.net_of(fx)
+    d = Plan(net["inputs"], net["output"], net["size_dict"], fx["ssa_path"], fx["sliced"]).describe()
+    # output leg 0 is an outer (block) leg, 1 is internal; chunking follows slice.rs:893-905
+    assert d["outer"] == [0] and d["internal"] == [1] and d["n_blocks"] == 2 and d["n_slices"] == 514
+    assert d["n_chunks"] == O.slice_parallel_chunks(257, 3) == 129
+    # internal slicing removes the sliced legs from every per-slice operand
+    fx = FX["shared_output_hyperedge"]
+    net = F.net_of(fx)
+    d = Plan(net["inputs"], net["output"], net["size_dict"], fx["ssa_path"], [4, 2]).describe()
+    assert all(4 not in s["left_legs"] + s["right_legs"] for s in d["steps"])
+    assert d["outer"] == [2] and d["internal"] == [4]
+
+
+@pytest.mark.parametrize("bad,match", [
+    (dict(sliced=[0, 0]), "repeated"),
+    (dict(path=[(0, 0)]), "itself"),
+    (dict(path=[(0, 5)]), "invalid"),
+    (dict(path=[]), "incomplete"),
+    (dict(size_dict={0: 2, 1: 0, 2: 2}), "dimension 0"),
+])
+def test_malformed_plans_are_rejected(bad, match):
+    inputs, output = [[0, 1], [1, 2]], [0, 2]
+    size_dict = bad.get("size_dict", {0: 2, 1: 2, 2: 2})
+    path = bad.get("path", [(0, 1)])
+    with pytest.raises(ValueError, match=match):
+        Plan(inputs, output, size_dict, path, bad.get("sliced", ()))
+
+
+def test_trace_leg_cannot_be_sliced():
+    with pytest.raises(ValueError, match="同一输入张量中重复"):
+        Plan([[0, 0]], [], {0: 2}, [], [0])
+
+
+def test_unused_size_dict_leg_rejected():
+    with pytest.raises(ValueError, match="size_dict"):
+        Plan([[0], [0]], [], {0: 2, 1: 2}, [(0, 1)])
+
+
+def test_executor_options_validated_before_device_work():
+    L = _lib.lib()
+    p = Plan([[0], [0]], [], {0: 2}, [(0, 1)], [0])
+    o = _lib.ExecOptions(2, 0, 0, 0, 0, 1, 0, 0)  # strip on a sliced plan
+    h = C.c_void_p()
+    rc = L.atn_executor_create(p.handle, C.byref(o), C.byref(h))
+    assert rc != 0 and b"unsliced" in L.atn_last_error()
+    o = _lib.ExecOptions(1, 2, 0, 0, 0, 0, 0, 0)  # split precision on float64
+    rc = L.atn_executor_create(p.handle, C.byref(o), C.byref(h))
+    assert rc != 0 and b"split precision" in L.atn_last_error()
+
+
+def test_linear_and_ssa_paths_round_trip():
+    for seed in range(10):
+        net = W.random_connected(6, 3, [2, 3], 1, seed)
+        ssa = P.greedy(net)
+        assert P.linear_to_ssa(P.ssa_to_linear(ssa, 6), 6) == [tuple(p) for p in ssa]
+
+
+def test_planner_simulation_agrees_with_oracle():
+    net, _, _ = W.random_circuit_amplitude(4, 4, 8, 0)
+    path = P.greedy(net)
+    st, _ = P.simulate_path_full(net, path)
+    ref = O.simulate(net, path)
+    assert abs(st.log10_flops - ref["log10_flops"]) < 1e-12
+    assert abs(st.log2_peak_size - ref["log2_peak_size"]) < 1e-9
+
+
+def test_slice_finder_meets_target():
+    net, _, _ = W.random_circuit_amplitude(4, 4, 8, 0)
+    path = P.greedy(net)
+    for target in (2 ** 6, 2 ** 8, 2 ** 10):
+        sl = P.find_slices_to_size(net, path, target)
+        sd = dict(net["size_dict"])
+        for l in sl:
+            sd[l] = 1
+        _, legs = P.simulate_path_full(dict(net, size_dict=sd), path)
+        assert all(np.prod([sd[l] for l in ls]) <= target for ls in legs)
