@@ -99,7 +99,8 @@ def _device_inputs(net_inputs, size_dict, arrays, context):
     out = []
     for a in arrays:
         if isinstance(a, torch.Tensor):
-            t = a.to(dev)
+            # materialise lazy conjugate / negative views before handing raw pointers over
+            t = a.resolve_conj().resolve_neg().to(dev)
         else:
             a = np.asarray(a)
             # np.ascontiguousarray would turn a rank-0 array into shape (1,)

@@ -258,3 +258,47 @@ def random_arrays(net, dtype, seed):
             a = a + 1j * rng.uniform(-1.0, 1.0, size=shape)
         out.append(np.array(a, dtype=dtype, order="C"))
     return out
+
+
+def brickwork_torch_tensors(slots, angles):
+    """Input tensors of the doubled brickwork network as torch tensors built
+    from ``angles`` (shape (layers, n_qubits, 2): RY angle, RZ angle), so that
+    gradients flow back to the angles through torch's autograd."""
+    import torch
+
+    cdt = torch.complex128 if angles.dtype == torch.float64 else torch.complex64
+    dev = angles.device
+    one = torch.ones((), dtype=angles.dtype, device=dev)
+    zero = torch.zeros((), dtype=angles.dtype, device=dev)
+    ket0 = torch.tensor([1.0, 0.0], dtype=cdt, device=dev)
+    cz = torch.diag(torch.tensor([1, 1, 1, -1], dtype=cdt, device=dev)).reshape(2, 2, 2, 2)
+    z = torch.diag(torch.tensor([1.0, -1.0], dtype=cdt, device=dev))
+    eye = torch.eye(2, dtype=cdt, device=dev)
+    cache = {}
+
+    def gate(layer, q):
+        key = (layer, q)
+        if key not in cache:
+            th, ph = angles[layer, q, 0], angles[layer, q, 1]
+            c, s = torch.cos(th / 2), torch.sin(th / 2)
+            ry = torch.stack([torch.stack([c, -s]), torch.stack([s, c])]).to(cdt)
+            e = torch.exp(torch.complex(zero, -0.5 * ph * one))
+            rz = torch.diag(torch.stack([e, e.conj()]))
+            cache[key] = rz @ ry
+        return cache[key]
+
+    out = []
+    for s in slots:
+        if s[0] == "ket0":
+            out.append(ket0)
+        elif s[0] == "u":
+            u = gate(s[1], s[2])
+            # resolve_conj materialises torch's lazy conjugate view
+            out.append(u.conj().resolve_conj() if s[3] else u)
+        elif s[0] == "cz":
+            out.append(cz)
+        elif s[0] == "z":
+            out.append(z)
+        else:
+            out.append(eye)
+    return [t.contiguous() for t in out]
