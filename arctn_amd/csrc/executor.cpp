@@ -424,7 +424,7 @@ Executor::Executor(std::shared_ptr<const Plan> plan, const ExecOptions& o) : P_(
   const Plan& P = *P_;
   es_ = (int)dtype_size(o_.dtype);
   if (o_.precision != NATIVE) {
-    if (o_.precision == C3M && o_.dtype != C64) throw Error("precision c3m applies to complex64 only");
+    if (o_.precision == C3M) throw Error("precision c3m is not available in this build");
     if ((o_.precision == BF16X3 || o_.precision == FP16X2) && !(o_.dtype == C64 || o_.dtype == F32))
       throw Error("split precision applies to float32 / complex64 only");
   }

@@ -609,6 +609,7 @@ int launch_gemm(int dt, int prec, int kernel, const GemmArgs& g, hipStream_t s) 
     run_simple(dt, g, s);
     return GEMM_SIMPLE;
   }
+  if (prec == C3M) throw Error("precision c3m is not available in this build");
   if ((prec == BF16X3 || prec == FP16X2) && (dt == F32 || dt == C64)) {
     int64_t N = dt == C64 ? 2 * g.n : g.n;
     uint64_t tiles = (uint64_t)((g.m + 127) / 128) * (uint64_t)((N + 127) / 128);
