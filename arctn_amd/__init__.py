@@ -5,7 +5,8 @@ Public surface (mirrors the reference's arctn_py / arctn Python API):
 lower-level ``Plan``; ``contract`` (autograd) lives in ``arctn_amd.autograd``.
 """
 
-from .api import CompiledContraction, Plan, arctn_contract, contract_sliced  # noqa: F401
+from .api import (CompiledContraction, GroupedContraction, Plan, arctn_contract, contract_grouped,  # noqa: F401
+                  contract_sliced)
 
 __version__ = "0.1.0"
-__all__ = ["CompiledContraction", "Plan", "arctn_contract", "contract_sliced"]
+__all__ = ["CompiledContraction", "GroupedContraction", "Plan", "arctn_contract", "contract_grouped", "contract_sliced"]

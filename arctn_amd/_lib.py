@@ -82,6 +82,10 @@ _SIGS = {
                            C.c_size_t, _VP, C.c_size_t, _VP, C.c_size_t, _VP]),
     "atn_grouped_execute": (C.c_int, [C.POINTER(_VP), C.c_size_t, C.c_int, C.c_int, C.POINTER(C.POINTER(_VP)),
                                       C.POINTER(_VP), C.POINTER(C.c_double), _VP, C.POINTER(Report)]),
+    "atn_grouped_create": (C.c_int, [C.POINTER(_VP), C.c_size_t, C.c_int, C.c_int, C.POINTER(_VP)]),
+    "atn_grouped_destroy": (None, [_VP]),
+    "atn_grouped_run": (C.c_int, [_VP, C.POINTER(C.POINTER(_VP)), C.POINTER(_VP), C.POINTER(C.c_double), _VP,
+                                  C.POINTER(Report)]),
 }
 
 _lib = None

@@ -382,3 +382,86 @@ def arctn_contract(inputs, output, size_dict, arrays, *, preset="greedy", seed=0
             "execution_dispatch_wall_s": exec_s, "execution_device_ms": rep["elapsed_ms"],
             "execution_timing_scope": "hip_events", "report": rep}
     return result, info
+
+
+class GroupedContraction:
+    """Many independent unsliced plans executed together in lockstep grouped
+    launches (one launch per kernel kind and step index over all plans).
+
+    Each plan's result follows CompiledContraction.execute / execute_stripped
+    (src/compiled.rs:369-441). The launch descriptors, the grouping into
+    memory-bounded groups and the device arena are built once here;
+    ``run`` replays them on new inputs.
+    """
+
+    def __init__(self, plans, dtype, strip=True):
+        self.plans = list(plans)
+        if isinstance(dtype, int):
+            self.code = dtype
+        else:
+            self.code = DTYPE_CODE.get(np.dtype(dtype))
+            if self.code is None:
+                raise ValueError(f"unsupported dtype {dtype!r}")
+        self.strip = bool(strip)
+        self._h = _lib.C.c_void_p()
+        arr = (_lib._VP * len(self.plans))(*[p.handle for p in self.plans])
+        _lib.check(_lib.lib().atn_grouped_create(arr, len(self.plans), self.code, int(self.strip), _lib.C.byref(self._h)))
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _lib._lib is not None:
+            _lib.lib().atn_grouped_destroy(self._h)
+            self._h = None
+
+    def run(self, dev_inputs, outs=None, report=False, stream=None):
+        """dev_inputs[p]: device tensors of plan p (the executor's dtype).
+        Returns (outputs, log10 exponents[, report])."""
+        torch = _torch()
+        n = len(self.plans)
+        if len(dev_inputs) != n:
+            raise ValueError(f"arrays 个数 {len(dev_inputs)} 与 plans 个数 {n} 不符")
+        if n == 0:
+            return ([], [], {}) if report else ([], [])
+        dev = dev_inputs[0][0].device if dev_inputs[0] else torch.device("cuda", torch.cuda.current_device())
+        if outs is None:
+            outs = [torch.empty([p.size_dict[l] for l in p.output], dtype=_torch_dtype(self.code), device=dev)
+                    for p in self.plans]
+        keep = [_lib.pointer_array([t.data_ptr() for t in d]) for d in dev_inputs]
+        in_arr = (_lib.C.POINTER(_lib._VP) * n)(*[_lib.C.cast(a, _lib.C.POINTER(_lib._VP)) for a in keep])
+        out_arr = _lib.pointer_array([t.data_ptr() for t in outs])
+        exps = (_lib.C.c_double * n)()
+        s = stream if stream is not None else torch.cuda.current_stream(dev)
+        rep = _lib.Report()
+        _lib.check(_lib.lib().atn_grouped_run(self._h, in_arr, out_arr, exps, s.cuda_stream, _lib.C.byref(rep)))
+        if report:
+            return outs, list(exps), rep.as_dict()
+        return outs, list(exps)
+
+
+def contract_grouped(plans, arrays, *, strip=True, report=False, stream=None):
+    """Contract many independent unsliced plans in lockstep grouped launches.
+
+    ``plans`` is a list of unsliced ``Plan``; ``arrays[p]`` the input arrays of
+    plan p (all of one dtype). With ``strip`` every step result is
+    renormalised by a power of two and the result of plan p is
+    ``outputs[p] * 10**exps[p]`` (execute_stripped). Returns ``(outputs, exps)``
+    (plus the report dict when ``report``); outputs are torch tensors, or NumPy
+    arrays for NumPy inputs.
+    """
+    if len(plans) != len(arrays):
+        raise ValueError(f"plans 个数 {len(plans)} 与 arrays 个数 {len(arrays)} 不符")
+    if not plans:
+        return ([], [], {}) if report else ([], [])
+    devs, code, numpy_in = [], None, False
+    for p, arrs in zip(plans, arrays):
+        d, c, n_in = _device_inputs(p.inputs, p.size_dict, arrs, "分组收缩失败")
+        if code is not None and c != code:
+            raise ValueError("分组收缩失败: 所有实例的 dtype 必须一致")
+        code, numpy_in = c, numpy_in or n_in
+        devs.append(d)
+    try:
+        g = GroupedContraction(plans, code, strip)
+        res = g.run(devs, report=report, stream=stream)
+    except ValueError as e:
+        raise ValueError(f"分组收缩失败: {e}") from None
+    outs = [_finish(o, numpy_in) for o in res[0]]
+    return (outs,) + tuple(res[1:])

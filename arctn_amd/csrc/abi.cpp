@@ -300,11 +300,12 @@ int atn_gemm(int dtype, int precision, int kernel, size_t batch, size_t m, size_
     g.lda = g.k, g.ldb = g.n, g.ldc = g.n;
     g.nbo = (int64_t)batch, g.nbi = 1;
     g.sao = (int64_t)stride_a, g.sbo = (int64_t)stride_b, g.sco = (int64_t)stride_c;
-    if (kernel < -1 || kernel > 2) throw Error("unknown GEMM kernel");
+    if (kernel < -1 || kernel > 3) throw Error("unknown GEMM kernel");
     if (precision < 0 || precision > 3) throw Error("unsupported precision code");
     hipStream_t s = static_cast<hipStream_t>(stream);
     void* scr = nullptr;
-    ATN_HIP(hipMallocAsync(&scr, 256, s));
+    g.scratch_bytes = gemm_scratch_bytes(dtype, precision, kernel, g);
+    ATN_HIP(hipMallocAsync(&scr, g.scratch_bytes, s));
     g.scratch = scr;
     try {
       launch_gemm(dtype, precision, kernel, g, s);
@@ -313,6 +314,44 @@ int atn_gemm(int dtype, int precision, int kernel, size_t batch, size_t m, size_
       throw;
     }
     ATN_HIP(hipFreeAsync(scr, s));
+  });
+}
+
+struct atn_grouped {
+  std::unique_ptr<GroupedExecutor> ex;
+};
+
+int atn_grouped_create(const atn_plan* const* plans, size_t n_plans, int dtype, int strip, atn_grouped** out) {
+  return guard([&] {
+    if (!out || (n_plans && !plans)) throw Error("NULL argument");
+    *out = nullptr;
+    check_dtype(dtype);
+    std::vector<std::shared_ptr<const Plan>> ps;
+    for (size_t i = 0; i < n_plans; ++i) {
+      if (!plans[i]) throw Error("NULL plan");
+      ps.push_back(plans[i]->p);
+    }
+    auto ex = std::make_unique<GroupedExecutor>(std::move(ps), dtype, strip != 0);
+    *out = new atn_grouped{std::move(ex)};
+  });
+}
+
+void atn_grouped_destroy(atn_grouped* g) { delete g; }
+
+int atn_grouped_run(atn_grouped* g, const void* const* const* inputs, void* const* outputs, double* log10_exponents,
+                    void* stream, atn_report* report) {
+  return guard([&] {
+    if (!g || (g->ex->n_plans() && (!inputs || !outputs))) throw Error("NULL argument");
+    std::vector<const void* const*> ins;
+    std::vector<void*> outs;
+    for (size_t i = 0; i < g->ex->n_plans(); ++i) {
+      if (!inputs[i] || !outputs[i]) throw Error(cat("NULL input array or output of plan ", i));
+      ins.push_back(inputs[i]);
+      outs.push_back(outputs[i]);
+    }
+    Report r;
+    g->ex->run(ins, outs, log10_exponents, static_cast<hipStream_t>(stream), report ? &r : nullptr);
+    to_report(r, report);
   });
 }
 

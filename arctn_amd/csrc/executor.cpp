@@ -169,11 +169,16 @@ void Executor::gen_gemm(const PairStep& st, size_t nbo, bool va, bool vb, const 
   double bytes = es * (((g.sao || g.nbo == 1) ? nz : (double)g.nbi) * (double)(g.m * g.k) +
                        ((g.sbo || g.nbo == 1) ? nz : (double)g.nbi) * (double)(g.k * g.n) + nz * (double)(g.m * g.n));
   Ref scr{Ref::WS, scratch_buf_};
+  size_t& scr_bytes = bufs_[scratch_buf_].bytes;
+  for (int kern : {GEMM_AUTO, GEMM_SIMPLE, GEMM_SKINNY, GEMM_MFMA, GEMM_SPLITK})  // autotune candidates
+    scr_bytes = std::max(scr_bytes, gemm_scratch_bytes(dt, prec, kern, g));
+  const int sb = scratch_buf_;
   emit(
       [=](const Ctx& cx) {
         GemmArgs q = g;
         q.a = cx.ptr(a), q.b = cx.ptr(b), q.c = cx.ptr(c);
         q.scratch = cx.ptr(scr);
+        q.scratch_bytes = bufs_[sb].bytes;
         launch_gemm(dt, prec, gemm_choice_[idx], q, cx.s);
       },
       flops, bytes);
@@ -292,7 +297,7 @@ void Executor::build(size_t B) {
   const bool sliced = P.n_slices > 1;
   const size_t out_numel = P.sub_out_numel;
   cur_ = &pre_;
-  scratch_buf_ = newbuf(256);  // FP16X2 operand scales; lives for the whole run
+  scratch_buf_ = newbuf(256);  // FP16X2 scales, split-K partials (grown per GEMM site); lives for the whole run
   if (o_.strip) {
     maxbits_buf_ = newbuf(8);
     expo_buf_ = newbuf(8);
@@ -562,9 +567,11 @@ void Executor::autotune(const Ctx& c, hipStream_t s) {
     GemmArgs q = site.g;
     q.a = c.ptr(site.a), q.b = c.ptr(site.b), q.c = c.ptr(site.c);
     q.scratch = c.ws + offs_[scratch_buf_];
+    q.scratch_bytes = bufs_[scratch_buf_].bytes;
     float best_ms = 1e30f;
     int choice = GEMM_SIMPLE;
-    for (int cand : {GEMM_SIMPLE, GEMM_SKINNY, GEMM_MFMA}) {
+    for (int cand : {GEMM_SIMPLE, GEMM_SKINNY, GEMM_MFMA, GEMM_SPLITK}) {
+      if (cand == GEMM_SPLITK && q.k < 256) continue;
       launch_gemm(o_.dtype, o_.precision, cand, q, s);
       ATN_HIP(hipEventRecord(e0, s));
       for (int r = 0; r < 3; ++r) launch_gemm(o_.dtype, o_.precision, cand, q, s);
@@ -797,7 +804,8 @@ void stage_step_forward(const Plan& p, size_t step, int dt, int prec, const void
   g.lda = g.k, g.ldb = g.n, g.ldc = g.n;
   g.nbo = 1, g.nbi = st.bt;
   g.sai = g.m * g.k, g.sbi = g.k * g.n, g.sci = g.m * g.n;
-  g.scratch = sc.get(256);
+  g.scratch_bytes = gemm_scratch_bytes(dt, prec, GEMM_AUTO, g);
+  g.scratch = sc.get(g.scratch_bytes);
   launch_gemm(dt, prec, GEMM_AUTO, g, s);
 }
 
@@ -848,44 +856,6 @@ void stage_finalize_adjoint(const Plan& p, int dt, const void* g, void* gin, hip
   const Finalize& f = p.cp.fin;
   Scratch sc(s);
   operand_adjoint(sc, dt, g, f.input_shape, f.sum, f.perm, gin);
-}
-
-// --------------------------------------------------------------- grouped -----
-void grouped_execute(const std::vector<std::shared_ptr<const Plan>>& plans, int dt, bool strip,
-                     const std::vector<const void* const*>& inputs, const std::vector<void*>& outputs, double* log10_exp,
-                     hipStream_t s, Report* rep) {
-  hipEvent_t e0 = nullptr, e1 = nullptr;
-  if (rep) {
-    ATN_HIP(hipEventCreate(&e0));
-    ATN_HIP(hipEventCreate(&e1));
-    ATN_HIP(hipEventRecord(e0, s));
-  }
-  double flops = 0;
-  size_t l0 = g_launch_count;
-  for (size_t p = 0; p < plans.size(); ++p) {
-    ExecOptions o;
-    o.dtype = dt;
-    o.strip = strip;
-    Executor ex(plans[p], o);
-    Report r;
-    ex.run(inputs[p], outputs[p], s, strip ? &r : nullptr);
-    if (log10_exp) log10_exp[p] = strip ? r.log10_exponent : 0.0;
-    flops += (dtype_complex(dt) ? 8.0 : 2.0) * plans[p]->flops_per_slice * (double)plans[p]->n_slices;
-    ATN_HIP(hipStreamSynchronize(s));
-  }
-  if (rep) {
-    ATN_HIP(hipEventRecord(e1, s));
-    ATN_HIP(hipEventSynchronize(e1));
-    float ms = 0;
-    ATN_HIP(hipEventElapsedTime(&ms, e0, e1));
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    rep->elapsed_ms = ms;
-    rep->flops = flops;
-    rep->tflops = ms > 0 ? flops / (ms * 1e-3) / 1e12 : 0;
-    rep->n_slices = plans.size();
-    rep->n_launches = g_launch_count - l0;
-  }
 }
 
 }  // namespace atn

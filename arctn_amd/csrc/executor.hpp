@@ -137,6 +137,32 @@ void stage_step_adjoint(const Plan& p, size_t step, int dt, const void* l, const
 void stage_finalize_forward(const Plan& p, int dt, const void* in, void* out, hipStream_t s);
 void stage_finalize_adjoint(const Plan& p, int dt, const void* g, void* gin, hipStream_t s);
 
+// Many independent unsliced plans run in lockstep grouped launches. Planned,
+// packed into memory-bounded groups and uploaded once; run() replays them.
+class GroupedExecutor {
+ public:
+  GroupedExecutor(std::vector<std::shared_ptr<const Plan>> plans, int dt, bool strip);
+  ~GroupedExecutor();
+  GroupedExecutor(const GroupedExecutor&) = delete;
+  GroupedExecutor& operator=(const GroupedExecutor&) = delete;
+  // inputs[p]: device input pointers of plan p; outputs[p]: its output. With
+  // strip, log10_exp[p] receives the exponent (result = output * 10^e).
+  void run(const std::vector<const void* const*>& inputs, const std::vector<void*>& outputs, double* log10_exp,
+           hipStream_t s, Report* rep);
+  size_t n_plans() const { return plans_.size(); }
+  size_t n_inputs(size_t p) const { return plans_[p]->cp.n_inputs; }
+
+ private:
+  struct Group;
+  void release();
+  std::vector<std::shared_ptr<const Plan>> plans_;
+  int dt_;
+  bool strip_;
+  std::vector<std::unique_ptr<Group>> groups_;
+  char* arena_ = nullptr;
+  size_t arena_bytes_ = 0;
+};
+
 void grouped_execute(const std::vector<std::shared_ptr<const Plan>>& plans, int dt, bool strip,
                      const std::vector<const void* const*>& inputs, const std::vector<void*>& outputs, double* log10_exp,
                      hipStream_t s, Report* rep);

@@ -161,6 +161,157 @@ __global__ __launch_bounds__(256) void gemm_smalln_kernel(GemmArgs g) {
   }
 }
 
+// ------------------------------------------------------------ split-K GEMM --
+// Few outputs and a long k (dot-like steps, e.g. the last steps of a closed
+// network): a workgroup owns a TM x TN output tile and one k range; its 256
+// threads stride over the range, each keeping TM*TN accumulators, and the tile
+// is reduced across the workgroup in a fixed order (wave shuffles, then the
+// four waves in order). With several k ranges the partial tiles go to scratch
+// and a second kernel adds them in range order, so the result does not depend
+// on scheduling.
+template <class T> struct Shfl;
+template <> struct Shfl<float> {
+  static __device__ float down(float v, int o) { return __shfl_down(v, o, 64); }
+};
+template <> struct Shfl<double> {
+  static __device__ double down(double v, int o) { return __shfl_down(v, o, 64); }
+};
+template <> struct Shfl<float2> {
+  static __device__ float2 down(float2 v, int o) { return make_float2(__shfl_down(v.x, o, 64), __shfl_down(v.y, o, 64)); }
+};
+template <> struct Shfl<double2> {
+  static __device__ double2 down(double2 v, int o) { return make_double2(__shfl_down(v.x, o, 64), __shfl_down(v.y, o, 64)); }
+};
+template <class T> __device__ inline T cadd(T a, T b);
+template <> __device__ inline float cadd(float a, float b) { return a + b; }
+template <> __device__ inline double cadd(double a, double b) { return a + b; }
+template <> __device__ inline float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+template <> __device__ inline double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+
+struct SplitK {
+  int64_t tiles_m, tiles_n, n_tiles;  // n_tiles = nz * tiles_m * tiles_n
+  int64_t k_chunk, n_split;
+  void* part;  // [n_split][nz * m * n] partial sums, or null when n_split == 1
+};
+
+// k split into n_split ranges of k_chunk (the last may be shorter).
+struct KSplit {
+  int64_t n_split = 1, k_chunk = 0;
+};
+constexpr size_t MAX_PARTIAL_BYTES = size_t(512) << 20;
+
+// want ranges of at least min_len (rounded to `align`), within `avail` bytes of partials
+static KSplit split_k(int64_t k, int64_t want, int64_t min_len, int64_t align, size_t part_bytes, size_t avail) {
+  KSplit r;
+  want = std::min<int64_t>(want, std::max<int64_t>(1, k / min_len));
+  if (part_bytes) want = std::min<int64_t>(want, (int64_t)(avail / part_bytes));
+  if (want <= 1) return r;
+  r.k_chunk = ((k + want - 1) / want + align - 1) / align * align;
+  r.n_split = (k + r.k_chunk - 1) / r.k_chunk;
+  if (r.n_split <= 1) r = KSplit{};
+  return r;
+}
+
+static void splitk_tile(int64_t n, int& tm, int& tn) {
+  if (n >= 9) tm = 1, tn = 16;
+  else if (n >= 5) tm = 2, tn = 8;
+  else if (n >= 3) tm = 4, tn = 4;
+  else if (n == 2) tm = 8, tn = 2;
+  else tm = 16, tn = 1;
+}
+
+static KSplit plan_splitk(int dt, const GemmArgs& g, size_t avail) {
+  int tm, tn;
+  splitk_tile(g.n, tm, tn);
+  const int64_t nz = g.nbo * g.nbi;
+  const int64_t n_tiles = nz * ((g.m + tm - 1) / tm) * ((g.n + tn - 1) / tn);
+  // enough workgroups to fill the chip, each k range at least 1024 long
+  const int64_t want = std::max<int64_t>(1, 2048 / std::max<int64_t>(n_tiles, 1));
+  return split_k(g.k, want, 1024, 256, dtype_size(dt) * (size_t)(nz * g.m * g.n), avail);
+}
+
+static KSplit plan_mfma_split(int dt, int prec, const GemmArgs& g, size_t avail) {
+  if (prec != NATIVE) return KSplit{};
+  const bool cplx = dtype_complex(dt);
+  const int64_t bm = (dt == F64 || dt == C128) ? 64 : 128;
+  const int64_t N = cplx ? 2 * g.n : g.n;
+  const int64_t tiles = g.nbo * g.nbi * ((g.m + bm - 1) / bm) * ((N + bm - 1) / bm);
+  if (tiles >= 512 || g.nbo * g.nbi > 65535) return KSplit{};
+  // about 1024 workgroups, each k range at least 512 long (a multiple of the k tile)
+  const int64_t want = (1024 + tiles - 1) / tiles;
+  return split_k(g.k, want, 512, 64, dtype_size(dt) * (size_t)(g.nbo * g.nbi * g.m * g.n), avail);
+}
+
+static size_t scratch_avail(const GemmArgs& g) { return g.scratch && g.scratch_bytes > 256 ? g.scratch_bytes - 256 : 0; }
+
+template <class T, int TM, int TN>
+__global__ __launch_bounds__(256) void gemm_splitk_kernel(GemmArgs g, SplitK sk) {
+  __shared__ T red[4][TM * TN];
+  const int64_t q0 = (int64_t)blockIdx.y * sk.k_chunk;
+  const int64_t q1 = q0 + sk.k_chunk < g.k ? q0 + sk.k_chunk : g.k;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (int64_t tile = blockIdx.x; tile < sk.n_tiles; tile += gridDim.x) {
+    const int64_t per_z = sk.tiles_m * sk.tiles_n;
+    const int64_t z = tile / per_z, r = tile - z * per_z;
+    const int64_t i0 = (r / sk.tiles_n) * TM, j0 = (r - (r / sk.tiles_n) * sk.tiles_n) * TN;
+    const int64_t zo = z / g.nbi, zi = z - zo * g.nbi;
+    const T* A = (const T*)g.a + zo * g.sao + zi * g.sai;
+    const T* B = (const T*)g.b + zo * g.sbo + zi * g.sbi;
+    T acc[TM][TN];
+#pragma unroll
+    for (int a = 0; a < TM; ++a)
+#pragma unroll
+      for (int b = 0; b < TN; ++b) acc[a][b] = CM<T>::zero();
+#pragma unroll 2
+    for (int64_t q = q0 + threadIdx.x; q < q1; q += 256) {
+      T av[TM], bv[TN];
+#pragma unroll
+      for (int a = 0; a < TM; ++a) av[a] = i0 + a < g.m ? A[(i0 + a) * g.lda + q] : CM<T>::zero();
+#pragma unroll
+      for (int b = 0; b < TN; ++b) bv[b] = j0 + b < g.n ? B[q * g.ldb + j0 + b] : CM<T>::zero();
+#pragma unroll
+      for (int a = 0; a < TM; ++a)
+#pragma unroll
+        for (int b = 0; b < TN; ++b) acc[a][b] = CM<T>::mac(acc[a][b], av[a], bv[b]);
+    }
+#pragma unroll
+    for (int a = 0; a < TM; ++a)
+#pragma unroll
+      for (int b = 0; b < TN; ++b) {
+        T v = acc[a][b];
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) v = cadd(v, Shfl<T>::down(v, o));
+        if (lane == 0) red[wave][a * TN + b] = v;
+      }
+    __syncthreads();
+    if (threadIdx.x < TM * TN) {
+      const int u = threadIdx.x, a = u / TN, b = u - (u / TN) * TN;
+      T v = cadd(cadd(cadd(red[0][u], red[1][u]), red[2][u]), red[3][u]);
+      if (i0 + a < g.m && j0 + b < g.n) {
+        if (sk.n_split == 1) {
+          ((T*)g.c)[zo * g.sco + zi * g.sci + (i0 + a) * g.ldc + j0 + b] = v;
+        } else {
+          const int64_t n_out = (int64_t)g.nbo * g.nbi * g.m * g.n;
+          ((T*)sk.part)[blockIdx.y * n_out + (z * g.m + i0 + a) * g.n + j0 + b] = v;
+        }
+      }
+    }
+    __syncthreads();
+  }
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void splitk_reduce_kernel(GemmArgs g, SplitK sk) {
+  const int64_t n_out = (int64_t)g.nbo * g.nbi * g.m * g.n;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n_out; e += (int64_t)gridDim.x * blockDim.x) {
+    T v = ((const T*)sk.part)[e];
+    for (int64_t sp = 1; sp < sk.n_split; ++sp) v = cadd(v, ((const T*)sk.part)[sp * n_out + e]);
+    const int64_t mn = g.m * g.n, z = e / mn, r = e - z * mn, i = r / g.n, j = r - i * g.n;
+    const int64_t zo = z / g.nbi, zi = z - zo * g.nbi;
+    ((T*)g.c)[zo * g.sco + zi * g.sci + i * g.ldc + j] = v;
+  }
+}
+
 // --------------------------------------------------------------- MFMA GEMM --
 template <class F> struct Mfma;
 template <> struct Mfma<float> {
@@ -205,12 +356,18 @@ __global__ __launch_bounds__(256) void gemm_mfma_kernel(GemmArgs g) {
   const int wm0 = (wave >> 1) * WM, wn0 = (wave & 1) * WN;
   const uint64_t nz = (uint64_t)g.nbo * g.nbi;
   const uint64_t ntile = (uint64_t)tiles_m * tiles_n;
+  // k range of this blockIdx.z (real units; kc is a multiple of BK)
+  const int64_t kbeg = g.kc ? (int64_t)blockIdx.z * g.kc * (CPLX ? 2 : 1) : 0;
+  const int64_t kend = g.kc && kbeg + g.kc * (CPLX ? 2 : 1) < K ? kbeg + g.kc * (CPLX ? 2 : 1) : K;
 
   for (uint64_t z = blockIdx.y; z < nz; z += gridDim.y) {
     uint64_t zo = z / g.nbi, zi = z - zo * g.nbi;
     const F* A = (const F*)g.a + (zo * g.sao + zi * g.sai) * (CPLX ? 2 : 1);
     const F* B = (const F*)g.b + (zo * g.sbo + zi * g.sbi) * (CPLX ? 2 : 1);
-    F* C = (F*)g.c + (zo * g.sco + zi * g.sci) * (CPLX ? 2 : 1);
+    // with split k, partial tiles go to part[blockIdx.z][z][M][N] (real units)
+    F* C = g.part ? (F*)g.part + ((uint64_t)blockIdx.z * nz + z) * (uint64_t)(M * N)
+                  : (F*)g.c + (zo * g.sco + zi * g.sci) * (CPLX ? 2 : 1);
+    const int64_t ldo = g.part ? N : ldc;
     for (uint64_t t = blockIdx.x; t < ntile; t += gridDim.x) {
       const int64_t m0 = (int64_t)(t / tiles_n) * BM, n0 = (int64_t)(t % tiles_n) * BN;
       V acc[TM][TN];
@@ -226,14 +383,14 @@ __global__ __launch_bounds__(256) void gemm_mfma_kernel(GemmArgs g) {
         for (int u = 0; u < A_PER; ++u) {
           int e = tid + 256 * u, col = e % BK, row = e / BK;
           int64_t gr = m0 + row, gc = k0 + col;
-          ra[u] = (gr < M && gc < K) ? A[gr * lda + gc] : (F)0;
+          ra[u] = (gr < M && gc < kend) ? A[gr * lda + gc] : (F)0;
         }
         if constexpr (CPLX) {
 #pragma unroll
           for (int u = 0; u < B_PER; ++u) {
             int e = tid + 256 * u, j = e % (BN / 2), p = e / (BN / 2);
             int64_t gp = k0 / 2 + p, gj = n0 / 2 + j;
-            bool ok = gp < g.k && gj < g.n;
+            bool ok = gp < kend / 2 && gj < g.n;
             rb[2 * u] = ok ? B[(gp * ldb + gj) * 2] : (F)0;
             rb[2 * u + 1] = ok ? B[(gp * ldb + gj) * 2 + 1] : (F)0;
           }
@@ -242,7 +399,7 @@ __global__ __launch_bounds__(256) void gemm_mfma_kernel(GemmArgs g) {
           for (int u = 0; u < B_PER; ++u) {
             int e = tid + 256 * u, col = e % BN, row = e / BN;
             int64_t gr = k0 + row, gc = n0 + col;
-            rb[u] = (gr < K && gc < N) ? B[gr * ldb + gc] : (F)0;
+            rb[u] = (gr < kend && gc < N) ? B[gr * ldb + gc] : (F)0;
           }
         }
       };
@@ -271,12 +428,12 @@ __global__ __launch_bounds__(256) void gemm_mfma_kernel(GemmArgs g) {
         }
       };
 
-      const int64_t nk = (K + BK - 1) / BK;
-      load(0);
+      const int64_t nk = (kend - kbeg + BK - 1) / BK;
+      load(kbeg);
       store();
       __syncthreads();
       for (int64_t kt = 0; kt < nk; ++kt) {
-        if (kt + 1 < nk) load((kt + 1) * BK);
+        if (kt + 1 < nk) load(kbeg + (kt + 1) * BK);
 #pragma unroll
         for (int kk = 0; kk < BK; kk += 4) {
           F af[TM], bf[TN];
@@ -304,7 +461,7 @@ __global__ __launch_bounds__(256) void gemm_mfma_kernel(GemmArgs g) {
 #pragma unroll
           for (int r = 0; r < 4; ++r) {
             int64_t row = m0 + wm0 + i * 16 + Mfma<F>::row(lane, r), col = n0 + wn0 + j * 16 + cl;
-            if (row < M && col < N) C[row * ldc + col] = acc[i][j][r];
+            if (row < M && col < N) C[row * ldo + col] = acc[i][j][r];
           }
     }
   }
@@ -548,12 +705,28 @@ static void run_simple(int dt, const GemmArgs& g, hipStream_t s, int opa = 0, in
 }
 
 template <class F, bool CPLX, int BM, int BN, int BK>
-static void run_mfma(const GemmArgs& g, hipStream_t s) {
+static void run_mfma(int dt, const GemmArgs& g0, hipStream_t s) {
+  static_assert(64 % BK == 0, "k ranges are multiples of 64");
+  GemmArgs g = g0;
+  const KSplit ks = plan_mfma_split(dt, NATIVE, g, scratch_avail(g));
+  g.kc = ks.n_split > 1 ? ks.k_chunk : 0;
+  g.part = ks.n_split > 1 ? (char*)g.scratch + 256 : nullptr;
   int64_t N = CPLX ? 2 * g.n : g.n;
   uint64_t tiles = (uint64_t)((g.m + BM - 1) / BM) * (uint64_t)((N + BN - 1) / BN);
-  dim3 grid((unsigned)std::min<uint64_t>(tiles, 1u << 20), zgrid(g));
+  dim3 grid((unsigned)std::min<uint64_t>(tiles, 1u << 20), zgrid(g), (unsigned)ks.n_split);
   g_last_kind = sizeof(F) == 4 ? K_GEMM_MFMA_F32 : K_GEMM_MFMA_F64;
   LAUNCH((gemm_mfma_kernel<F, CPLX, BM, BN, BK>), grid, dim3(256), 0, s, g);
+  if (ks.n_split > 1) {
+    SplitK sk{};
+    sk.n_split = ks.n_split;
+    sk.part = g.part;
+    const int64_t n_out = g.nbo * g.nbi * g.m * g.n;
+    dim3 rg((unsigned)std::min<int64_t>((n_out + 255) / 256, 65536));
+    dispatch_dt(dt, [&](auto tag) {
+      using T = decltype(tag);
+      LAUNCH((splitk_reduce_kernel<T>), rg, dim3(256), 0, s, g, sk);
+    });
+  }
 }
 
 static bool mfma_worth(int dt, const GemmArgs& g) {
@@ -595,11 +768,59 @@ static bool run_skinny(int dt, const GemmArgs& g, hipStream_t s) {
   return false;
 }
 
+
+template <class T, int TM, int TN>
+static void run_splitk_t(const GemmArgs& g, const KSplit& ks, hipStream_t s) {
+  SplitK sk{};
+  const int64_t nz = g.nbo * g.nbi;
+  sk.tiles_m = (g.m + TM - 1) / TM, sk.tiles_n = (g.n + TN - 1) / TN;
+  sk.n_tiles = nz * sk.tiles_m * sk.tiles_n;
+  sk.n_split = ks.n_split;
+  sk.k_chunk = ks.n_split > 1 ? ks.k_chunk : g.k;
+  sk.part = ks.n_split > 1 ? (char*)g.scratch + 256 : nullptr;
+  const int64_t n_out = nz * g.m * g.n;
+  dim3 grid((unsigned)std::min<int64_t>(sk.n_tiles, 65536), (unsigned)sk.n_split);
+  LAUNCH((gemm_splitk_kernel<T, TM, TN>), grid, dim3(256), 0, s, g, sk);
+  if (sk.n_split > 1)
+    LAUNCH((splitk_reduce_kernel<T>), dim3((unsigned)std::min<int64_t>((n_out + 255) / 256, 65536)), dim3(256), 0, s, g, sk);
+}
+
+static void run_splitk(int dt, const GemmArgs& g, hipStream_t s) {
+  g_last_kind = K_GEMM_SPLITK;
+  const KSplit ks = plan_splitk(dt, g, scratch_avail(g));
+  int tm, tn;
+  splitk_tile(g.n, tm, tn);
+  dispatch_dt(dt, [&](auto tag) {
+    using T = decltype(tag);
+    if (tn == 16) run_splitk_t<T, 1, 16>(g, ks, s);
+    else if (tn == 8) run_splitk_t<T, 2, 8>(g, ks, s);
+    else if (tn == 4) run_splitk_t<T, 4, 4>(g, ks, s);
+    else if (tn == 2) run_splitk_t<T, 8, 2>(g, ks, s);
+    else run_splitk_t<T, 16, 1>(g, ks, s);
+  });
+}
+
+// Dot-like shapes: a long k with too few outputs (or skinny columns) to fill the chip.
+static bool splitk_worth(const GemmArgs& g) {
+  const uint64_t nz = (uint64_t)g.nbo * g.nbi;
+  uint64_t par = nz * (uint64_t)g.m * (uint64_t)g.n;
+  if (g.m <= 32 && g.n >= 64) par = nz * (uint64_t)g.n;
+  else if (g.n <= 32 && g.m >= 64) par = nz * (uint64_t)g.m;
+  return g.k >= 256 && par < 32768;
+}
+
+static int auto_kernel(int dt, const GemmArgs& g) {
+  if (mfma_worth(dt, g)) return GEMM_MFMA;
+  if (splitk_worth(g)) return GEMM_SPLITK;
+  if ((g.m <= 32 && g.n >= 64) || (g.n <= 32 && g.m >= 64)) return GEMM_SKINNY;
+  return GEMM_SIMPLE;
+}
+
 int launch_gemm(int dt, int prec, int kernel, const GemmArgs& g, hipStream_t s) {
-  if (kernel == GEMM_AUTO) {
-    if (mfma_worth(dt, g)) kernel = GEMM_MFMA;
-    else if ((g.m <= 32 && g.n >= 64) || (g.n <= 32 && g.m >= 64)) kernel = GEMM_SKINNY;
-    else kernel = GEMM_SIMPLE;
+  if (kernel == GEMM_AUTO) kernel = auto_kernel(dt, g);
+  if (kernel == GEMM_SPLITK) {
+    run_splitk(dt, g, s);
+    return GEMM_SPLITK;
   }
   if (kernel == GEMM_SKINNY) {
     if (run_skinny(dt, g, s)) return GEMM_SKINNY;
@@ -657,13 +878,22 @@ int launch_gemm(int dt, int prec, int kernel, const GemmArgs& g, hipStream_t s) 
     return GEMM_MFMA;
   }
   switch (dt) {
-    case F32: run_mfma<float, false, 128, 128, 16>(g, s); break;
-    case C64: run_mfma<float, true, 128, 128, 16>(g, s); break;
-    case F64: run_mfma<double, false, 64, 64, 16>(g, s); break;
-    case C128: run_mfma<double, true, 64, 64, 16>(g, s); break;
+    case F32: run_mfma<float, false, 128, 128, 16>(dt, g, s); break;
+    case C64: run_mfma<float, true, 128, 128, 16>(dt, g, s); break;
+    case F64: run_mfma<double, false, 64, 64, 16>(dt, g, s); break;
+    case C128: run_mfma<double, true, 64, 64, 16>(dt, g, s); break;
     default: throw Error("unsupported dtype code");
   }
   return GEMM_MFMA;
+}
+
+size_t gemm_scratch_bytes(int dt, int prec, int kernel, const GemmArgs& g) {
+  if (kernel == GEMM_AUTO) kernel = auto_kernel(dt, g);
+  KSplit ks;
+  if (kernel == GEMM_SPLITK) ks = plan_splitk(dt, g, MAX_PARTIAL_BYTES);
+  else if (kernel == GEMM_MFMA) ks = plan_mfma_split(dt, prec, g, MAX_PARTIAL_BYTES);
+  size_t part = ks.n_split > 1 ? (size_t)ks.n_split * dtype_size(dt) * (size_t)(g.nbo * g.nbi * g.m * g.n) : 0;
+  return 256 + part;
 }
 
 void launch_gemm_adjoint(int dt, int opa, int opb, const GemmArgs& g, hipStream_t s) { run_simple(dt, g, s, opa, opb); }

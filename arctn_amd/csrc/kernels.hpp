@@ -100,12 +100,21 @@ struct GemmArgs {
   int64_t lda, ldb, ldc;       // row strides in elements
   int64_t sao, sai, sbo, sbi, sco, sci;  // batch strides (outer slice batch, inner plan batch)
   int64_t nbo, nbi;
-  // FP16X2 only: 16 bytes of device scratch for the operands' max-magnitude bits.
+  // Device scratch (or null) of scratch_bytes: the FP16X2 operand max-magnitude
+  // bits (first 16 bytes) and split-K partial sums (from byte 256). With less
+  // scratch than gemm_scratch_bytes() asks for, split-K uses fewer k ranges.
   void* scratch = nullptr;
+  size_t scratch_bytes = 0;
+  // Set by launch_gemm for the MFMA kernel: k range length per blockIdx.z (0 =
+  // all of k) and the partial-sum buffer when k is split.
+  int64_t kc = 0;
+  void* part = nullptr;
 };
-enum GemmKernel { GEMM_AUTO = -1, GEMM_SIMPLE = 0, GEMM_MFMA = 1, GEMM_SKINNY = 2 };
+enum GemmKernel { GEMM_AUTO = -1, GEMM_SIMPLE = 0, GEMM_MFMA = 1, GEMM_SKINNY = 2, GEMM_SPLITK = 3 };
 // Returns the kernel used.
 int launch_gemm(int dt, int prec, int kernel, const GemmArgs& g, hipStream_t s);
+// Device scratch launch_gemm can use for this GEMM (FP16X2 scales, split-K partials).
+size_t gemm_scratch_bytes(int dt, int prec, int kernel, const GemmArgs& g);
 // Conjugating variants for the adjoint: opA/opB: 0 = N, 1 = conj-transpose (as stored).
 void launch_gemm_adjoint(int dt, int opa, int opb, const GemmArgs& g, hipStream_t s);
 
@@ -113,7 +122,7 @@ extern thread_local size_t g_launch_count;
 // Kind of the last kernel launched on this thread (profiling labels).
 enum KernelKind { K_OTHER = 0, K_PERMUTE_TILE, K_PERMUTE_GENERIC, K_SUM, K_TRACE_DIAG, K_GATHER, K_REDUCE, K_SCATTER,
                   K_GEMM_SIMPLE, K_GEMM_SMALL, K_GEMM_MFMA_F32, K_GEMM_MFMA_F64, K_GEMM_MFMA_BF16X3, K_GEMM_MFMA_FP16X2, K_GEMM_MFMA_C3M,
-                  K_STRIP, K_COPY, K_NKINDS };
+                  K_GEMM_SPLITK, K_STRIP, K_COPY, K_NKINDS };
 extern thread_local int g_last_kind;
 const char* kernel_kind_name(int k);
 

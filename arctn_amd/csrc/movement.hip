@@ -16,7 +16,7 @@ const char* kernel_kind_name(int k) {
   static const char* names[K_NKINDS] = {"other", "permute_tile", "permute_generic", "sum_axis", "trace_diag", "slice_gather",
                                         "slice_reduce_f64", "output_scatter", "gemm_simple", "gemm_skinny", "gemm_mfma_f32",
                                         "gemm_mfma_f64", "gemm_mfma_bf16x3", "gemm_mfma_fp16x2", "gemm_mfma_c3m",
-                                        "strip", "copy"};
+                                        "gemm_splitk", "strip", "copy"};
   return (k >= 0 && k < K_NKINDS) ? names[k] : "other";
 }
 

@@ -450,6 +450,30 @@ def sliced_cost_log10(net, path, sliced):
     return st.log10_flops + n_log2 * math.log10(2.0)
 
 
+def objective_score_log2(stats, flops_weight=1.0, read_write_weight=64.0):
+    """The reference's fixed planner objective (objective.rs:38-41, 87-102):
+    log2(flops_weight * flops + read_write_weight * read_write)."""
+    log2_flops = stats.log10_flops / math.log10(2.0)
+    return _logaddexp2(math.log2(flops_weight) + log2_flops, math.log2(read_write_weight) + stats.log2_read_write)
+
+
+def random_greedy(net, ntrials, seed):
+    """Best of ``ntrials`` random-greedy trials by the fixed objective
+    (greedy.rs:325-412; ties keep the lower trial). Returns (path, stats)."""
+    import numpy as np
+
+    best = None
+    for trial in range(max(ntrials, 1)):
+        rng = np.random.default_rng([seed, trial])
+        cm, t, fn = _rgreedy_params(trial, rng)
+        path = greedy(net, costmod=cm, temperature=t, nbranch=8 if trial > 2 else 1, cost_fn=fn, rng=rng)
+        st, _ = simulate_path_full(net, path)
+        score = objective_score_log2(st)
+        if best is None or score < best[2]:
+            best = (path, st, score)
+    return best[0], best[1]
+
+
 def random_greedy_sliced(net, ntrials, seed, target_size, allow_output_slicing=False):
     """Random-greedy trials (greedy.rs:325-412, nbranch 8), each sliced to
     ``target_size`` with ``find_slices_to_size``; the trial with the lowest

@@ -23,9 +23,43 @@ PLANS = {
     # 36-qubit 6x6 grid, 12 cycles, 6 open qubits (64 amplitudes), target 2^26.
     "circuit_6x6_c12_o6_t26": dict(rows=6, cols=6, cycles=12, seed=0, open=6, target=2**26, trials=24),
 }
+# 512 random 3-regular networks, n uniform over even [40, 120], dims {2, 3, 4},
+# float32 N(0, 1) entries, scalar output; per-instance random-greedy path, seed = instance id.
+UNEVEN = {
+    "uneven_3regular_512": dict(n_inst=512, seed=0, n_lo=40, n_hi=120, dims=[2, 3, 4], trials=8),
+}
+
+
+def build_uneven(name):
+    p = UNEVEN[name]
+    t0 = time.time()
+    nets = workloads.uneven_networks(p["n_inst"], p["seed"], p["n_lo"], p["n_hi"], p["dims"])
+    paths, costs = [], []
+    for i, (net, _) in enumerate(nets):
+        path, st = planner.random_greedy(net, p["trials"], i)
+        paths.append([list(x) for x in path])
+        costs.append(st.log10_flops)
+    doc = dict(name=name, workload=p, ssa_paths=paths, log10_mults=costs, planning_seconds=time.time() - t0)
+    with open(os.path.join(HERE, name + ".json"), "w") as f:
+        json.dump(doc, f, separators=(",", ":"))
+    print(name, "instances", len(paths), "planning s", round(time.time() - t0, 1), flush=True)
+
+
+def load_uneven(name, dtype=None):
+    """Returns (list of (net, arrays, ssa_path), plan_doc)."""
+    import numpy as np
+
+    with open(os.path.join(HERE, name + ".json")) as f:
+        doc = json.load(f)
+    p = doc["workload"]
+    nets = workloads.uneven_networks(p["n_inst"], p["seed"], p["n_lo"], p["n_hi"], p["dims"],
+                                     dtype or np.float32)
+    return [(net, arrays, [tuple(x) for x in path]) for (net, arrays), path in zip(nets, doc["ssa_paths"])], doc
 
 
 def build(name):
+    if name in UNEVEN:
+        return build_uneven(name)
     p = PLANS[name]
     net, _, _ = workloads.random_circuit_amplitude(
         p["rows"], p["cols"], p["cycles"], p["seed"], open_qubits=range(p["open"]))
@@ -54,5 +88,5 @@ def load(name):
 
 
 if __name__ == "__main__":
-    for n in sys.argv[1:] or list(PLANS):
+    for n in sys.argv[1:] or list(PLANS) + list(UNEVEN):
         build(n)

@@ -302,3 +302,32 @@ def brickwork_torch_tensors(slots, angles):
         else:
             out.append(eye)
     return [t.contiguous() for t in out]
+
+
+def uneven_networks(n_inst, seed=0, n_lo=40, n_hi=120, dims=(2, 3, 4), dtype=np.float32):
+    """Independent random 3-regular networks of uneven sizes, each contracted to a scalar.
+
+    Instance i: vertex count uniform over the even integers in [n_lo, n_hi]
+    (3-regular needs an even count), multigraph without self-loops from
+    ``random_regular`` with graph seed [seed, i], per-edge dims uniform over
+    ``dims``, N(0, 1) entries. Returns a list of (net, arrays).
+    """
+    rng = np.random.default_rng([seed, 0x5EED])
+    out = []
+    for i in range(n_inst):
+        n = 2 * int(rng.integers((n_lo + 1) // 2, n_hi // 2 + 1))
+        out.append(random_regular(n, 3, list(dims), [seed, i], dtype))
+    return out
+
+
+def uneven_instances(n_inst, seed=0, n_lo=40, n_hi=120, dims=(2, 3, 4), ntrials=8, dtype=np.float32):
+    """``uneven_networks`` plus, per instance i, the best of ``ntrials``
+    random-greedy paths with seed = i (greedy.rs:375-412). Returns a list of
+    (net, arrays, ssa_path)."""
+    from . import planner as P
+
+    out = []
+    for i, (net, arrays) in enumerate(uneven_networks(n_inst, seed, n_lo, n_hi, dims, dtype)):
+        path, _ = P.random_greedy(net, ntrials, i)
+        out.append((net, arrays, path))
+    return out

@@ -179,7 +179,8 @@ int atn_tensor_op(int dtype, int op, size_t rank, const size_t* shape, const siz
 /* Batched strided GEMM used by the executor, exported for kernel tests:
  * C[b] = A[b] . B[b], row-major, batch strides in elements (0 = broadcast).
  * kernel: -1 automatic choice, 0 one-thread-per-element kernel, 1 MFMA kernel,
- * 2 skinny kernel (m <= 32 with n >= 64, or n <= 32 with m >= 64; else falls back to 0). */
+ * 2 skinny kernel (m <= 32 with n >= 64, or n <= 32 with m >= 64; else falls back to 0),
+ * 3 split-K kernel (workgroups split k; partial sums added in a fixed order). */
 int atn_gemm(int dtype, int precision, int kernel, size_t batch, size_t m, size_t k, size_t n, const void* d_a,
              size_t stride_a, const void* d_b, size_t stride_b, void* d_c, size_t stride_c, void* stream);
 
@@ -191,6 +192,17 @@ int atn_gemm(int dtype, int precision, int kernel, size_t batch, size_t m, size_
 int atn_grouped_execute(const atn_plan* const* plans, size_t n_plans, int dtype, int strip,
                         const void* const* const* inputs, void* const* outputs, double* log10_exponents,
                         void* stream, atn_report* report);
+
+/* The same grouped execution as a reusable object: plans are compiled into
+ * grouped launch descriptors, packed into memory-bounded groups and given a
+ * device arena once at creation; every run replays them with new input and
+ * output pointers. Replaces repeated CompiledContraction::execute /
+ * execute_stripped calls (src/compiled.rs:369-381) over many plans. */
+typedef struct atn_grouped atn_grouped;
+int atn_grouped_create(const atn_plan* const* plans, size_t n_plans, int dtype, int strip, atn_grouped** out);
+void atn_grouped_destroy(atn_grouped* g);
+int atn_grouped_run(atn_grouped* g, const void* const* const* inputs, void* const* outputs, double* log10_exponents,
+                    void* stream, atn_report* report);
 
 #ifdef __cplusplus
 }

@@ -82,16 +82,19 @@ def test_reductions_and_gathers_bit_exact(cuda, dtype):
 
 GEMM_SHAPES = [(1, 1, 1, 1), (1, 3, 4, 5), (2, 17, 33, 9), (1, 128, 64, 128), (3, 130, 70, 129), (1, 256, 256, 256),
                (1, 1000, 48, 300), (1, 16, 4, 4096), (4, 64, 512, 64), (1, 8, 256, 1000), (2, 1000, 16, 8),
-               (1, 3, 5, 100), (1, 200, 7, 3), (1, 32, 128, 2000)]
+               (1, 3, 5, 100), (1, 200, 7, 3), (1, 32, 128, 2000),
+               (1, 1, 100000, 1), (3, 2, 5000, 3), (1, 5, 70000, 7), (2, 4, 300, 16), (1, 3, 40000, 64),
+               (2, 100, 30000, 40)]
 # (dtype, precision code, per-product relative error bound)
 PREC_CASES = [(np.float32, 0, 1e-6), (np.float64, 0, 1e-14), (np.complex64, 0, 1e-6), (np.complex128, 0, 1e-14),
               (np.float32, 2, 1e-5), (np.complex64, 2, 1e-5), (np.float32, 3, 1e-6), (np.complex64, 3, 1e-6)]
 
 
 @pytest.mark.parametrize("case", range(len(PREC_CASES)))
-@pytest.mark.parametrize("kernel", [0, 1, 2])
+@pytest.mark.parametrize("kernel", [0, 1, 2, 3])
 def test_gemm_against_numpy(cuda, case, kernel):
-    """Simple (0), MFMA (1, native or split precision) and skinny (2) GEMMs vs complex128 NumPy."""
+    """Simple (0), MFMA (1, native or split precision), skinny (2) and split-K (3)
+    GEMMs vs complex128 NumPy."""
     import torch
     from arctn_amd import _lib
 
@@ -126,7 +129,7 @@ def test_gemm_broadcast_batch(cuda, dtype):
     a = _rand((m, k), dtype, rng)
     b = _rand((bt, k, n), dtype, rng)
     want = np.matmul(a.astype(np.complex128)[None], b.astype(np.complex128))
-    for kernel in (0, 1):
+    for kernel in (0, 1, 3):
         ta, tb = torch.from_numpy(a).cuda(), torch.from_numpy(b).cuda()
         tc = torch.empty((bt, m, n), dtype=ta.dtype, device="cuda")
         _lib.check(_lib.lib().atn_gemm(code, 0, kernel, bt, m, k, n, ta.data_ptr(), 0, tb.data_ptr(), k * n,
