@@ -63,6 +63,8 @@ static ExecOptions to_opts(const atn_exec_options* o) {
   e.strip = o->strip != 0;
   e.max_batch = o->max_batch;
   e.workspace_limit = o->workspace_limit;
+  e.slice_begin = o->slice_begin;
+  e.slice_end = o->slice_end;
   return e;
 }
 

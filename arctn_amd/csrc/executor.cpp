@@ -341,7 +341,7 @@ void Executor::build(size_t B) {
     gen_body(B);
     record_ = false;
     body_end_ = t_;
-    tail_nb_ = P.n_slices % B;
+    tail_nb_ = n_run_ % B;
     if (tail_nb_) {
       cur_ = &tail_;
       replay_ = true;
@@ -435,6 +435,14 @@ Executor::Executor(std::shared_ptr<const Plan> plan, const ExecOptions& o) : P_(
   }
   if (o_.strip && P.n_slices > 1) throw Error("strip is available for unsliced plans (execute_stripped)");
   if (!o_.batched) o_.use_graph = false;
+  g_lo_ = o_.slice_begin;
+  const size_t g_hi = o_.slice_end ? o_.slice_end : P.n_slices;
+  if (g_lo_ != 0 || g_hi != P.n_slices) {
+    if (!o_.batched) throw Error("slice ranges need the batched form");
+    if (!(g_lo_ < g_hi && g_hi <= P.n_slices))
+      throw Error(cat("slice range [", g_lo_, ", ", g_hi, ") out of bounds for ", P.n_slices, " slices"));
+  }
+  n_run_ = g_hi - g_lo_;
   size_t limit = o_.workspace_limit;
   if (!limit) {
     size_t fr = 0, tot = 0;
@@ -443,7 +451,7 @@ Executor::Executor(std::shared_ptr<const Plan> plan, const ExecOptions& o) : P_(
   }
   size_t B = 1;
   if (o_.batched) {
-    B = P.n_slices;
+    B = n_run_;
     if (o_.max_batch) B = std::min(B, o_.max_batch);
   }
   for (;;) {
@@ -478,13 +486,13 @@ void Executor::enqueue(const Ctx& c0, const std::function<void(const Op&, const 
   };
   run(pre_);
   if (o_.batched) {
-    size_t nfull = P.n_slices / B_;
+    size_t nfull = n_run_ / B_;
     for (size_t q = 0; q < nfull; ++q) {
-      c.g0 = q * B_;
+      c.g0 = g_lo_ + q * B_;
       run(body_);
     }
     if (tail_nb_) {
-      c.g0 = nfull * B_;
+      c.g0 = g_lo_ + nfull * B_;
       run(tail_);
     }
   } else {
@@ -641,10 +649,10 @@ void Executor::run(const void* const* inputs, void* out, hipStream_t s, Report* 
     (void)hipEventDestroy(e1);
     rep->elapsed_ms = ms;
     double c8 = dtype_complex(o_.dtype) ? 8.0 : 2.0;
-    rep->flops = c8 * P.flops_per_slice * (double)P.n_slices;
+    rep->flops = c8 * P.flops_per_slice * (double)n_run_;
     rep->tflops = ms > 0 ? rep->flops / (ms * 1e-3) / 1e12 : 0;
-    rep->slices_per_s = ms > 0 ? (double)P.n_slices / (ms * 1e-3) : 0;
-    rep->n_slices = P.n_slices;
+    rep->slices_per_s = ms > 0 ? (double)n_run_ / (ms * 1e-3) : 0;
+    rep->n_slices = n_run_;
     rep->batch = B_;
     rep->n_launches = launches_per_run_;
     rep->workspace_bytes = ws_bytes_;

@@ -19,6 +19,8 @@ struct ExecOptions {
   bool strip = false;
   size_t max_batch = 0;
   size_t workspace_limit = 0;
+  // global slice range [slice_begin, slice_end) of the batched form; 0, 0 = all slices
+  size_t slice_begin = 0, slice_end = 0;
 };
 
 struct Report {
@@ -102,6 +104,7 @@ class Executor {
   ExecOptions o_;
   int es_;
   size_t B_ = 1;
+  size_t g_lo_ = 0, n_run_ = 1;  // slices run: [g_lo_, g_lo_ + n_run_)
   std::vector<Buf> bufs_;
   std::vector<size_t> offs_;
   std::vector<Op> pre_, body_, tail_, post_;

@@ -53,6 +53,9 @@ def main():
     ap.add_argument("--cpu-slices", type=int, default=0, help="CPU sample size (0 = three per core)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a device copy of 2 GiB (for scaling HBM traffic counters), then exit")
+    ap.add_argument("--subset-slices", type=int, default=16,
+                    help="precision workload: slices of output block 0 compared with the CPU oracle")
+    ap.add_argument("--plain-loop", action="store_true", help="grouped workload: also time the per-instance loop")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the slice range of each GPU "
@@ -69,6 +72,11 @@ def main():
         torch.cuda.synchronize()
         log("[bench] copied 2 GiB (read 2 GiB, write 2 GiB)")
         return
+
+    if args.workload.startswith("uneven_"):
+        return bench_grouped(args)
+    if args.workload == "circuit_6x6_c12_o6_t26":
+        return bench_precision(args)
 
     from arctn_amd import Plan
     from arctn_amd.api import profile
@@ -191,6 +199,178 @@ def main():
         "kernels": kernels,
     }
     line.update(plain)
+    print(json.dumps(line), flush=True)
+
+
+def bench_grouped(args):
+    """512 uneven random 3-regular networks (float32, scalar output) in lockstep
+    grouped launches; a step is one contraction of every instance."""
+    import numpy as np
+    import torch
+
+    from arctn_amd import GroupedContraction, Plan
+    from arctn_amd.plans.make_plans import load_uneven
+    from oracle import pyoracle as O
+
+    insts, doc = load_uneven(args.workload)
+    dev = torch.device("cuda", 0)
+    plans = [Plan(n["inputs"], n["output"], n["size_dict"], p) for n, _, p in insts]
+    arrs = [[torch.from_numpy(x).to(dev) for x in a] for _, a, _ in insts]
+    flops = sum(2.0 * p.info()["flops_per_slice"] for p in plans)
+    t0 = time.perf_counter()
+    G = GroupedContraction(plans, np.float32, strip=True)
+    log(f"[bench] {len(plans)} instances, {flops:.3e} FLOP per step; grouped executor built in "
+        f"{time.perf_counter() - t0:.2f} s")
+    for w in range(args.warmup):
+        G.run(arrs)
+        torch.cuda.synchronize()
+        log(f"[bench] warmup {w + 1}/{args.warmup} done")
+    stream = torch.cuda.current_stream(dev)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record(stream)
+    for _ in range(args.steps):
+        outs, exps = G.run(arrs)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / args.steps
+    _, _, rep = G.run(arrs, report=True)
+    log(f"[bench] {args.steps} steps: {ms:.1f} ms/step, {flops / ms / 1e9:.2f} TFLOP/s, {rep['n_launches']} launches, "
+        f"{rep['batch']} groups")
+    vals = [(float(o.cpu()), e) for o, e in zip(outs, exps)]
+    # accuracy against the float64 oracle on every instance the CPU finishes quickly
+    cheap = [i for i, c in enumerate(doc["log10_mults"]) if c <= 8.0]
+    errs, errs32 = [], []
+    t0 = time.perf_counter()
+    for i in cheap:
+        net, a, path = insts[i]
+        m64, e64 = O.contract(net, [x.astype(np.float64) for x in a], path, mode="stripped")
+        m32, e32 = O.contract(net, a, path, mode="stripped")
+        ref = float(m64)
+        m, e = vals[i]
+        errs.append(abs(m * 10 ** (e - e64) - ref) / abs(ref))
+        errs32.append(abs(float(m32) * 10 ** (e32 - e64) - ref) / abs(ref))
+    cpu_s = time.perf_counter() - t0
+    log(f"[bench] float64 oracle on {len(cheap)} instances: max rel err {max(errs):.2e} "
+        f"(CPU float32 oracle on the same plans: {max(errs32):.2e})")
+    plain = {}
+    if args.plain_loop:
+        t0 = time.perf_counter()
+        for p, a in zip(plans, arrs):
+            p.run(a, 0, strip=True)
+            p.release()
+        torch.cuda.synchronize()
+        pms = (time.perf_counter() - t0) * 1e3
+        plain = {"per_instance_loop_ms": pms, "speedup_vs_per_instance_loop": pms / ms}
+        log(f"[bench] per-instance executor loop: {pms:.1f} ms -> {pms / ms:.2f}x")
+    cpu = None
+    if not args.skip_cpu:
+        # CPU oracle (float32, the executed arithmetic) on the cheapest instances, about 10 s of work
+        order = sorted(range(len(insts)), key=lambda i: doc["log10_mults"][i])
+        done, mults, t0 = 0, 0.0, time.perf_counter()
+        for i in order:
+            net, a, path = insts[i]
+            O.contract(net, a, path, mode="stripped")
+            done += 1
+            mults += 10 ** doc["log10_mults"][i]
+            if time.perf_counter() - t0 > 10.0:
+                break
+        dt = time.perf_counter() - t0
+        cpu = {"value": 2.0 * mults / dt / 1e12, "unit": "TFLOP/s", "cores": 1, "kind": "port",
+               "sample": f"the {done} cheapest instances (float32, stripped), oracle CPU restatement, {dt:.1f} s",
+               "instances_per_s": done / dt}
+    line = {
+        "metric": "grouped contraction TFLOP/s of uneven instances", "value": round(flops / ms / 1e9, 3),
+        "unit": "TFLOP/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (seeded random 3-regular graphs, N(0,1) entries)",
+        "config": {"workload": args.workload, "instances": len(plans), "vertices": "even in [40, 120]",
+                   "bond_dims": [2, 3, 4], "paths": "random greedy, seed = instance id", "strip": True,
+                   "flops_per_step": flops},
+        "instances_per_s": round(len(plans) / (ms * 1e-3), 2), "n_launches": rep["n_launches"],
+        "groups": rep["batch"], "workspace_bytes": rep["workspace_bytes"],
+        "accuracy": {"instances_checked": len(cheap), "max_rel_err_vs_f64": max(errs),
+                     "median_rel_err_vs_f64": float(np.median(errs)),
+                     "cpu_f32_max_rel_err_vs_f64": max(errs32), "oracle_seconds": cpu_s},
+        "cpu_baseline": cpu,
+    }
+    line.update(plain)
+    print(json.dumps(line), flush=True)
+
+
+def bench_precision(args):
+    """36-qubit 6x6 circuit, 12 cycles, 64 amplitudes (output-sliced): FP32 MFMA,
+    BF16x3 and FP16x2 on a slice subset against the complex128 CPU oracle."""
+    import numpy as np
+    import torch
+
+    from arctn_amd import Plan
+    from arctn_amd.plans.make_plans import load
+    from oracle import pyoracle as O
+
+    net, arrays, doc = load(args.workload)
+    plan = Plan(net["inputs"], net["output"], net["size_dict"], doc["ssa_path"], doc["sliced_legs"])
+    info, d = plan.info(), plan.describe()
+    S = min(args.subset_slices, info["n_slices"] // max(info["n_blocks"], 1))
+    dev = torch.device("cuda", 0)
+    x64 = [torch.from_numpy(a).to(dev) for a in arrays]
+    x128 = [t.to(torch.complex128) for t in x64]
+    log(f"[bench] plan: {info['n_slices']} slices in {info['n_blocks']} output blocks; subset: slices 0..{S - 1} "
+        f"of block 0 (outer legs {d['outer']} = 0)")
+    # CPU oracle on the subset: outer legs fixed to index 0, internal slices [0, S)
+    outer = set(d["outer"])
+    sub_inputs, sub_arrays = [], []
+    for legs, a in zip(net["inputs"], arrays):
+        a = a.astype(np.complex128)
+        for ax in reversed(range(len(legs))):
+            if legs[ax] in outer:
+                a = np.take(a, 0, axis=ax)
+        sub_inputs.append([l for l in legs if l not in outer])
+        sub_arrays.append(np.array(a, order="C", copy=True))
+    sub_net = {"inputs": sub_inputs, "output": [l for l in net["output"] if l not in outer],
+               "size_dict": {k: v for k, v in net["size_dict"].items() if k not in outer}}
+    cores = min(16, os.cpu_count() or 1)
+    t0 = time.perf_counter()
+    ref = O.contract_slice_range(sub_net, sub_arrays, [tuple(p) for p in doc["ssa_path"]], d["internal"], 0, S,
+                                 threads=cores)
+    cpu_s = time.perf_counter() - t0
+    log(f"[bench] CPU complex128 oracle on {S} slices: {cpu_s:.1f} s on {cores} threads")
+    # GPU block-0 entries: outer legs at index 0 in the full output
+    sel = tuple(0 if l in outer else slice(None) for l in net["output"])
+    ref = ref.reshape(-1)
+    results = {}
+    best = None
+    for prec, code, xs in (("native", 2, x64), ("bf16x3", 2, x64), ("fp16x2", 2, x64), ("complex128", 3, x128)):
+        opts = dict(batched=True, use_graph=True, slice_range=(0, S))
+        if code == 2:
+            opts["precision"] = prec
+        for _ in range(2):
+            out, rep = plan.run(xs, code, report=True, **opts)
+        got = out.cpu().numpy()[sel].reshape(-1).astype(np.complex128)
+        rel = np.abs(got - ref) / np.abs(ref)
+        results[prec] = {"max_rel_err": float(rel.max()), "median_rel_err": float(np.median(rel)),
+                         "norm_rel_err": float(np.linalg.norm(got - ref) / np.linalg.norm(ref)),
+                         "ms": rep["elapsed_ms"], "slices_per_s": S / (rep["elapsed_ms"] * 1e-3),
+                         "tflops": rep["tflops"]}
+        plan.release()
+        log(f"[bench] {prec}: max rel err {rel.max():.2e} over {len(ref)} amplitudes, {rep['elapsed_ms']:.1f} ms "
+            f"for {S} slices")
+        if code == 2 and results[prec]["max_rel_err"] <= 1e-5 and (best is None or rep["elapsed_ms"] < results[best]["ms"]):
+            best = prec
+    best = best or "native"
+    r = results[best]
+    line = {
+        "metric": "max relative error per precision variant vs complex128 CPU oracle (slice subset)",
+        "value": round(r["tflops"], 3), "unit": "TFLOP/s", "n_gpus": 1, "steps": 2, "warmup": 1,
+        "ms_per_step": round(r["ms"], 3), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "complex64", "data": "synthetic (seeded random circuit, random-init gates)",
+        "config": {"workload": args.workload, "qubits": 36, "grid": "6x6", "cycles": 12, "open_qubits": 6,
+                   "n_slices": info["n_slices"], "n_blocks": info["n_blocks"], "subset_slices": S,
+                   "fastest_within_1e-5": best},
+        "variants": results,
+        "cpu_baseline": {"value": S / cpu_s, "unit": "slices/s", "cores": cores, "kind": "port",
+                         "sample": f"slices 0..{S - 1} of block 0 (complex128), oracle CPU restatement, {cpu_s:.1f} s"},
+    }
     print(json.dumps(line), flush=True)
 
 

@@ -91,6 +91,8 @@ typedef struct {
                            step, exponent kept in FP64 (execute_stripped) */
   size_t max_batch;     /* slices per batched launch; 0 = largest that fits workspace_limit */
   size_t workspace_limit;  /* bytes; 0 = 60% of free device memory */
+  size_t slice_begin;   /* batched form: run global slices [slice_begin, slice_end) only and */
+  size_t slice_end;     /* return their partial sum (the share of one device); 0, 0 = all */
 } atn_exec_options;
 
 typedef struct {

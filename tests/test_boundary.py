@@ -158,3 +158,15 @@ def test_slice_finder_meets_target():
             sd[l] = 1
         _, legs = P.simulate_path_full(dict(net, size_dict=sd), path)
         assert all(np.prod([sd[l] for l in ls]) <= target for ls in legs)
+
+
+def test_slice_range_validated_before_device_work():
+    L = _lib.lib()
+    p = Plan([[0], [0]], [], {0: 4}, [(0, 1)], [0])
+    h = C.c_void_p()
+    o = _lib.ExecOptions(1, 0, 0, 0, 0, 0, 0, 0, 1, 3)  # plain form
+    assert L.atn_executor_create(p.handle, C.byref(o), C.byref(h)) != 0
+    assert b"batched" in L.atn_last_error()
+    o = _lib.ExecOptions(1, 0, 1, 0, 0, 0, 0, 0, 2, 9)  # beyond the 4 slices
+    assert L.atn_executor_create(p.handle, C.byref(o), C.byref(h)) != 0
+    assert b"out of bounds" in L.atn_last_error()

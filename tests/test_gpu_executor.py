@@ -184,3 +184,38 @@ def test_sliced_circuit_plain_batched_agree(cuda):
         assert abs(complex(got) - complex(want)) / abs(complex(want)) < 1e-5, mode
         outs.append(complex(got))
     assert outs[1] == outs[2]  # graph replay == eager batched, bit for bit
+
+
+def test_slice_ranges_partial_sums(cuda):
+    """Batched runs over global slice ranges return the partial sums of those
+    slices (one device's share, tnmpi.rs:279-284): internal-only ranges match the
+    oracle's slice-range contraction, and consecutive ranges add up to the whole."""
+    import torch
+
+    net, arrays, _ = W.random_circuit_amplitude(4, 4, 8, 0)
+    path = P.greedy(net)
+    sliced = P.find_slices_to_size(net, path, 2 ** 7)
+    a128 = [a.astype(np.complex128) for a in arrays]
+    plan = Plan(net["inputs"], net["output"], net["size_dict"], path, sliced)
+    n = plan.info()["n_slices"]
+    dev = [torch.from_numpy(a).cuda() for a in a128]
+    cuts = [0, 1, n // 3, n // 2 + 1, n]
+    total = 0
+    for lo, hi in zip(cuts[:-1], cuts[1:]):
+        for graph in (False, True):
+            got, _ = plan.run(dev, 3, batched=True, use_graph=graph, slice_range=(lo, hi), max_batch=5)
+            ref = O.contract_slice_range(net, a128, path, sliced, lo, hi)
+            assert abs(complex(got.cpu()) - complex(ref)) <= 1e-12 * max(abs(complex(ref)), 1e-300), (lo, hi, graph)
+        total += complex(got.cpu())
+    full, _ = plan.run(dev, 3, batched=True)
+    assert abs(total - complex(full.cpu())) <= 1e-12 * abs(complex(full.cpu()))
+    # output-sliced plan: ranges may cross block boundaries
+    fx = FX["mixed_ragged_chunks"]
+    fnet = F.net_of(fx)
+    arrs = [torch.from_numpy(a).cuda() for a in W.random_arrays(fnet, np.float64, 3)]
+    fplan = Plan(fnet["inputs"], fnet["output"], fnet["size_dict"], fx["ssa_path"], fx["sliced"])
+    nf = fplan.info()["n_slices"]
+    whole, _ = fplan.run(arrs, 1, batched=True)
+    parts = sum(fplan.run(arrs, 1, batched=True, slice_range=r)[0].cpu().numpy()
+                for r in [(0, 100), (100, nf // 2 + 7), (nf // 2 + 7, nf)])
+    np.testing.assert_allclose(parts, whole.cpu().numpy(), rtol=1e-12, atol=1e-12)
