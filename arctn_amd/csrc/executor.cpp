@@ -170,7 +170,7 @@ void Executor::gen_gemm(const PairStep& st, size_t nbo, bool va, bool vb, const 
                        ((g.sbo || g.nbo == 1) ? nz : (double)g.nbi) * (double)(g.k * g.n) + nz * (double)(g.m * g.n));
   Ref scr{Ref::WS, scratch_buf_};
   size_t& scr_bytes = bufs_[scratch_buf_].bytes;
-  for (int kern : {GEMM_AUTO, GEMM_SIMPLE, GEMM_SKINNY, GEMM_MFMA, GEMM_SPLITK})  // autotune candidates
+  for (int kern : {GEMM_AUTO, GEMM_SIMPLE, GEMM_SKINNY, GEMM_MFMA, GEMM_SPLITK, GEMM_PRESPLIT})  // autotune candidates
     scr_bytes = std::max(scr_bytes, gemm_scratch_bytes(dt, prec, kern, g));
   const int sb = scratch_buf_;
   emit(
@@ -578,8 +578,9 @@ void Executor::autotune(const Ctx& c, hipStream_t s) {
     q.scratch_bytes = bufs_[scratch_buf_].bytes;
     float best_ms = 1e30f;
     int choice = GEMM_SIMPLE;
-    for (int cand : {GEMM_SIMPLE, GEMM_SKINNY, GEMM_MFMA, GEMM_SPLITK}) {
+    for (int cand : {GEMM_SIMPLE, GEMM_SKINNY, GEMM_MFMA, GEMM_SPLITK, GEMM_PRESPLIT}) {
       if (cand == GEMM_SPLITK && q.k < 256) continue;
+      if (cand == GEMM_PRESPLIT && o_.precision == NATIVE) continue;
       launch_gemm(o_.dtype, o_.precision, cand, q, s);
       ATN_HIP(hipEventRecord(e0, s));
       for (int r = 0; r < 3; ++r) launch_gemm(o_.dtype, o_.precision, cand, q, s);

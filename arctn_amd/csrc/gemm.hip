@@ -10,6 +10,7 @@
 // The expansion is built while staging B into LDS, so HBM traffic is that of
 // the complex operands; the MFMA work is 8*m*k*n real FLOPs as for 4M.
 #include <algorithm>
+#include <type_traits>
 
 #include "kernels.hpp"
 
@@ -674,6 +675,243 @@ __global__ __launch_bounds__(256, 2) void gemm_split_kernel(GemmArgs g) {
 constexpr size_t SPLIT_LDS = 4 * 128 * (64 + 8) * 2;
 
 // ------------------------------------------------------------------ dispatch --
+// ------------------------------------------------ pre-split operand GEMM --
+// Each operand is split once into (hi, lo) 16-bit arrays by a memory-bound
+// pass, so the GEMM kernel only copies 16-byte chunks into LDS and issues
+// MFMAs (no per-tile conversion work, and the transposition of B happens in
+// the pass, not in LDS stores). Layouts, with Kp = K' rounded up to 8 halves
+// and zero padded:
+//   A: [za][m][Kp]            (complex A read as real [m][2k])
+//   B: [zb][n][Kp] transposed (complex: row j holds re, im of B[p][j] at 2p, 2p+1)
+// za / zb count the distinct operand matrices of the batch (1 when shared).
+struct PreSplit {
+  const void *ahi, *alo, *bhi, *blo;
+  int64_t kp;
+};
+
+static __host__ __device__ inline int64_t n_mats(int64_t so, int64_t si, int64_t nbo, int64_t nbi) {
+  return (so ? nbo : 1) * (si ? nbi : 1);
+}
+
+// A -> [za][m][Kp] hi/lo; one thread per 4 consecutive halves of a row.
+template <bool CPLX, bool F16>
+__global__ __launch_bounds__(256) void split_a_kernel(GemmArgs g, PreSplit ps) {
+  using H = typename Half<F16>::H;
+  const int64_t K = CPLX ? 2 * g.k : g.k, lda = CPLX ? 2 * g.lda : g.lda, kp = ps.kp;
+  const int64_t inner = g.sai ? g.nbi : 1, nm = n_mats(g.sao, g.sai, g.nbo, g.nbi);
+  const int64_t per_row = kp / 4, total = nm * g.m * per_row;
+  float sa = 1.f;
+  if constexpr (F16) sa = ldexpf(1.f, pow2_shift(((const unsigned long long*)g.scratch)[0]));
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t mat = e / (g.m * per_row), r = e - mat * g.m * per_row, row = r / per_row, c = (r - row * per_row) * 4;
+    const int64_t zo = mat / inner, zi = mat - zo * inner;
+    const float* src = (const float*)g.a + (zo * g.sao + zi * g.sai) * (CPLX ? 2 : 1) + row * lda;
+    H h[4], l[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      float x = c + u < K ? src[c + u] * sa : 0.f;
+      split16<F16>(x, h[u], l[u]);
+    }
+    const int64_t off = (mat * g.m + row) * kp + c;
+    *reinterpret_cast<uint2*>((H*)ps.ahi + off) = *reinterpret_cast<const uint2*>(h);
+    *reinterpret_cast<uint2*>((H*)ps.alo + off) = *reinterpret_cast<const uint2*>(l);
+  }
+}
+
+// B [k][n] -> transposed [zb][n][Kp] hi/lo through a 32 (k) x 64 (n) LDS tile.
+template <bool CPLX, bool F16>
+__global__ __launch_bounds__(256) void split_bt_kernel(GemmArgs g, PreSplit ps) {
+  using H = typename Half<F16>::H;
+  using E = typename std::conditional<CPLX, float2, float>::type;
+  constexpr int TK = 32, TN = 64, EH = CPLX ? 2 : 1;  // halves per element
+  __shared__ E tile[TK][TN + 1];
+  const int64_t kp = ps.kp, inner = g.sbi ? g.nbi : 1, nm = n_mats(g.sbo, g.sbi, g.nbo, g.nbi);
+  const int64_t tk = (kp / EH + TK - 1) / TK, tn = (g.n + TN - 1) / TN, total = nm * tk * tn;
+  float sb = 1.f;
+  if constexpr (F16) sb = ldexpf(1.f, pow2_shift(((const unsigned long long*)g.scratch)[1]));
+  const int t = threadIdx.x;
+  for (int64_t b = blockIdx.x; b < total; b += gridDim.x) {
+    const int64_t mat = b / (tk * tn), r = b - mat * tk * tn, k0 = (r / tn) * TK, n0 = (r - (r / tn) * tn) * TN;
+    const int64_t zo = mat / inner, zi = mat - zo * inner;
+    const E* src = (const E*)g.b + zo * g.sbo + zi * g.sbi;
+#pragma unroll
+    for (int s = 0; s < TK / 4; ++s) {
+      const int kk = t / TN + 4 * s, nn = t % TN;
+      const int64_t gk = k0 + kk, gn = n0 + nn;
+      E v{};
+      if (gk < g.k && gn < g.n) v = src[gk * g.ldb + gn];
+      tile[kk][nn] = v;
+    }
+    __syncthreads();
+    {  // thread -> (row n, 8 consecutive k elements)
+      const int nn = t / 4, kc = (t % 4) * 8;
+      const int64_t gn = n0 + nn;
+      if (gn < g.n) {
+        H h[8 * EH], l[8 * EH];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+          E v = tile[kc + i][nn];
+          if constexpr (CPLX) {
+            split16<F16>(v.x * sb, h[2 * i], l[2 * i]);
+            split16<F16>(v.y * sb, h[2 * i + 1], l[2 * i + 1]);
+          } else {
+            split16<F16>(v * sb, h[i], l[i]);
+          }
+        }
+        const int64_t col = (k0 + kc) * EH;  // halves
+        if (col < kp) {
+          const int64_t off = (mat * g.n + gn) * kp + col;
+          // kp is a multiple of 8 halves and col of 8: whole 16-byte chunks stay in the row
+#pragma unroll
+          for (int c = 0; c < EH; ++c)
+            if (col + 8 * c < kp) {
+              *reinterpret_cast<uint4*>((H*)ps.bhi + off + 8 * c) = *reinterpret_cast<const uint4*>(h + 8 * c);
+              *reinterpret_cast<uint4*>((H*)ps.blo + off + 8 * c) = *reinterpret_cast<const uint4*>(l + 8 * c);
+            }
+        }
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// Complex B' rows from a compact chunk of (re, im) pairs: row 2j = (re, -im),
+// row 2j + 1 = (im, re). 16-bit halves, little-endian pairs in 32-bit words.
+__device__ inline uint4 neg_im(uint4 v) { return make_uint4(v.x ^ 0x80000000u, v.y ^ 0x80000000u, v.z ^ 0x80000000u, v.w ^ 0x80000000u); }
+__device__ inline uint32_t swap16(uint32_t x) { return (x >> 16) | (x << 16); }
+__device__ inline uint4 swap_pairs(uint4 v) { return make_uint4(swap16(v.x), swap16(v.y), swap16(v.z), swap16(v.w)); }
+
+template <bool CPLX, bool F16>
+__global__ __launch_bounds__(256, 2) void gemm_presplit_kernel(GemmArgs g, PreSplit ps) {
+  using H = typename Half<F16>::H;
+  using V = typename Half<F16>::V;
+  constexpr int BM = 128, BN = 128, BK = 64;
+  constexpr int WM = 64, WN = 64, TM = 4, TN = 4;
+  constexpr int SK = BK + 8;  // row stride in halves (144 B): conflict-free 16-byte rows
+  constexpr int A_Q = BM * (BK / 8) / 256;                       // 16-byte A chunks per thread (4)
+  constexpr int B_Q = (CPLX ? BN / 2 : BN) * (BK / 8) / 256;     // 16-byte B chunks per thread (2 / 4)
+  extern __shared__ __align__(16) char smem_split[];
+  H* ahi = reinterpret_cast<H*>(smem_split);
+  H* alo = ahi + BM * SK;
+  H* bhi = alo + BM * SK;
+  H* blo = bhi + BN * SK;
+
+  float unscale = 1.f;
+  if constexpr (F16) {
+    const unsigned long long* mb = (const unsigned long long*)g.scratch;
+    unscale = ldexpf(1.f, -(pow2_shift(mb[0]) + pow2_shift(mb[1])));
+  }
+  const int64_t M = g.m, N = CPLX ? 2 * g.n : g.n, kp = ps.kp;
+  const int64_t ldc = CPLX ? 2 * g.ldc : g.ldc;
+  const int tiles_n = (int)((N + BN - 1) / BN), tiles_m = (int)((M + BM - 1) / BM);
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm0 = (wave >> 1) * WM, wn0 = (wave & 1) * WN;
+  const uint64_t nz = (uint64_t)g.nbo * g.nbi, ntile = (uint64_t)tiles_m * tiles_n;
+  const int64_t ainner = g.sai ? g.nbi : 1, binner = g.sbi ? g.nbi : 1;
+
+  for (uint64_t z = blockIdx.y; z < nz; z += gridDim.y) {
+    const int64_t zo = (int64_t)(z / g.nbi), zi = (int64_t)(z - zo * g.nbi);
+    const int64_t amat = (g.sao ? zo : 0) * ainner + (g.sai ? zi : 0);
+    const int64_t bmat = (g.sbo ? zo : 0) * binner + (g.sbi ? zi : 0);
+    const H* Ah = (const H*)ps.ahi + amat * M * kp;
+    const H* Al = (const H*)ps.alo + amat * M * kp;
+    const H* Bh = (const H*)ps.bhi + bmat * g.n * kp;
+    const H* Bl = (const H*)ps.blo + bmat * g.n * kp;
+    float* C = (float*)g.c + (zo * g.sco + zi * g.sci) * (CPLX ? 2 : 1);
+    for (uint64_t t = blockIdx.x; t < ntile; t += gridDim.x) {
+      const int64_t m0 = (int64_t)(t / tiles_n) * BM, n0 = (int64_t)(t % tiles_n) * BN;
+      v4f acc[TM][TN];
+#pragma unroll
+      for (int i = 0; i < TM; ++i)
+#pragma unroll
+        for (int j = 0; j < TN; ++j) acc[i][j] = v4f{0, 0, 0, 0};
+      uint4 rah[A_Q], ral[A_Q], rbh[B_Q], rbl[B_Q];
+      auto load = [&](int64_t k0) {
+#pragma unroll
+        for (int u = 0; u < A_Q; ++u) {
+          const int q = tid + 256 * u, row = q >> 3, ch = (q & 7) * 8;
+          const int64_t gr = m0 + row, gc = k0 + ch;
+          const bool ok = gr < M && gc < kp;
+          rah[u] = ok ? *reinterpret_cast<const uint4*>(Ah + gr * kp + gc) : make_uint4(0, 0, 0, 0);
+          ral[u] = ok ? *reinterpret_cast<const uint4*>(Al + gr * kp + gc) : make_uint4(0, 0, 0, 0);
+        }
+#pragma unroll
+        for (int u = 0; u < B_Q; ++u) {
+          const int q = tid + 256 * u, row = q >> 3, ch = (q & 7) * 8;
+          const int64_t gr = (CPLX ? n0 / 2 : n0) + row, gc = k0 + ch;
+          const bool ok = gr < g.n && gc < kp;
+          rbh[u] = ok ? *reinterpret_cast<const uint4*>(Bh + gr * kp + gc) : make_uint4(0, 0, 0, 0);
+          rbl[u] = ok ? *reinterpret_cast<const uint4*>(Bl + gr * kp + gc) : make_uint4(0, 0, 0, 0);
+        }
+      };
+      auto store = [&]() {
+#pragma unroll
+        for (int u = 0; u < A_Q; ++u) {
+          const int q = tid + 256 * u, row = q >> 3, ch = (q & 7) * 8;
+          *reinterpret_cast<uint4*>(ahi + row * SK + ch) = rah[u];
+          *reinterpret_cast<uint4*>(alo + row * SK + ch) = ral[u];
+        }
+#pragma unroll
+        for (int u = 0; u < B_Q; ++u) {
+          const int q = tid + 256 * u, row = q >> 3, ch = (q & 7) * 8;
+          if constexpr (CPLX) {
+            *reinterpret_cast<uint4*>(bhi + (2 * row) * SK + ch) = neg_im(rbh[u]);
+            *reinterpret_cast<uint4*>(blo + (2 * row) * SK + ch) = neg_im(rbl[u]);
+            *reinterpret_cast<uint4*>(bhi + (2 * row + 1) * SK + ch) = swap_pairs(rbh[u]);
+            *reinterpret_cast<uint4*>(blo + (2 * row + 1) * SK + ch) = swap_pairs(rbl[u]);
+          } else {
+            *reinterpret_cast<uint4*>(bhi + row * SK + ch) = rbh[u];
+            *reinterpret_cast<uint4*>(blo + row * SK + ch) = rbl[u];
+          }
+        }
+      };
+      const int64_t nk = (kp + BK - 1) / BK;
+      load(0);
+      store();
+      __syncthreads();
+      const int cl = lane & 15;
+      for (int64_t kt = 0; kt < nk; ++kt) {
+        if (kt + 1 < nk) load((kt + 1) * BK);
+#pragma unroll
+        for (int ks = 0; ks < BK; ks += 32) {
+          const int kg = ks + (lane >> 4) * 8;
+          V ah[TM], al[TM];
+#pragma unroll
+          for (int i = 0; i < TM; ++i) {
+            ah[i] = *reinterpret_cast<const V*>(&ahi[(wm0 + i * 16 + cl) * SK + kg]);
+            al[i] = *reinterpret_cast<const V*>(&alo[(wm0 + i * 16 + cl) * SK + kg]);
+          }
+#pragma unroll
+          for (int j = 0; j < TN; ++j) {
+            V bh = *reinterpret_cast<const V*>(&bhi[(wn0 + j * 16 + cl) * SK + kg]);
+            V bl = *reinterpret_cast<const V*>(&blo[(wn0 + j * 16 + cl) * SK + kg]);
+#pragma unroll
+            for (int i = 0; i < TM; ++i) {
+              acc[i][j] = Half<F16>::mfma(al[i], bh, acc[i][j]);
+              acc[i][j] = Half<F16>::mfma(ah[i], bl, acc[i][j]);
+              acc[i][j] = Half<F16>::mfma(ah[i], bh, acc[i][j]);
+            }
+          }
+        }
+        __syncthreads();
+        if (kt + 1 < nk) {
+          store();
+          __syncthreads();
+        }
+      }
+#pragma unroll
+      for (int i = 0; i < TM; ++i)
+#pragma unroll
+        for (int j = 0; j < TN; ++j)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            int64_t row = m0 + wm0 + i * 16 + 4 * (lane >> 4) + r, col = n0 + wn0 + j * 16 + cl;
+            if (row < M && col < N) C[row * ldc + col] = acc[i][j][r] * unscale;
+          }
+    }
+  }
+}
+
 template <class F>
 static void dispatch_dt(int dt, F&& f) {
   switch (dt) {
@@ -809,15 +1047,37 @@ static bool splitk_worth(const GemmArgs& g) {
   return g.k >= 256 && par < 32768;
 }
 
-static int auto_kernel(int dt, const GemmArgs& g) {
-  if (mfma_worth(dt, g)) return GEMM_MFMA;
+// Scratch of the pre-split operands (each half array 256-byte aligned).
+struct PreSplitSizes {
+  int64_t kp;
+  size_t a_half, b_half, total;
+};
+static PreSplitSizes presplit_sizes(int dt, const GemmArgs& g) {
+  PreSplitSizes r;
+  const int64_t K = dt == C64 ? 2 * g.k : g.k;
+  r.kp = (K + 7) / 8 * 8;
+  auto r256 = [](size_t b) { return (b + 255) / 256 * 256; };
+  r.a_half = r256((size_t)n_mats(g.sao, g.sai, g.nbo, g.nbi) * (size_t)g.m * (size_t)r.kp * 2);
+  r.b_half = r256((size_t)n_mats(g.sbo, g.sbi, g.nbo, g.nbi) * (size_t)g.n * (size_t)r.kp * 2);
+  r.total = 2 * r.a_half + 2 * r.b_half;
+  return r;
+}
+
+static int auto_kernel(int dt, int prec, const GemmArgs& g) {
+  if (mfma_worth(dt, g)) {
+    // Splitting inside the kernel repeats B's conversion once per 128-row tile
+    // of A (and A's once per column tile and batch entry): with three or more
+    // row tiles one separate splitting pass is cheaper.
+    const bool split_prec = (prec == BF16X3 || prec == FP16X2) && (dt == F32 || dt == C64);
+    return split_prec && (g.m + 127) / 128 >= 3 ? GEMM_PRESPLIT : GEMM_MFMA;
+  }
   if (splitk_worth(g)) return GEMM_SPLITK;
   if ((g.m <= 32 && g.n >= 64) || (g.n <= 32 && g.m >= 64)) return GEMM_SKINNY;
   return GEMM_SIMPLE;
 }
 
 int launch_gemm(int dt, int prec, int kernel, const GemmArgs& g, hipStream_t s) {
-  if (kernel == GEMM_AUTO) kernel = auto_kernel(dt, g);
+  if (kernel == GEMM_AUTO) kernel = auto_kernel(dt, prec, g);
   if (kernel == GEMM_SPLITK) {
     run_splitk(dt, g, s);
     return GEMM_SPLITK;
@@ -831,7 +1091,11 @@ int launch_gemm(int dt, int prec, int kernel, const GemmArgs& g, hipStream_t s) 
     return GEMM_SIMPLE;
   }
   if (prec == C3M) throw Error("precision c3m is not available in this build");
-  if ((prec == BF16X3 || prec == FP16X2) && (dt == F32 || dt == C64)) {
+  const bool split_prec = (prec == BF16X3 || prec == FP16X2) && (dt == F32 || dt == C64);
+  if (kernel == GEMM_PRESPLIT && !split_prec) kernel = GEMM_MFMA;
+  if (split_prec) {
+    const PreSplitSizes psz = presplit_sizes(dt, g);
+    const bool pre = kernel == GEMM_PRESPLIT && g.scratch && g.scratch_bytes >= 256 + psz.total;
     int64_t N = dt == C64 ? 2 * g.n : g.n;
     uint64_t tiles = (uint64_t)((g.m + 127) / 128) * (uint64_t)((N + 127) / 128);
     dim3 grid((unsigned)std::min<uint64_t>(tiles, 1u << 20), zgrid(g));
@@ -847,9 +1111,37 @@ int launch_gemm(int dt, int prec, int kernel, const GemmArgs& g, hipStream_t s) 
     } else {
       g_last_kind = K_GEMM_MFMA_BF16X3;
     }
+    bool f16 = prec == FP16X2, cx = dt == C64;
+    if (pre) {  // operands split once into 16-bit (hi, lo) arrays, then the copy-only GEMM
+      PreSplit ps;
+      char* base = (char*)g.scratch + 256;
+      ps.ahi = base, ps.alo = base + psz.a_half, ps.bhi = base + 2 * psz.a_half, ps.blo = base + 2 * psz.a_half + psz.b_half;
+      ps.kp = psz.kp;
+      const int64_t na = n_mats(g.sao, g.sai, g.nbo, g.nbi) * g.m * psz.kp / 4;
+      const int64_t tiles_b = n_mats(g.sbo, g.sbi, g.nbo, g.nbi) * ((psz.kp / (cx ? 2 : 1) + 31) / 32) * ((g.n + 63) / 64);
+      const dim3 ga((unsigned)std::min<int64_t>((na + 255) / 256, 65536)), gb((unsigned)std::min<int64_t>(tiles_b, 65536));
+#define PRESPLIT_LAUNCH(C_, F_)                                                                            \
+  do {                                                                                                     \
+    static bool attr_set = false;                                                                          \
+    if (!attr_set) {                                                                                       \
+      ATN_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&gemm_presplit_kernel<C_, F_>),            \
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)SPLIT_LDS));            \
+      attr_set = true;                                                                                     \
+    }                                                                                                      \
+    LAUNCH((split_a_kernel<C_, F_>), ga, dim3(256), 0, s, g, ps);                                         \
+    LAUNCH((split_bt_kernel<C_, F_>), gb, dim3(256), 0, s, g, ps);                                        \
+    g_last_kind = F_ ? K_GEMM_MFMA_FP16X2 : K_GEMM_MFMA_BF16X3;                                           \
+    LAUNCH((gemm_presplit_kernel<C_, F_>), grid, dim3(256), SPLIT_LDS, s, g, ps);                         \
+  } while (0)
+      if (cx && f16) PRESPLIT_LAUNCH(true, true);
+      else if (cx) PRESPLIT_LAUNCH(true, false);
+      else if (f16) PRESPLIT_LAUNCH(false, true);
+      else PRESPLIT_LAUNCH(false, false);
+#undef PRESPLIT_LAUNCH
+      return GEMM_PRESPLIT;
+    }
     int64_t cf = dt == C64 ? 2 : 1;
     bool vec = (cf * g.lda) % 4 == 0 && (uintptr_t)g.a % 16 == 0 && (cf * g.sai) % 4 == 0 && (cf * g.sao) % 4 == 0;
-    bool f16 = prec == FP16X2, cx = dt == C64;
     // 72 KiB of dynamic LDS (gfx950 has 160 KiB per CU): raise the per-kernel limit once.
 #define SPLIT_LAUNCH(C_, F_, V_)                                                                          \
   do {                                                                                                   \
@@ -888,11 +1180,13 @@ int launch_gemm(int dt, int prec, int kernel, const GemmArgs& g, hipStream_t s) 
 }
 
 size_t gemm_scratch_bytes(int dt, int prec, int kernel, const GemmArgs& g) {
-  if (kernel == GEMM_AUTO) kernel = auto_kernel(dt, g);
+  if (kernel == GEMM_AUTO) kernel = auto_kernel(dt, prec, g);
   KSplit ks;
   if (kernel == GEMM_SPLITK) ks = plan_splitk(dt, g, MAX_PARTIAL_BYTES);
   else if (kernel == GEMM_MFMA) ks = plan_mfma_split(dt, prec, g, MAX_PARTIAL_BYTES);
   size_t part = ks.n_split > 1 ? (size_t)ks.n_split * dtype_size(dt) * (size_t)(g.nbo * g.nbi * g.m * g.n) : 0;
+  if (kernel == GEMM_PRESPLIT && (prec == BF16X3 || prec == FP16X2) && (dt == F32 || dt == C64))
+    part = presplit_sizes(dt, g).total;
   return 256 + part;
 }
 

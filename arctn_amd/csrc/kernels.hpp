@@ -110,7 +110,10 @@ struct GemmArgs {
   int64_t kc = 0;
   void* part = nullptr;
 };
-enum GemmKernel { GEMM_AUTO = -1, GEMM_SIMPLE = 0, GEMM_MFMA = 1, GEMM_SKINNY = 2, GEMM_SPLITK = 3 };
+// GEMM_PRESPLIT: split-precision MFMA GEMM whose operands are split into
+// 16-bit (hi, lo) arrays by a separate pass (GEMM_MFMA splits them while staging
+// each tile); for native precision it is GEMM_MFMA.
+enum GemmKernel { GEMM_AUTO = -1, GEMM_SIMPLE = 0, GEMM_MFMA = 1, GEMM_SKINNY = 2, GEMM_SPLITK = 3, GEMM_PRESPLIT = 4 };
 // Returns the kernel used.
 int launch_gemm(int dt, int prec, int kernel, const GemmArgs& g, hipStream_t s);
 // Device scratch launch_gemm can use for this GEMM (FP16X2 scales, split-K partials).

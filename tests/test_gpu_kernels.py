@@ -91,16 +91,16 @@ PREC_CASES = [(np.float32, 0, 1e-6), (np.float64, 0, 1e-14), (np.complex64, 0, 1
 
 
 @pytest.mark.parametrize("case", range(len(PREC_CASES)))
-@pytest.mark.parametrize("kernel", [0, 1, 2, 3])
+@pytest.mark.parametrize("kernel", [0, 1, 2, 3, 4])
 def test_gemm_against_numpy(cuda, case, kernel):
-    """Simple (0), MFMA (1, native or split precision), skinny (2) and split-K (3)
-    GEMMs vs complex128 NumPy."""
+    """Simple (0), MFMA (1, native or split precision), skinny (2), split-K (3)
+    and pre-split MFMA (4) GEMMs vs complex128 NumPy."""
     import torch
     from arctn_amd import _lib
 
     dtype, prec, eps = PREC_CASES[case]
-    if prec and kernel != 1:
-        pytest.skip("split precision applies to the MFMA kernel")
+    if prec and kernel not in (1, 4):
+        pytest.skip("split precision applies to the MFMA kernels")
     code = {np.float32: 0, np.float64: 1, np.complex64: 2, np.complex128: 3}[dtype]
     rng = np.random.default_rng(5)
     for bt, m, k, n in GEMM_SHAPES:
@@ -129,10 +129,10 @@ def test_gemm_broadcast_batch(cuda, dtype):
     a = _rand((m, k), dtype, rng)
     b = _rand((bt, k, n), dtype, rng)
     want = np.matmul(a.astype(np.complex128)[None], b.astype(np.complex128))
-    for kernel in (0, 1, 3):
+    for kernel, prec in ((0, 0), (1, 0), (3, 0), (1, 2), (1, 3), (4, 2), (4, 3)):
         ta, tb = torch.from_numpy(a).cuda(), torch.from_numpy(b).cuda()
         tc = torch.empty((bt, m, n), dtype=ta.dtype, device="cuda")
-        _lib.check(_lib.lib().atn_gemm(code, 0, kernel, bt, m, k, n, ta.data_ptr(), 0, tb.data_ptr(), k * n,
+        _lib.check(_lib.lib().atn_gemm(code, prec, kernel, bt, m, k, n, ta.data_ptr(), 0, tb.data_ptr(), k * n,
                                        tc.data_ptr(), m * n, None))
         err = np.abs(tc.cpu().numpy() - want).max() / np.abs(want).max()
-        assert err < 1e-5
+        assert err < 1e-5, (kernel, prec, err)
