@@ -302,7 +302,7 @@ int atn_gemm(int dtype, int precision, int kernel, size_t batch, size_t m, size_
     g.lda = g.k, g.ldb = g.n, g.ldc = g.n;
     g.nbo = (int64_t)batch, g.nbi = 1;
     g.sao = (int64_t)stride_a, g.sbo = (int64_t)stride_b, g.sco = (int64_t)stride_c;
-    if (kernel < -1 || kernel > 4) throw Error("unknown GEMM kernel");
+    if (kernel < -1 || kernel > 7) throw Error("unknown GEMM kernel");
     if (precision < 0 || precision > 3) throw Error("unsupported precision code");
     hipStream_t s = static_cast<hipStream_t>(stream);
     void* scr = nullptr;

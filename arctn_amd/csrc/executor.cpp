@@ -170,7 +170,8 @@ void Executor::gen_gemm(const PairStep& st, size_t nbo, bool va, bool vb, const 
                        ((g.sbo || g.nbo == 1) ? nz : (double)g.nbi) * (double)(g.k * g.n) + nz * (double)(g.m * g.n));
   Ref scr{Ref::WS, scratch_buf_};
   size_t& scr_bytes = bufs_[scratch_buf_].bytes;
-  for (int kern : {GEMM_AUTO, GEMM_SIMPLE, GEMM_SKINNY, GEMM_MFMA, GEMM_SPLITK, GEMM_PRESPLIT})  // autotune candidates
+  for (int kern : {GEMM_AUTO, GEMM_SIMPLE, GEMM_SKINNY, GEMM_MFMA, GEMM_SPLITK, GEMM_PRESPLIT, GEMM_MFMA_R32,
+                   GEMM_MFMA_R64, GEMM_MFMA_R128})  // autotune candidates
     scr_bytes = std::max(scr_bytes, gemm_scratch_bytes(dt, prec, kern, g));
   const int sb = scratch_buf_;
   emit(
@@ -560,13 +561,15 @@ void Executor::autotune(const Ctx& c, hipStream_t s) {
   // Time the candidate kernels of every GEMM site on its real operands once,
   // keyed by shape, and keep the fastest (the executor's workspace holds
   // operands of the right size; values do not influence timing).
-  std::map<std::tuple<int64_t, int64_t, int64_t, int64_t, int64_t>, int> best;
+  std::map<std::tuple<int64_t, int64_t, int64_t, int64_t, int64_t, int>, int> best;
   hipEvent_t e0, e1;
   ATN_HIP(hipEventCreate(&e0));
   ATN_HIP(hipEventCreate(&e1));
   for (size_t i = 0; i < gemm_sites_.size(); ++i) {
     auto& site = gemm_sites_[i];
-    auto key = std::make_tuple(site.g.m, site.g.k, site.g.n, site.g.nbo, site.g.nbi);
+    // shared (stride-0) operands change the memory traffic: part of the key
+    const int shared = (site.g.sao == 0) | (site.g.sai == 0) << 1 | (site.g.sbo == 0) << 2 | (site.g.sbi == 0) << 3;
+    auto key = std::make_tuple(site.g.m, site.g.k, site.g.n, site.g.nbo, site.g.nbi, shared);
     auto it = best.find(key);
     if (it != best.end()) {
       gemm_choice_[i] = it->second;
@@ -578,9 +581,13 @@ void Executor::autotune(const Ctx& c, hipStream_t s) {
     q.scratch_bytes = bufs_[scratch_buf_].bytes;
     float best_ms = 1e30f;
     int choice = GEMM_SIMPLE;
-    for (int cand : {GEMM_SIMPLE, GEMM_SKINNY, GEMM_MFMA, GEMM_SPLITK, GEMM_PRESPLIT}) {
+    const bool split_prec = o_.precision != NATIVE;
+    for (int cand : {GEMM_SIMPLE, GEMM_SKINNY, GEMM_SPLITK, GEMM_PRESPLIT, GEMM_MFMA_R32, GEMM_MFMA_R64, GEMM_MFMA_R128}) {
       if (cand == GEMM_SPLITK && q.k < 256) continue;
-      if (cand == GEMM_PRESPLIT && o_.precision == NATIVE) continue;
+      if (cand == GEMM_PRESPLIT && !split_prec) continue;
+      // split precision has one in-kernel-split MFMA tile shape
+      if (split_prec && (cand == GEMM_MFMA_R32 || cand == GEMM_MFMA_R64)) continue;
+      if ((o_.dtype == F64 || o_.dtype == C128) && cand == GEMM_MFMA_R128) continue;
       launch_gemm(o_.dtype, o_.precision, cand, q, s);
       ATN_HIP(hipEventRecord(e0, s));
       for (int r = 0; r < 3; ++r) launch_gemm(o_.dtype, o_.precision, cand, q, s);

@@ -122,6 +122,73 @@ __global__ __launch_bounds__(256) void gemm_smallm_kernel(GemmArgs g) {
   }
 }
 
+// Streaming small-m GEMM: each thread owns V consecutive columns (16-byte
+// loads and stores) and all m rows; KU rows of B are loaded before their
+// multiply-adds, so every thread keeps KU 16-byte loads in flight. A (m x k) is
+// wave-uniform and read through the scalar cache. B and C are streamed once.
+template <class T, int V> struct Vec;
+template <> struct Vec<float, 4> { using W = float4; };
+template <> struct Vec<float, 2> { using W = float2; };
+template <> struct Vec<float, 1> { using W = float; };
+template <> struct Vec<float2, 2> { using W = float4; };
+template <> struct Vec<float2, 1> { using W = float2; };
+template <> struct Vec<double, 2> { using W = double2; };
+template <> struct Vec<double, 1> { using W = double; };
+template <> struct Vec<double2, 1> { using W = double2; };
+
+template <class T, int MM, int V>
+__global__ __launch_bounds__(256) void gemm_stream_kernel(GemmArgs g) {
+  constexpr int KU = 8;
+  using W = typename Vec<T, V>::W;
+  static_assert(sizeof(W) == V * sizeof(T), "vector width");
+  const uint64_t nz = (uint64_t)g.nbo * g.nbi;
+  for (uint64_t z = blockIdx.y; z < nz; z += gridDim.y) {
+    const uint64_t zo = z / g.nbi, zi = z - zo * g.nbi;
+    const T* A = (const T*)g.a + zo * g.sao + zi * g.sai;
+    const T* B = (const T*)g.b + zo * g.sbo + zi * g.sbi;
+    T* C = (T*)g.c + zo * g.sco + zi * g.sci;
+    for (int64_t j0 = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) * V; j0 < g.n;
+         j0 += (int64_t)gridDim.x * blockDim.x * V) {
+      T acc[MM][V];
+#pragma unroll
+      for (int i = 0; i < MM; ++i)
+#pragma unroll
+        for (int v = 0; v < V; ++v) acc[i][v] = CM<T>::zero();
+      int64_t p = 0;
+      for (; p + KU <= g.k; p += KU) {
+        W b[KU];
+#pragma unroll
+        for (int u = 0; u < KU; ++u) b[u] = *reinterpret_cast<const W*>(B + (p + u) * g.ldb + j0);
+#pragma unroll
+        for (int u = 0; u < KU; ++u) {
+          const T* bu = reinterpret_cast<const T*>(&b[u]);
+#pragma unroll
+          for (int i = 0; i < MM; ++i)
+            if (i < g.m) {
+              const T a = A[i * g.lda + p + u];
+#pragma unroll
+              for (int v = 0; v < V; ++v) acc[i][v] = CM<T>::mac(acc[i][v], a, bu[v]);
+            }
+        }
+      }
+      for (; p < g.k; ++p) {
+        W bw = *reinterpret_cast<const W*>(B + p * g.ldb + j0);
+        const T* bu = reinterpret_cast<const T*>(&bw);
+#pragma unroll
+        for (int i = 0; i < MM; ++i)
+          if (i < g.m) {
+            const T a = A[i * g.lda + p];
+#pragma unroll
+            for (int v = 0; v < V; ++v) acc[i][v] = CM<T>::mac(acc[i][v], a, bu[v]);
+          }
+      }
+#pragma unroll
+      for (int i = 0; i < MM; ++i)
+        if (i < g.m) *reinterpret_cast<W*>(C + i * g.ldc + j0) = *reinterpret_cast<const W*>(acc[i]);
+    }
+  }
+}
+
 // Small n (n <= NN): one thread per output row i keeps all n accumulators;
 // B (k x n) is wave-uniform.
 template <class T, int NN>
@@ -231,12 +298,31 @@ static KSplit plan_splitk(int dt, const GemmArgs& g, size_t avail) {
   return split_k(g.k, want, 1024, 256, dtype_size(dt) * (size_t)(nz * g.m * g.n), avail);
 }
 
-static KSplit plan_mfma_split(int dt, int prec, const GemmArgs& g, size_t avail) {
+// Row tile of the native MFMA kernel.
+static int mfma_rows(int dt, int64_t m) {
+  const bool f64 = dt == F64 || dt == C128;
+  if (m <= 32) return 32;
+  if (m <= 64 || f64) return 64;
+  return 128;
+}
+
+// Row tile of an MFMA kernel choice (GEMM_MFMA: from m).
+static int mfma_rows_for(int dt, int kernel, int64_t m) {
+  const bool f64 = dt == F64 || dt == C128;
+  switch (kernel) {
+    case GEMM_MFMA_R32: return 32;
+    case GEMM_MFMA_R64: return 64;
+    case GEMM_MFMA_R128: return f64 ? 64 : 128;
+    default: return mfma_rows(dt, m);
+  }
+}
+
+static KSplit plan_mfma_split(int dt, int prec, const GemmArgs& g, size_t avail, int bm) {
   if (prec != NATIVE) return KSplit{};
   const bool cplx = dtype_complex(dt);
-  const int64_t bm = (dt == F64 || dt == C128) ? 64 : 128;
+  const int64_t bn = (dt == F64 || dt == C128) ? 64 : 128;
   const int64_t N = cplx ? 2 * g.n : g.n;
-  const int64_t tiles = g.nbo * g.nbi * ((g.m + bm - 1) / bm) * ((N + bm - 1) / bm);
+  const int64_t tiles = g.nbo * g.nbi * ((g.m + bm - 1) / bm) * ((N + bn - 1) / bn);
   if (tiles >= 512 || g.nbo * g.nbi > 65535) return KSplit{};
   // about 1024 workgroups, each k range at least 512 long (a multiple of the k tile)
   const int64_t want = (1024 + tiles - 1) / tiles;
@@ -946,7 +1032,7 @@ template <class F, bool CPLX, int BM, int BN, int BK>
 static void run_mfma(int dt, const GemmArgs& g0, hipStream_t s) {
   static_assert(64 % BK == 0, "k ranges are multiples of 64");
   GemmArgs g = g0;
-  const KSplit ks = plan_mfma_split(dt, NATIVE, g, scratch_avail(g));
+  const KSplit ks = plan_mfma_split(dt, NATIVE, g, scratch_avail(g), BM);
   g.kc = ks.n_split > 1 ? ks.k_chunk : 0;
   g.part = ks.n_split > 1 ? (char*)g.scratch + 256 : nullptr;
   int64_t N = CPLX ? 2 * g.n : g.n;
@@ -967,10 +1053,13 @@ static void run_mfma(int dt, const GemmArgs& g0, hipStream_t s) {
   }
 }
 
+// The MFMA kernels zero-pad partial tiles; with 32-row tiles they pay off from
+// m = 16 (k of 16 real units) or m = 32 (k of 8), measured against the
+// streaming kernel on gfx950.
 static bool mfma_worth(int dt, const GemmArgs& g) {
   bool cplx = dtype_complex(dt);
   int64_t K = cplx ? 2 * g.k : g.k, N = cplx ? 2 * g.n : g.n;
-  return g.m >= 32 && N >= 32 && K >= 16;
+  return N >= 32 && ((g.m >= 32 && K >= 8) || (g.m >= 16 && K >= 16));
 }
 
 template <bool SMALL_M, int X>
@@ -986,7 +1075,70 @@ static void run_skinny_t(int dt, const GemmArgs& g, hipStream_t s) {
   });
 }
 
+template <class T, int MM, int V>
+static void run_stream_t(const GemmArgs& g, hipStream_t s) {
+  const int64_t cols = (g.n + V - 1) / V;
+  unsigned gx = (unsigned)std::min<int64_t>((cols + 255) / 256, 65536);
+  dim3 grid(gx, zgrid(g));
+  if ((uint64_t)grid.y * gx > (1u << 22)) grid.y = std::max<unsigned>(1, (1u << 22) / gx);
+  LAUNCH((gemm_stream_kernel<T, MM, V>), grid, dim3(256), 0, s, g);
+}
+
+// 16-byte column vectors when every row start of B and C is 16-byte aligned.
+static int stream_width(int dt, const GemmArgs& g) {
+  const int es = (int)dtype_size(dt);
+  int v = 16 / es;
+  auto ok = [&](int w) {
+    const int64_t b = (int64_t)w * es;
+    return g.n % w == 0 && g.ldb % w == 0 && g.ldc % w == 0 && g.sbo % w == 0 && g.sbi % w == 0 && g.sco % w == 0 &&
+           g.sci % w == 0 && (uintptr_t)g.b % b == 0 && (uintptr_t)g.c % b == 0;
+  };
+  while (v > 1 && !ok(v)) v /= 2;
+  return v;
+}
+
+// Small m streamed over n: m <= 64 (real) or m <= 32 (complex), accumulators
+// per thread kept at 64 words or fewer.
+static bool run_stream(int dt, const GemmArgs& g, hipStream_t s) {
+  const bool cplx = dtype_complex(dt);
+  if (g.n < 64 || g.m > (cplx ? 32 : 64)) return false;
+  g_last_kind = K_GEMM_SMALL;
+  const int w = stream_width(dt, g);
+  const int mm = g.m <= 4 ? 4 : g.m <= 8 ? 8 : g.m <= 16 ? 16 : g.m <= 32 ? 32 : 64;
+  switch (dt) {
+    case F32:
+      if (mm == 4) { if (w == 4) run_stream_t<float, 4, 4>(g, s); else run_stream_t<float, 4, 1>(g, s); }
+      else if (mm == 8) { if (w == 4) run_stream_t<float, 8, 4>(g, s); else run_stream_t<float, 8, 1>(g, s); }
+      else if (mm == 16) { if (w == 4) run_stream_t<float, 16, 4>(g, s); else run_stream_t<float, 16, 1>(g, s); }
+      else if (mm == 32) { if (w >= 2) run_stream_t<float, 32, 2>(g, s); else run_stream_t<float, 32, 1>(g, s); }
+      else run_stream_t<float, 64, 1>(g, s);
+      break;
+    case C64:
+      if (mm == 4) { if (w == 2) run_stream_t<float2, 4, 2>(g, s); else run_stream_t<float2, 4, 1>(g, s); }
+      else if (mm == 8) { if (w == 2) run_stream_t<float2, 8, 2>(g, s); else run_stream_t<float2, 8, 1>(g, s); }
+      else if (mm == 16) { if (w == 2) run_stream_t<float2, 16, 2>(g, s); else run_stream_t<float2, 16, 1>(g, s); }
+      else run_stream_t<float2, 32, 1>(g, s);
+      break;
+    case F64:
+      if (mm == 4) { if (w == 2) run_stream_t<double, 4, 2>(g, s); else run_stream_t<double, 4, 1>(g, s); }
+      else if (mm == 8) { if (w == 2) run_stream_t<double, 8, 2>(g, s); else run_stream_t<double, 8, 1>(g, s); }
+      else if (mm == 16) { if (w == 2) run_stream_t<double, 16, 2>(g, s); else run_stream_t<double, 16, 1>(g, s); }
+      else if (mm == 32) run_stream_t<double, 32, 1>(g, s);
+      else run_stream_t<double, 64, 1>(g, s);
+      break;
+    case C128:
+      if (mm == 4) run_stream_t<double2, 4, 1>(g, s);
+      else if (mm == 8) run_stream_t<double2, 8, 1>(g, s);
+      else if (mm == 16) run_stream_t<double2, 16, 1>(g, s);
+      else run_stream_t<double2, 32, 1>(g, s);
+      break;
+    default: throw Error("unsupported dtype code");
+  }
+  return true;
+}
+
 static bool run_skinny(int dt, const GemmArgs& g, hipStream_t s) {
+  if (run_stream(dt, g, s)) return true;
   if (g.m <= 32 && g.n >= 64) {
     g_last_kind = K_GEMM_SMALL;
     if (g.m <= 4) run_skinny_t<true, 4>(dt, g, s);
@@ -1072,7 +1224,7 @@ static int auto_kernel(int dt, int prec, const GemmArgs& g) {
     return split_prec && (g.m + 127) / 128 >= 3 ? GEMM_PRESPLIT : GEMM_MFMA;
   }
   if (splitk_worth(g)) return GEMM_SPLITK;
-  if ((g.m <= 32 && g.n >= 64) || (g.n <= 32 && g.m >= 64)) return GEMM_SKINNY;
+  if ((g.m <= (dtype_complex(dt) ? 32 : 64) && g.n >= 64) || (g.n <= 32 && g.m >= 64)) return GEMM_SKINNY;
   return GEMM_SIMPLE;
 }
 
@@ -1169,11 +1321,27 @@ int launch_gemm(int dt, int prec, int kernel, const GemmArgs& g, hipStream_t s) 
 #undef SPLIT_LAUNCH
     return GEMM_MFMA;
   }
+  // row tile: 32 or 64 rows for skinny operands, else 128 (FP32) / 64 (FP64)
+  const int bm = mfma_rows_for(dt, kernel, g.m);
   switch (dt) {
-    case F32: run_mfma<float, false, 128, 128, 16>(dt, g, s); break;
-    case C64: run_mfma<float, true, 128, 128, 16>(dt, g, s); break;
-    case F64: run_mfma<double, false, 64, 64, 16>(dt, g, s); break;
-    case C128: run_mfma<double, true, 64, 64, 16>(dt, g, s); break;
+    case F32:
+      if (bm == 32) run_mfma<float, false, 32, 128, 16>(dt, g, s);
+      else if (bm == 64) run_mfma<float, false, 64, 128, 16>(dt, g, s);
+      else run_mfma<float, false, 128, 128, 16>(dt, g, s);
+      break;
+    case C64:
+      if (bm == 32) run_mfma<float, true, 32, 128, 16>(dt, g, s);
+      else if (bm == 64) run_mfma<float, true, 64, 128, 16>(dt, g, s);
+      else run_mfma<float, true, 128, 128, 16>(dt, g, s);
+      break;
+    case F64:
+      if (bm == 32) run_mfma<double, false, 32, 64, 16>(dt, g, s);
+      else run_mfma<double, false, 64, 64, 16>(dt, g, s);
+      break;
+    case C128:
+      if (bm == 32) run_mfma<double, true, 32, 64, 16>(dt, g, s);
+      else run_mfma<double, true, 64, 64, 16>(dt, g, s);
+      break;
     default: throw Error("unsupported dtype code");
   }
   return GEMM_MFMA;
@@ -1183,7 +1351,8 @@ size_t gemm_scratch_bytes(int dt, int prec, int kernel, const GemmArgs& g) {
   if (kernel == GEMM_AUTO) kernel = auto_kernel(dt, prec, g);
   KSplit ks;
   if (kernel == GEMM_SPLITK) ks = plan_splitk(dt, g, MAX_PARTIAL_BYTES);
-  else if (kernel == GEMM_MFMA) ks = plan_mfma_split(dt, prec, g, MAX_PARTIAL_BYTES);
+  else if (kernel == GEMM_MFMA || kernel >= GEMM_MFMA_R32)
+    ks = plan_mfma_split(dt, prec, g, MAX_PARTIAL_BYTES, mfma_rows_for(dt, kernel, g.m));
   size_t part = ks.n_split > 1 ? (size_t)ks.n_split * dtype_size(dt) * (size_t)(g.nbo * g.nbi * g.m * g.n) : 0;
   if (kernel == GEMM_PRESPLIT && (prec == BF16X3 || prec == FP16X2) && (dt == F32 || dt == C64))
     part = presplit_sizes(dt, g).total;

@@ -113,7 +113,12 @@ struct GemmArgs {
 // GEMM_PRESPLIT: split-precision MFMA GEMM whose operands are split into
 // 16-bit (hi, lo) arrays by a separate pass (GEMM_MFMA splits them while staging
 // each tile); for native precision it is GEMM_MFMA.
-enum GemmKernel { GEMM_AUTO = -1, GEMM_SIMPLE = 0, GEMM_MFMA = 1, GEMM_SKINNY = 2, GEMM_SPLITK = 3, GEMM_PRESPLIT = 4 };
+// GEMM_MFMA_R32 / R64 / R128: the native MFMA kernel with that row tile
+// (GEMM_MFMA picks the row tile from m; FP64 tiles have at most 64 rows).
+enum GemmKernel {
+  GEMM_AUTO = -1, GEMM_SIMPLE = 0, GEMM_MFMA = 1, GEMM_SKINNY = 2, GEMM_SPLITK = 3, GEMM_PRESPLIT = 4,
+  GEMM_MFMA_R32 = 5, GEMM_MFMA_R64 = 6, GEMM_MFMA_R128 = 7
+};
 // Returns the kernel used.
 int launch_gemm(int dt, int prec, int kernel, const GemmArgs& g, hipStream_t s);
 // Device scratch launch_gemm can use for this GEMM (FP16X2 scales, split-K partials).

@@ -47,6 +47,7 @@ def main():
     ap.add_argument("--precision", default="auto",
                     help="native | bf16x3 | fp16x2 | auto (fastest variant with rel. error <= 1e-5)")
     ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true", help="skip the per-step GEMM kernel and tile search")
     ap.add_argument("--skip-plain", action="store_true")
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-profile", action="store_true")
@@ -90,7 +91,7 @@ def main():
     x64 = [torch.from_numpy(a).to(dev) for a in arrays]
     x128 = [t.to(torch.complex128) for t in x64]
     stream = torch.cuda.current_stream(dev)
-    form = dict(batched=True, use_graph=not args.no_graph, precision=args.precision)
+    form = dict(batched=True, use_graph=not args.no_graph, precision=args.precision, autotune=not args.no_autotune)
     flops = 8.0 * info["flops_per_slice"] * info["n_slices"]
 
     log(f"[bench] plan: {info['n_steps']} steps, {info['n_slices']} slices, "
@@ -189,7 +190,8 @@ def main():
         "warmup": args.warmup, "ms_per_step": round(ms, 3), "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "complex64", "data": "synthetic (seeded random circuit, random-init gates)",
         "config": {"workload": args.workload, "qubits": 30, "grid": "5x6", "cycles": 14,
-                   "n_slices": info["n_slices"], "target_size": 2 ** 24, "form": "batched" + ("" if args.no_graph else "+hipgraph"),
+                   "n_slices": info["n_slices"], "target_size": 2 ** 24,
+                   "form": "batched" + ("" if args.no_graph else "+hipgraph") + ("" if args.no_autotune else "+autotune"),
                    "precision": form["precision"], "flops_per_contraction": flops},
         "precision_trials": trial,
         "slices_per_s": round(info["n_slices"] / (ms * 1e-3), 2),

@@ -219,3 +219,16 @@ def test_slice_ranges_partial_sums(cuda):
     parts = sum(fplan.run(arrs, 1, batched=True, slice_range=r)[0].cpu().numpy()
                 for r in [(0, 100), (100, nf // 2 + 7), (nf // 2 + 7, nf)])
     np.testing.assert_allclose(parts, whole.cpu().numpy(), rtol=1e-12, atol=1e-12)
+
+
+@pytest.mark.parametrize("precision", ["native", "bf16x3", "fp16x2"])
+def test_autotuned_batched_graph_forms(cuda, precision):
+    """The autotuner's per-step kernel / tile choices keep the result within 1e-5."""
+    net, arrays, _ = W.random_circuit_amplitude(4, 4, 8, 0)
+    path = P.greedy(net)
+    sliced = P.find_slices_to_size(net, path, 2 ** 7)
+    want = complex(O.contract(net, [a.astype(np.complex128) for a in arrays], path, "compiled"))
+    for graph in (False, True):
+        got = contract_sliced(net["inputs"], net["output"], net["size_dict"], arrays, sliced, ssa_path=path,
+                              batched=True, use_graph=graph, autotune=True, precision=precision)
+        assert abs(complex(got) - want) / abs(want) < 1e-5, graph

@@ -91,7 +91,7 @@ PREC_CASES = [(np.float32, 0, 1e-6), (np.float64, 0, 1e-14), (np.complex64, 0, 1
 
 
 @pytest.mark.parametrize("case", range(len(PREC_CASES)))
-@pytest.mark.parametrize("kernel", [0, 1, 2, 3, 4])
+@pytest.mark.parametrize("kernel", [0, 1, 2, 3, 4, 5, 6, 7])
 def test_gemm_against_numpy(cuda, case, kernel):
     """Simple (0), MFMA (1, native or split precision), skinny (2), split-K (3)
     and pre-split MFMA (4) GEMMs vs complex128 NumPy."""
@@ -101,6 +101,8 @@ def test_gemm_against_numpy(cuda, case, kernel):
     dtype, prec, eps = PREC_CASES[case]
     if prec and kernel not in (1, 4):
         pytest.skip("split precision applies to the MFMA kernels")
+    if kernel == 7 and dtype in (np.float64, np.complex128):
+        pytest.skip("FP64 MFMA tiles have at most 64 rows (7 runs as 6)")
     code = {np.float32: 0, np.float64: 1, np.complex64: 2, np.complex128: 3}[dtype]
     rng = np.random.default_rng(5)
     for bt, m, k, n in GEMM_SHAPES:
