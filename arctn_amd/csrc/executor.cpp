@@ -580,13 +580,22 @@ void Executor::autotune(const Ctx& c, hipStream_t s) {
     q.scratch = c.ws + offs_[scratch_buf_];
     q.scratch_bytes = bufs_[scratch_buf_].bytes;
     float best_ms = 1e30f;
-    int choice = GEMM_SIMPLE;
+    int choice = GEMM_AUTO;
     const bool split_prec = o_.precision != NATIVE;
-    for (int cand : {GEMM_SIMPLE, GEMM_SKINNY, GEMM_SPLITK, GEMM_PRESPLIT, GEMM_MFMA_R32, GEMM_MFMA_R64, GEMM_MFMA_R128}) {
-      if (cand == GEMM_SPLITK && q.k < 256) continue;
-      if (cand == GEMM_PRESPLIT && !split_prec) continue;
-      // split precision has one in-kernel-split MFMA tile shape
-      if (split_prec && (cand == GEMM_MFMA_R32 || cand == GEMM_MFMA_R64)) continue;
+    const double macs = (double)q.nbo * (double)q.nbi * (double)q.m * (double)q.n * (double)q.k;
+    const double outs = (double)q.nbo * (double)q.nbi * (double)q.m * (double)q.n;
+    for (int cand : {GEMM_SIMPLE, GEMM_SKINNY, GEMM_SPLITK, GEMM_PRESPLIT, GEMM_MFMA, GEMM_MFMA_R32, GEMM_MFMA_R64,
+                     GEMM_MFMA_R128}) {
+      // candidates that cannot win are not timed (the one-thread-per-output kernel
+      // on large products, split-K with many outputs, row tiles far from m)
+      if (cand == GEMM_SIMPLE && macs > (double)(1 << 26)) continue;
+      if (cand == GEMM_SKINNY && q.m > 64 && q.n > 32) continue;
+      if (cand == GEMM_SPLITK && (q.k < 256 || outs > (double)(1 << 16))) continue;
+      // split products (GEMM_MFMA splits while staging) only where they can pay off
+      if ((cand == GEMM_PRESPLIT || cand == GEMM_MFMA) && (!split_prec || q.m < 64)) continue;
+      if (cand == GEMM_MFMA_R32 && q.m > 64) continue;
+      if (cand == GEMM_MFMA_R64 && q.m > 256) continue;
+      if (cand == GEMM_MFMA_R128 && q.m < 64) continue;
       if ((o_.dtype == F64 || o_.dtype == C128) && cand == GEMM_MFMA_R128) continue;
       launch_gemm(o_.dtype, o_.precision, cand, q, s);
       ATN_HIP(hipEventRecord(e0, s));

@@ -759,6 +759,10 @@ __global__ __launch_bounds__(256, 2) void gemm_split_kernel(GemmArgs g) {
 }
 
 constexpr size_t SPLIT_LDS = 4 * 128 * (64 + 8) * 2;
+// Measured on gfx950 (complex64 1024x512x8192 x64, BF16x3): BK 64 / one buffer
+// 302 TFLOP/s algorithmic, BK 32 / two buffers 277.
+constexpr int PRESPLIT_BK = 64, PRESPLIT_NBUF = 1;
+constexpr size_t PRESPLIT_LDS = PRESPLIT_NBUF * 4 * 128 * PRESPLIT_BK * 2;
 
 // ------------------------------------------------------------------ dispatch --
 // ------------------------------------------------ pre-split operand GEMM --
@@ -867,20 +871,34 @@ __device__ inline uint4 neg_im(uint4 v) { return make_uint4(v.x ^ 0x80000000u, v
 __device__ inline uint32_t swap16(uint32_t x) { return (x >> 16) | (x << 16); }
 __device__ inline uint4 swap_pairs(uint4 v) { return make_uint4(swap16(v.x), swap16(v.y), swap16(v.z), swap16(v.w)); }
 
-template <bool CPLX, bool F16>
+// LDS tiles are 128 rows of BK halves (no padding), NBUF buffers. The
+// 16-byte chunk c of row r sits at chunk c ^ f(r) (f(r) = (r >> 1) & 7 for 8
+// chunks per row, (r >> 2) & 3 for 4): the 16 rows of one MFMA fragment read
+// and the rows of one 16-lane store each cover all 64 banks once. With two
+// buffers the next tile is stored into the other buffer while the waves still
+// compute on this one (one barrier per k tile); with one buffer (half the LDS,
+// two workgroups per CU) the store waits for a barrier.
+// Complex B: LDS row r (< 64) holds (re, -im) of compact row r and produces
+// output column 2r (real part); row 64 + r holds (im, re) and produces 2r + 1.
+template <int BK>
+__device__ inline int swz(int row, int ch) {
+  constexpr int CH = BK / 8;  // 16-byte chunks per row
+  const int f = CH == 8 ? ((row >> 1) & 7) : ((row >> 2) & 3);
+  return row * BK + (((ch >> 3) ^ f) << 3);
+}
+
+template <bool CPLX, bool F16, int BK, int NBUF>
 __global__ __launch_bounds__(256, 2) void gemm_presplit_kernel(GemmArgs g, PreSplit ps) {
   using H = typename Half<F16>::H;
   using V = typename Half<F16>::V;
-  constexpr int BM = 128, BN = 128, BK = 64;
+  static_assert(BK == 32 || BK == 64, "k tile");
+  constexpr int BM = 128, BN = 128, CH = BK / 8;
   constexpr int WM = 64, WN = 64, TM = 4, TN = 4;
-  constexpr int SK = BK + 8;  // row stride in halves (144 B): conflict-free 16-byte rows
-  constexpr int A_Q = BM * (BK / 8) / 256;                       // 16-byte A chunks per thread (4)
-  constexpr int B_Q = (CPLX ? BN / 2 : BN) * (BK / 8) / 256;     // 16-byte B chunks per thread (2 / 4)
+  constexpr int A_Q = BM * CH / 256;                    // 16-byte A chunks per thread
+  constexpr int B_Q = (CPLX ? BN / 2 : BN) * CH / 256;  // 16-byte B chunks per thread
+  constexpr int TILE = 128 * BK;                        // halves per LDS array
   extern __shared__ __align__(16) char smem_split[];
-  H* ahi = reinterpret_cast<H*>(smem_split);
-  H* alo = ahi + BM * SK;
-  H* bhi = alo + BM * SK;
-  H* blo = bhi + BN * SK;
+  H* lds = reinterpret_cast<H*>(smem_split);  // [NBUF buffers][ahi, alo, bhi, blo][TILE]
 
   float unscale = 1.f;
   if constexpr (F16) {
@@ -915,7 +933,7 @@ __global__ __launch_bounds__(256, 2) void gemm_presplit_kernel(GemmArgs g, PreSp
       auto load = [&](int64_t k0) {
 #pragma unroll
         for (int u = 0; u < A_Q; ++u) {
-          const int q = tid + 256 * u, row = q >> 3, ch = (q & 7) * 8;
+          const int q = tid + 256 * u, row = q / CH, ch = (q % CH) * 8;
           const int64_t gr = m0 + row, gc = k0 + ch;
           const bool ok = gr < M && gc < kp;
           rah[u] = ok ? *reinterpret_cast<const uint4*>(Ah + gr * kp + gc) : make_uint4(0, 0, 0, 0);
@@ -923,54 +941,63 @@ __global__ __launch_bounds__(256, 2) void gemm_presplit_kernel(GemmArgs g, PreSp
         }
 #pragma unroll
         for (int u = 0; u < B_Q; ++u) {
-          const int q = tid + 256 * u, row = q >> 3, ch = (q & 7) * 8;
+          const int q = tid + 256 * u, row = q / CH, ch = (q % CH) * 8;
           const int64_t gr = (CPLX ? n0 / 2 : n0) + row, gc = k0 + ch;
           const bool ok = gr < g.n && gc < kp;
           rbh[u] = ok ? *reinterpret_cast<const uint4*>(Bh + gr * kp + gc) : make_uint4(0, 0, 0, 0);
           rbl[u] = ok ? *reinterpret_cast<const uint4*>(Bl + gr * kp + gc) : make_uint4(0, 0, 0, 0);
         }
       };
-      auto store = [&]() {
+      auto store = [&](int buf) {
+        H* ahi = lds + buf * 4 * TILE;
+        H* alo = ahi + TILE;
+        H* bhi = alo + TILE;
+        H* blo = bhi + TILE;
 #pragma unroll
         for (int u = 0; u < A_Q; ++u) {
-          const int q = tid + 256 * u, row = q >> 3, ch = (q & 7) * 8;
-          *reinterpret_cast<uint4*>(ahi + row * SK + ch) = rah[u];
-          *reinterpret_cast<uint4*>(alo + row * SK + ch) = ral[u];
+          const int q = tid + 256 * u, row = q / CH, ch = (q % CH) * 8;
+          *reinterpret_cast<uint4*>(ahi + swz<BK>(row, ch)) = rah[u];
+          *reinterpret_cast<uint4*>(alo + swz<BK>(row, ch)) = ral[u];
         }
 #pragma unroll
         for (int u = 0; u < B_Q; ++u) {
-          const int q = tid + 256 * u, row = q >> 3, ch = (q & 7) * 8;
+          const int q = tid + 256 * u, row = q / CH, ch = (q % CH) * 8;
           if constexpr (CPLX) {
-            *reinterpret_cast<uint4*>(bhi + (2 * row) * SK + ch) = neg_im(rbh[u]);
-            *reinterpret_cast<uint4*>(blo + (2 * row) * SK + ch) = neg_im(rbl[u]);
-            *reinterpret_cast<uint4*>(bhi + (2 * row + 1) * SK + ch) = swap_pairs(rbh[u]);
-            *reinterpret_cast<uint4*>(blo + (2 * row + 1) * SK + ch) = swap_pairs(rbl[u]);
+            *reinterpret_cast<uint4*>(bhi + swz<BK>(row, ch)) = neg_im(rbh[u]);
+            *reinterpret_cast<uint4*>(blo + swz<BK>(row, ch)) = neg_im(rbl[u]);
+            *reinterpret_cast<uint4*>(bhi + swz<BK>(row + 64, ch)) = swap_pairs(rbh[u]);
+            *reinterpret_cast<uint4*>(blo + swz<BK>(row + 64, ch)) = swap_pairs(rbl[u]);
           } else {
-            *reinterpret_cast<uint4*>(bhi + row * SK + ch) = rbh[u];
-            *reinterpret_cast<uint4*>(blo + row * SK + ch) = rbl[u];
+            *reinterpret_cast<uint4*>(bhi + swz<BK>(row, ch)) = rbh[u];
+            *reinterpret_cast<uint4*>(blo + swz<BK>(row, ch)) = rbl[u];
           }
         }
       };
       const int64_t nk = (kp + BK - 1) / BK;
-      load(0);
-      store();
-      __syncthreads();
       const int cl = lane & 15;
+      load(0);
+      store(0);
+      __syncthreads();
       for (int64_t kt = 0; kt < nk; ++kt) {
+        const int buf = NBUF == 2 ? (int)(kt & 1) : 0;
         if (kt + 1 < nk) load((kt + 1) * BK);
+        const H* ahi = lds + buf * 4 * TILE;
+        const H* alo = ahi + TILE;
+        const H* bhi = alo + TILE;
+        const H* blo = bhi + TILE;
 #pragma unroll
         for (int ks = 0; ks < BK; ks += 32) {
           const int kg = ks + (lane >> 4) * 8;
           V ah[TM], al[TM];
 #pragma unroll
           for (int i = 0; i < TM; ++i) {
-            ah[i] = *reinterpret_cast<const V*>(&ahi[(wm0 + i * 16 + cl) * SK + kg]);
-            al[i] = *reinterpret_cast<const V*>(&alo[(wm0 + i * 16 + cl) * SK + kg]);
+            ah[i] = *reinterpret_cast<const V*>(&ahi[swz<BK>(wm0 + i * 16 + cl, kg)]);
+            al[i] = *reinterpret_cast<const V*>(&alo[swz<BK>(wm0 + i * 16 + cl, kg)]);
           }
 #pragma unroll
           for (int j = 0; j < TN; ++j) {
-            V bh = *reinterpret_cast<const V*>(&bhi[(wn0 + j * 16 + cl) * SK + kg]);
-            V bl = *reinterpret_cast<const V*>(&blo[(wn0 + j * 16 + cl) * SK + kg]);
+            V bh = *reinterpret_cast<const V*>(&bhi[swz<BK>(wn0 + j * 16 + cl, kg)]);
+            V bl = *reinterpret_cast<const V*>(&blo[swz<BK>(wn0 + j * 16 + cl, kg)]);
 #pragma unroll
             for (int i = 0; i < TM; ++i) {
               acc[i][j] = Half<F16>::mfma(al[i], bh, acc[i][j]);
@@ -979,10 +1006,15 @@ __global__ __launch_bounds__(256, 2) void gemm_presplit_kernel(GemmArgs g, PreSp
             }
           }
         }
-        __syncthreads();
-        if (kt + 1 < nk) {
-          store();
+        if constexpr (NBUF == 2) {
+          if (kt + 1 < nk) store(buf ^ 1);
           __syncthreads();
+        } else {
+          __syncthreads();
+          if (kt + 1 < nk) {
+            store(0);
+            __syncthreads();
+          }
         }
       }
 #pragma unroll
@@ -991,7 +1023,9 @@ __global__ __launch_bounds__(256, 2) void gemm_presplit_kernel(GemmArgs g, PreSp
         for (int j = 0; j < TN; ++j)
 #pragma unroll
           for (int r = 0; r < 4; ++r) {
-            int64_t row = m0 + wm0 + i * 16 + 4 * (lane >> 4) + r, col = n0 + wn0 + j * 16 + cl;
+            const int lrow = wn0 + j * 16 + cl;  // LDS row of B' = tile column
+            const int64_t row = m0 + wm0 + i * 16 + 4 * (lane >> 4) + r;
+            const int64_t col = CPLX ? n0 + 2 * (lrow & 63) + (lrow >> 6) : n0 + lrow;
             if (row < M && col < N) C[row * ldc + col] = acc[i][j][r] * unscale;
           }
     }
@@ -1215,14 +1249,28 @@ static PreSplitSizes presplit_sizes(int dt, const GemmArgs& g) {
   return r;
 }
 
+// Split-precision products pay off on MFMA-bound shapes only; memory-bound
+// skinny steps run the FP32 kernels (same time, full FP32 accuracy).
+static bool split_worth(int dt, const GemmArgs& g) {
+  const bool cplx = dtype_complex(dt);
+  const int64_t K = cplx ? 2 * g.k : g.k, N = cplx ? 2 * g.n : g.n;
+  return g.m >= 64 && N >= 128 && K >= 64;
+}
+
+static int native_mfma_kernel(int dt, int64_t m) {
+  const int bm = mfma_rows(dt, m);
+  return bm == 32 ? GEMM_MFMA_R32 : bm == 64 ? GEMM_MFMA_R64 : GEMM_MFMA_R128;
+}
+
 static int auto_kernel(int dt, int prec, const GemmArgs& g) {
-  if (mfma_worth(dt, g)) {
+  const bool split_prec = (prec == BF16X3 || prec == FP16X2) && (dt == F32 || dt == C64);
+  if (split_prec && split_worth(dt, g)) {
     // Splitting inside the kernel repeats B's conversion once per 128-row tile
     // of A (and A's once per column tile and batch entry): with three or more
     // row tiles one separate splitting pass is cheaper.
-    const bool split_prec = (prec == BF16X3 || prec == FP16X2) && (dt == F32 || dt == C64);
-    return split_prec && (g.m + 127) / 128 >= 3 ? GEMM_PRESPLIT : GEMM_MFMA;
+    return (g.m + 127) / 128 >= 3 ? GEMM_PRESPLIT : GEMM_MFMA;
   }
+  if (mfma_worth(dt, g)) return native_mfma_kernel(dt, g.m);
   if (splitk_worth(g)) return GEMM_SPLITK;
   if ((g.m <= (dtype_complex(dt) ? 32 : 64) && g.n >= 64) || (g.n <= 32 && g.m >= 64)) return GEMM_SKINNY;
   return GEMM_SIMPLE;
@@ -1245,7 +1293,9 @@ int launch_gemm(int dt, int prec, int kernel, const GemmArgs& g, hipStream_t s) 
   if (prec == C3M) throw Error("precision c3m is not available in this build");
   const bool split_prec = (prec == BF16X3 || prec == FP16X2) && (dt == F32 || dt == C64);
   if (kernel == GEMM_PRESPLIT && !split_prec) kernel = GEMM_MFMA;
-  if (split_prec) {
+  // split precision runs on GEMM_MFMA (split while staging) and GEMM_PRESPLIT;
+  // the explicit row-tile kernels are the FP32 / FP64 MFMA kernels in any mode
+  if (split_prec && (kernel == GEMM_MFMA || kernel == GEMM_PRESPLIT)) {
     const PreSplitSizes psz = presplit_sizes(dt, g);
     const bool pre = kernel == GEMM_PRESPLIT && g.scratch && g.scratch_bytes >= 256 + psz.total;
     int64_t N = dt == C64 ? 2 * g.n : g.n;
@@ -1276,14 +1326,14 @@ int launch_gemm(int dt, int prec, int kernel, const GemmArgs& g, hipStream_t s) 
   do {                                                                                                     \
     static bool attr_set = false;                                                                          \
     if (!attr_set) {                                                                                       \
-      ATN_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&gemm_presplit_kernel<C_, F_>),            \
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)SPLIT_LDS));            \
+      ATN_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&gemm_presplit_kernel<C_, F_, PRESPLIT_BK, PRESPLIT_NBUF>), \
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)PRESPLIT_LDS));         \
       attr_set = true;                                                                                     \
     }                                                                                                      \
     LAUNCH((split_a_kernel<C_, F_>), ga, dim3(256), 0, s, g, ps);                                         \
     LAUNCH((split_bt_kernel<C_, F_>), gb, dim3(256), 0, s, g, ps);                                        \
     g_last_kind = F_ ? K_GEMM_MFMA_FP16X2 : K_GEMM_MFMA_BF16X3;                                           \
-    LAUNCH((gemm_presplit_kernel<C_, F_>), grid, dim3(256), SPLIT_LDS, s, g, ps);                         \
+    LAUNCH((gemm_presplit_kernel<C_, F_, PRESPLIT_BK, PRESPLIT_NBUF>), grid, dim3(256), PRESPLIT_LDS, s, g, ps);         \
   } while (0)
       if (cx && f16) PRESPLIT_LAUNCH(true, true);
       else if (cx) PRESPLIT_LAUNCH(true, false);

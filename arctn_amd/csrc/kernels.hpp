@@ -113,8 +113,10 @@ struct GemmArgs {
 // GEMM_PRESPLIT: split-precision MFMA GEMM whose operands are split into
 // 16-bit (hi, lo) arrays by a separate pass (GEMM_MFMA splits them while staging
 // each tile); for native precision it is GEMM_MFMA.
-// GEMM_MFMA_R32 / R64 / R128: the native MFMA kernel with that row tile
-// (GEMM_MFMA picks the row tile from m; FP64 tiles have at most 64 rows).
+// GEMM_MFMA_R32 / R64 / R128: the FP32 / FP64 MFMA kernel with that row tile
+// whatever the precision mode (GEMM_MFMA picks the row tile from m, or splits
+// the operands while staging under split precision; FP64 tiles have at most
+// 64 rows).
 enum GemmKernel {
   GEMM_AUTO = -1, GEMM_SIMPLE = 0, GEMM_MFMA = 1, GEMM_SKINNY = 2, GEMM_SPLITK = 3, GEMM_PRESPLIT = 4,
   GEMM_MFMA_R32 = 5, GEMM_MFMA_R64 = 6, GEMM_MFMA_R128 = 7

@@ -51,13 +51,23 @@ def main():
     ap.add_argument("--skip-plain", action="store_true")
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-profile", action="store_true")
+    ap.add_argument("--skip-ref", action="store_true",
+                    help="skip the complex128 reference contraction (rel_err is then null; for profiler runs)")
     ap.add_argument("--cpu-slices", type=int, default=0, help="CPU sample size (0 = three per core)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a device copy of 2 GiB (for scaling HBM traffic counters), then exit")
     ap.add_argument("--subset-slices", type=int, default=16,
                     help="precision workload: slices of output block 0 compared with the CPU oracle")
     ap.add_argument("--plain-loop", action="store_true", help="grouped workload: also time the per-instance loop")
+    ap.add_argument("--traffic-from", nargs=4, metavar=("FETCH_CSV", "WRITE_CSV", "COPY_FETCH_CSV", "COPY_WRITE_CSV"),
+                    help="turn rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE counter files of a bench run and of a "
+                         "--calibrate-copy run into the HBM traffic per launch of the split-precision GEMM kind, "
+                         "written to profiles/")
+    ap.add_argument("--traffic-last", type=int, default=0,
+                    help="with --traffic-from: count only the last N executor dispatches (one contraction)")
     args = ap.parse_args()
+    if args.traffic_from:
+        return traffic_from(*args.traffic_from, last=args.traffic_last)
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the slice range of each GPU "
               "would be one contiguous share of the slices, reduced once at the end.", file=sys.stderr)
@@ -96,26 +106,33 @@ def main():
 
     log(f"[bench] plan: {info['n_steps']} steps, {info['n_slices']} slices, "
         f"{info['n_varying_steps']} slice-dependent steps, {flops:.3e} FLOP per contraction")
-    ref_out, _ = plan.run(x128, 3, batched=True)
-    ref = complex(ref_out.cpu())
-    plan.release()
-    log(f"[bench] complex128 reference amplitude {ref:.9e}")
+    ref = None
+    if not args.skip_ref:
+        ref_out, _ = plan.run(x128, 3, batched=True)
+        ref = complex(ref_out.cpu())
+        plan.release()
+        log(f"[bench] complex128 reference amplitude {ref:.9e}")
 
-    # Precision form: the fastest variant whose error meets the 1e-5 bar.
-    cands = ["native", "bf16x3", "fp16x2"] if args.precision == "auto" else [args.precision]
+    # Form: the fastest (precision, autotune) variant whose error meets the 1e-5 bar.
+    precs = ["native", "bf16x3", "fp16x2"] if args.precision == "auto" else [args.precision]
+    tunes = [False] if args.no_autotune else [False, True]
+    cands = [(p, t) for p in precs for t in tunes]
     trial = {}
-    for prec in cands:
-        f = dict(form, precision=prec)
+    for prec, tune in cands:
+        f = dict(form, precision=prec, autotune=tune)
         for _ in range(2):
             o, rep = plan.run(x64, 2, report=True, **f)
-        err = abs(complex(o.cpu()) - ref) / abs(ref)
-        trial[prec] = {"ms": rep["elapsed_ms"], "rel_err": err}
-        log(f"[bench] candidate {prec}: {rep['elapsed_ms']:.2f} ms, rel err {err:.2e}")
+        err = abs(complex(o.cpu()) - ref) / abs(ref) if ref is not None else None
+        name = prec + ("+autotune" if tune else "")
+        trial[name] = {"ms": rep["elapsed_ms"], "rel_err": err, "launches": rep["n_launches"]}
+        log(f"[bench] candidate {name}: {rep['elapsed_ms']:.2f} ms, {rep['n_launches']} launches, rel err "
+            + (f"{err:.2e}" if err is not None else "n/a"))
         plan.release()
-    ok = [p for p in cands if trial[p]["rel_err"] <= 1e-5] or cands
-    prec = min(ok, key=lambda p: trial[p]["ms"])
-    form["precision"] = prec
-    log(f"[bench] measuring precision form {prec}")
+    key = lambda c: c[0] + ("+autotune" if c[1] else "")
+    ok = [c for c in cands if trial[key(c)]["rel_err"] is not None and trial[key(c)]["rel_err"] <= 1e-5] or cands
+    prec, tune = min(ok, key=lambda c: trial[key(c)]["ms"])
+    form["precision"], form["autotune"] = prec, tune
+    log(f"[bench] measuring form {key((prec, tune))} ({trial[key((prec, tune))]['launches']} launches per contraction)")
 
     out = torch.empty([], dtype=torch.complex64, device=dev)
     for w in range(args.warmup):
@@ -135,8 +152,9 @@ def main():
     value = flops / (ms * 1e-3) / 1e12
     log(f"[bench] {args.steps} steps: {ms:.2f} ms/step (wall {wall_ms:.2f}), {value:.2f} TFLOP/s")
     got = complex(out.cpu())
-    rel_err = abs(got - ref) / abs(ref)
-    log(f"[bench] amplitude {got:.6e} vs complex128 {ref:.6e}: rel err {rel_err:.2e}")
+    rel_err = abs(got - ref) / abs(ref) if ref is not None else None
+    if ref is not None:
+        log(f"[bench] amplitude {got:.6e} vs complex128 {ref:.6e}: rel err {rel_err:.2e}")
 
     roofline = None
     kernels = {}
@@ -148,7 +166,14 @@ def main():
             ach = k["flops"] / (k["ms"] * 1e-3)
             roofline = {"bound": "mfma", "achieved": ach / 1e12, "peak": PEAKS[dom] / 1e12, "unit": "TFLOP/s",
                         "frac": ach / PEAKS[dom], "traffic": None, "kernel": dom,
-                        "avg_launch_ms": k["ms"] / k["launches"], "launches": k["launches"]}
+                        "avg_launch_ms": k["ms"] / k["launches"], "launches": k["launches"],
+                        "algorithmic_bytes_per_launch": k["bytes"] / k["launches"]}
+            tf = os.path.join(ROOT, "profiles", f"traffic_{dom}_{args.workload}.json")
+            if os.path.exists(tf):
+                with open(tf) as f:
+                    t = json.load(f)
+                roofline["traffic"] = t["hbm_bytes_per_launch"]
+                roofline["traffic_source"] = os.path.relpath(tf, ROOT)
         else:
             ach = k["bytes"] / (k["ms"] * 1e-3)
             roofline = {"bound": "hbm", "achieved": ach / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
@@ -156,6 +181,18 @@ def main():
                         "avg_launch_ms": k["ms"] / k["launches"], "launches": k["launches"]}
         log(f"[bench] dominant kernel {dom}: {roofline['achieved']:.1f} {roofline['unit']} "
             f"({100 * roofline['frac']:.1f}% of peak)")
+        # whole-plan bound: each kernel kind at max(FLOPs / its MFMA peak, bytes / HBM peak)
+        t_min = 0.0
+        for name, kk in kernels.items():
+            peak = PEAKS.get(name, 157.3e12)
+            if name in ("gemm_mfma_bf16x3", "gemm_mfma_fp16x2"):
+                peak = PEAKS[name] / 3  # three 16-bit products per algorithmic product
+            t_min += max(kk["flops"] / peak, kk["bytes"] / HBM_PEAK)
+        prof_ms = sum(kk["ms"] for kk in kernels.values())
+        roofline["plan"] = {"t_min_ms": t_min * 1e3, "t_kernels_ms": prof_ms, "frac": t_min * 1e3 / prof_ms,
+                            "model": "sum over kernel kinds of max(FLOPs / MFMA peak, bytes / 8 TB/s)"}
+        log(f"[bench] plan roofline: {t_min * 1e3:.1f} ms bound vs {prof_ms:.1f} ms measured "
+            f"({100 * t_min * 1e3 / prof_ms:.1f}%)")
     plan.release()
 
     plain = {}
@@ -165,7 +202,7 @@ def main():
         p_out, rep = plan.run(x64, 2, batched=False, report=True)
         plain = {"plain_fp32_ms": rep["elapsed_ms"], "plain_fp32_tflops": rep["tflops"],
                  "plain_slices_per_s": rep["slices_per_s"], "plain_launches": rep["n_launches"],
-                 "plain_rel_err": abs(complex(p_out.cpu()) - ref) / abs(ref),
+                 "plain_rel_err": abs(complex(p_out.cpu()) - ref) / abs(ref) if ref is not None else None,
                  "speedup_vs_plain": rep["elapsed_ms"] / ms}
         log(f"[bench] plain per-slice FP32 path: {rep['elapsed_ms']:.1f} ms -> speedup {rep['elapsed_ms'] / ms:.2f}x")
 
@@ -191,7 +228,7 @@ def main():
         "vs_baseline": None, "dtype": "complex64", "data": "synthetic (seeded random circuit, random-init gates)",
         "config": {"workload": args.workload, "qubits": 30, "grid": "5x6", "cycles": 14,
                    "n_slices": info["n_slices"], "target_size": 2 ** 24,
-                   "form": "batched" + ("" if args.no_graph else "+hipgraph") + ("" if args.no_autotune else "+autotune"),
+                   "form": "batched" + ("" if args.no_graph else "+hipgraph") + ("+autotune" if form["autotune"] else ""),
                    "precision": form["precision"], "flops_per_contraction": flops},
         "precision_trials": trial,
         "slices_per_s": round(info["n_slices"] / (ms * 1e-3), 2),
@@ -202,6 +239,55 @@ def main():
     }
     line.update(plain)
     print(json.dumps(line), flush=True)
+
+
+# kernels that make up one launch of a split-precision GEMM kind
+SPLIT_KERNELS = ("gemm_presplit_kernel", "split_a_kernel", "split_bt_kernel", "maxabs_kernel", "gemm_split_kernel")
+
+
+def traffic_from(fetch_csv, write_csv, copy_fetch_csv, copy_write_csv, last=0):
+    """HBM bytes per launch of the split-precision GEMM kind from rocprofv3 PMC files.
+
+    FETCH_SIZE / WRITE_SIZE are read in KiB per dispatch; each is scaled by the
+    ratio of the known bytes of a 2 GiB device copy to what the same counter
+    read for that copy. With ``last`` > 0 only the last ``last`` dispatches of the
+    executor's kernels count (the timed contraction of a ``--steps 1`` run)."""
+    import csv
+
+    def per_kernel(path, counter):
+        rows = [r for r in csv.DictReader(open(path)) if r["Counter_Name"] == counter]
+        if last:
+            ids = sorted({int(r["Dispatch_Id"]) for r in rows if "atn::" in r["Kernel_Name"]})[-last:]
+            keep = set(ids)
+            rows = [r for r in rows if int(r["Dispatch_Id"]) in keep]
+        tot, disp = {}, {}
+        for r in rows:
+            name = r["Kernel_Name"]
+            tot[name] = tot.get(name, 0.0) + float(r["Counter_Value"])
+            disp.setdefault(name, set()).add(r["Dispatch_Id"])
+        return tot, {k: len(v) for k, v in disp.items()}
+
+    copy_bytes = 2.0 * 2 ** 30
+    scale = {}
+    for counter, path in (("FETCH_SIZE", copy_fetch_csv), ("WRITE_SIZE", copy_write_csv)):
+        rows = [r for r in csv.DictReader(open(path)) if r["Counter_Name"] == counter]
+        tot = {}
+        for r in rows:
+            tot[r["Kernel_Name"]] = tot.get(r["Kernel_Name"], 0.0) + float(r["Counter_Value"])
+        kib = max(tot.values())  # the copy is the largest kernel of that run
+        scale[counter] = copy_bytes / (kib * 1024.0)
+    out = {}
+    for counter, path in (("FETCH_SIZE", fetch_csv), ("WRITE_SIZE", write_csv)):
+        tot, disp = per_kernel(path, counter)
+        kib = sum(v for k, v in tot.items() if any(s in k for s in SPLIT_KERNELS))
+        launches = sum(v for k, v in disp.items() if "gemm_presplit_kernel" in k or "gemm_split_kernel" in k)
+        out[counter] = kib * 1024.0 * scale[counter] / max(launches, 1)
+        out[counter + "_launches"] = launches
+    res = {"hbm_bytes_per_launch": out["FETCH_SIZE"] + out["WRITE_SIZE"], "fetch_bytes_per_launch": out["FETCH_SIZE"],
+           "write_bytes_per_launch": out["WRITE_SIZE"], "launches": out["FETCH_SIZE_launches"],
+           "copy_scale": scale, "kernels": list(SPLIT_KERNELS)}
+    print(json.dumps(res), flush=True)
+    return res
 
 
 def bench_grouped(args):
