@@ -50,8 +50,8 @@ class Report(C.Structure):
 
 
 class KernelStat(C.Structure):
-    _fields_ = [("name", C.c_char * 32), ("launches", C.c_size_t), ("ms", C.c_double), ("flops", C.c_double),
-                ("bytes", C.c_double)]
+    _fields_ = [("name", C.c_char * 32), ("calls", C.c_size_t), ("launches", C.c_size_t), ("ms", C.c_double),
+                ("flops", C.c_double), ("bytes", C.c_double)]
 
 
 # name -> (restype, argtypes)
@@ -80,9 +80,10 @@ _SIGS = {
                                 C.c_size_t, _VP, _VP, _VP]),
     "atn_gemm": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_size_t, C.c_size_t, C.c_size_t, C.c_size_t, _VP,
                            C.c_size_t, _VP, C.c_size_t, _VP, C.c_size_t, _VP]),
-    "atn_grouped_execute": (C.c_int, [C.POINTER(_VP), C.c_size_t, C.c_int, C.c_int, C.POINTER(C.POINTER(_VP)),
-                                      C.POINTER(_VP), C.POINTER(C.c_double), _VP, C.POINTER(Report)]),
-    "atn_grouped_create": (C.c_int, [C.POINTER(_VP), C.c_size_t, C.c_int, C.c_int, C.POINTER(_VP)]),
+    "atn_grouped_execute": (C.c_int, [C.POINTER(_VP), C.c_size_t, C.c_int, C.c_int, C.c_int,
+                                      C.POINTER(C.POINTER(_VP)), C.POINTER(_VP), C.POINTER(C.c_double), _VP,
+                                      C.POINTER(Report)]),
+    "atn_grouped_create": (C.c_int, [C.POINTER(_VP), C.c_size_t, C.c_int, C.c_int, C.c_int, C.POINTER(_VP)]),
     "atn_grouped_destroy": (None, [_VP]),
     "atn_grouped_run": (C.c_int, [_VP, C.POINTER(C.POINTER(_VP)), C.POINTER(_VP), C.POINTER(C.c_double), _VP,
                                   C.POINTER(Report)]),

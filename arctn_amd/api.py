@@ -204,8 +204,8 @@ def profile(plan, dev_inputs, dtype_code, out=None, stream=None, **opts):
     stats = (_lib.KernelStat * 32)()
     n = _lib.C.c_size_t(0)
     _lib.check(_lib.lib().atn_executor_profile(ex.h, ptrs, out.data_ptr(), s.cuda_stream, stats, 32, _lib.C.byref(n)))
-    return {stats[i].name.decode(): {"launches": stats[i].launches, "ms": stats[i].ms, "flops": stats[i].flops,
-                                     "bytes": stats[i].bytes} for i in range(n.value)}
+    return {stats[i].name.decode(): {"calls": stats[i].calls, "launches": stats[i].launches, "ms": stats[i].ms,
+                                     "flops": stats[i].flops, "bytes": stats[i].bytes} for i in range(n.value)}
 
 
 def _finish(out, numpy_in):
@@ -397,8 +397,11 @@ class GroupedContraction:
     ``run`` replays them on new inputs.
     """
 
-    def __init__(self, plans, dtype, strip=True):
+    def __init__(self, plans, dtype, strip=True, precision="native"):
+        if precision not in _lib.PREC:
+            raise ValueError(f"unknown precision {precision!r}; use one of {sorted(_lib.PREC)}")
         self.plans = list(plans)
+        self.precision = precision
         if isinstance(dtype, int):
             self.code = dtype
         else:
@@ -408,7 +411,8 @@ class GroupedContraction:
         self.strip = bool(strip)
         self._h = _lib.C.c_void_p()
         arr = (_lib._VP * len(self.plans))(*[p.handle for p in self.plans])
-        _lib.check(_lib.lib().atn_grouped_create(arr, len(self.plans), self.code, int(self.strip), _lib.C.byref(self._h)))
+        _lib.check(_lib.lib().atn_grouped_create(arr, len(self.plans), self.code, _lib.PREC[precision], int(self.strip),
+                                                 _lib.C.byref(self._h)))
 
     def __del__(self):
         if getattr(self, "_h", None) and _lib._lib is not None:
@@ -440,7 +444,7 @@ class GroupedContraction:
         return outs, list(exps)
 
 
-def contract_grouped(plans, arrays, *, strip=True, report=False, stream=None):
+def contract_grouped(plans, arrays, *, strip=True, precision="native", report=False, stream=None):
     """Contract many independent unsliced plans in lockstep grouped launches.
 
     ``plans`` is a list of unsliced ``Plan``; ``arrays[p]`` the input arrays of
@@ -462,7 +466,7 @@ def contract_grouped(plans, arrays, *, strip=True, report=False, stream=None):
         code, numpy_in = c, numpy_in or n_in
         devs.append(d)
     try:
-        g = GroupedContraction(plans, code, strip)
+        g = GroupedContraction(plans, code, strip, precision)
         res = g.run(devs, report=report, stream=stream)
     except ValueError as e:
         raise ValueError(f"分组收缩失败: {e}") from None

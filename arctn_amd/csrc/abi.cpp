@@ -162,10 +162,11 @@ int atn_executor_profile(atn_executor* ex, const void* const* d_inputs, void* d_
     ex->ex->profile(d_inputs, d_out, static_cast<hipStream_t>(stream), ks);
     size_t n = 0;
     for (size_t k = 0; k < ks.size() && n < max_stats; ++k) {
-      if (!ks[k].launches) continue;
+      if (!ks[k].calls) continue;
       atn_kernel_stat& o = stats[n++];
       std::memset(o.name, 0, sizeof o.name);
       std::strncpy(o.name, kernel_kind_name((int)k), sizeof o.name - 1);
+      o.calls = ks[k].calls;
       o.launches = ks[k].launches;
       o.ms = ks[k].ms;
       o.flops = ks[k].flops;
@@ -323,17 +324,19 @@ struct atn_grouped {
   std::unique_ptr<GroupedExecutor> ex;
 };
 
-int atn_grouped_create(const atn_plan* const* plans, size_t n_plans, int dtype, int strip, atn_grouped** out) {
+int atn_grouped_create(const atn_plan* const* plans, size_t n_plans, int dtype, int precision, int strip,
+                       atn_grouped** out) {
   return guard([&] {
     if (!out || (n_plans && !plans)) throw Error("NULL argument");
     *out = nullptr;
     check_dtype(dtype);
+    if (precision < 0 || precision > 3) throw Error("unsupported precision code");
     std::vector<std::shared_ptr<const Plan>> ps;
     for (size_t i = 0; i < n_plans; ++i) {
       if (!plans[i]) throw Error("NULL plan");
       ps.push_back(plans[i]->p);
     }
-    auto ex = std::make_unique<GroupedExecutor>(std::move(ps), dtype, strip != 0);
+    auto ex = std::make_unique<GroupedExecutor>(std::move(ps), dtype, strip != 0, precision);
     *out = new atn_grouped{std::move(ex)};
   });
 }
@@ -357,10 +360,12 @@ int atn_grouped_run(atn_grouped* g, const void* const* const* inputs, void* cons
   });
 }
 
-int atn_grouped_execute(const atn_plan* const* plans, size_t n_plans, int dtype, int strip, const void* const* const* inputs,
-                        void* const* outputs, double* log10_exponents, void* stream, atn_report* report) {
+int atn_grouped_execute(const atn_plan* const* plans, size_t n_plans, int dtype, int precision, int strip,
+                        const void* const* const* inputs, void* const* outputs, double* log10_exponents, void* stream,
+                        atn_report* report) {
   return guard([&] {
     check_dtype(dtype);
+    if (precision < 0 || precision > 3) throw Error("unsupported precision code");
     std::vector<std::shared_ptr<const Plan>> ps;
     std::vector<const void* const*> ins;
     std::vector<void*> outs;
@@ -371,7 +376,8 @@ int atn_grouped_execute(const atn_plan* const* plans, size_t n_plans, int dtype,
       outs.push_back(outputs[i]);
     }
     Report r;
-    grouped_execute(ps, dtype, strip != 0, ins, outs, log10_exponents, static_cast<hipStream_t>(stream), report ? &r : nullptr);
+    grouped_execute(ps, dtype, strip != 0, precision, ins, outs, log10_exponents, static_cast<hipStream_t>(stream),
+                    report ? &r : nullptr);
     to_report(r, report);
   });
 }

@@ -545,6 +545,7 @@ void Executor::profile(const void* const* inputs, void* out, hipStream_t s, std:
       float ms = 0;
       ATN_HIP(hipEventElapsedTime(&ms, r.a, r.b));
       auto& k = stats[r.kind];
+      k.calls += 1;
       k.launches += r.launches;
       k.ms += ms;
       k.flops += r.flops;

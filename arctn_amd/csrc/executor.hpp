@@ -60,7 +60,7 @@ struct Op {
 };
 
 struct KernelStat {
-  size_t launches = 0;
+  size_t calls = 0, launches = 0;
   double ms = 0, flops = 0, bytes = 0;
 };
 
@@ -144,7 +144,9 @@ void stage_finalize_adjoint(const Plan& p, int dt, const void* g, void* gin, hip
 // packed into memory-bounded groups and uploaded once; run() replays them.
 class GroupedExecutor {
  public:
-  GroupedExecutor(std::vector<std::shared_ptr<const Plan>> plans, int dt, bool strip);
+  // prec: NATIVE, BF16X3 or FP16X2 (split products on the MFMA-bound steps
+  // launched on their own; float32 / complex64 only)
+  GroupedExecutor(std::vector<std::shared_ptr<const Plan>> plans, int dt, bool strip, int prec = NATIVE);
   ~GroupedExecutor();
   GroupedExecutor(const GroupedExecutor&) = delete;
   GroupedExecutor& operator=(const GroupedExecutor&) = delete;
@@ -161,12 +163,13 @@ class GroupedExecutor {
   std::vector<std::shared_ptr<const Plan>> plans_;
   int dt_;
   bool strip_;
+  int prec_;
   std::vector<std::unique_ptr<Group>> groups_;
   char* arena_ = nullptr;
   size_t arena_bytes_ = 0;
 };
 
-void grouped_execute(const std::vector<std::shared_ptr<const Plan>>& plans, int dt, bool strip,
+void grouped_execute(const std::vector<std::shared_ptr<const Plan>>& plans, int dt, bool strip, int prec,
                      const std::vector<const void* const*>& inputs, const std::vector<void*>& outputs, double* log10_exp,
                      hipStream_t s, Report* rep);
 

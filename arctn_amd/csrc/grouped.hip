@@ -467,8 +467,10 @@ struct GroupedExecutor::Group {
   std::vector<char*> bases;  // last uploaded base table
 };
 
-GroupedExecutor::GroupedExecutor(std::vector<std::shared_ptr<const Plan>> plans, int dt, bool strip)
-    : plans_(std::move(plans)), dt_(dt), strip_(strip) {
+GroupedExecutor::GroupedExecutor(std::vector<std::shared_ptr<const Plan>> plans, int dt, bool strip, int prec)
+    : plans_(std::move(plans)), dt_(dt), strip_(strip), prec_(prec) {
+  if (prec == C3M) throw Error("precision c3m is not available in this build");
+  if (prec != NATIVE && !(dt == F32 || dt == C64)) throw Error("split precision applies to float32 / complex64 only");
   const size_t np = plans_.size();
   size_t max_steps = 0;
   for (auto& p : plans_) {
@@ -556,7 +558,7 @@ GroupedExecutor::GroupedExecutor(std::vector<std::shared_ptr<const Plan>> plans,
         }
         if (L.n_gemm) L.gemm_off = append(gemms.data(), gemms.size() * sizeof(GGemmDesc));
         if (L.n_strip) L.strip_off = append(strips.data(), strips.size() * sizeof(GStripDesc));
-        for (auto& sg : L.mid) G.gemm_scratch = std::max(G.gemm_scratch, gemm_scratch_bytes(dt, NATIVE, GEMM_AUTO, sg.gemm));
+        for (auto& sg : L.mid) G.gemm_scratch = std::max(G.gemm_scratch, gemm_scratch_bytes(dt, prec, GEMM_AUTO, sg.gemm));
       }
       G.slots_off = append(nullptr, 2 * ng * 8);
       G.expo_off = G.slots_off + ng * 8;
@@ -635,7 +637,7 @@ void GroupedExecutor::run(const std::vector<const void* const*>& inputs, const s
             g.a = at(sg.in), g.b = at(sg.in2), g.c = at(sg.out);
             g.scratch = dev + G.gemm_scratch_off;
             g.scratch_bytes = G.gemm_scratch;
-            launch_gemm(dt, NATIVE, GEMM_AUTO, g, s);
+            launch_gemm(dt, prec_, GEMM_AUTO, g, s);
             break;
           }
           case Single::SUM: launch_sum_axis(dt, sg.A, sg.d, sg.C, at(sg.in), at(sg.out), s); break;
@@ -718,10 +720,10 @@ void GroupedExecutor::run(const std::vector<const void* const*>& inputs, const s
     for (size_t p = 0; p < np; ++p) log10_exp[p] = strip_ ? (double)expo_host[p] * std::log10(2.0) : 0.0;
 }
 
-void grouped_execute(const std::vector<std::shared_ptr<const Plan>>& plans, int dt, bool strip,
+void grouped_execute(const std::vector<std::shared_ptr<const Plan>>& plans, int dt, bool strip, int prec,
                      const std::vector<const void* const*>& inputs, const std::vector<void*>& outputs, double* log10_exp,
                      hipStream_t s, Report* rep) {
-  GroupedExecutor ex(plans, dt, strip);
+  GroupedExecutor ex(plans, dt, strip, prec);
   ex.run(inputs, outputs, log10_exp, s, rep);
   ATN_HIP(hipStreamSynchronize(s));  // the arena is freed when ex goes out of scope
 }

@@ -166,8 +166,8 @@ def main():
             ach = k["flops"] / (k["ms"] * 1e-3)
             roofline = {"bound": "mfma", "achieved": ach / 1e12, "peak": PEAKS[dom] / 1e12, "unit": "TFLOP/s",
                         "frac": ach / PEAKS[dom], "traffic": None, "kernel": dom,
-                        "avg_launch_ms": k["ms"] / k["launches"], "launches": k["launches"],
-                        "algorithmic_bytes_per_launch": k["bytes"] / k["launches"]}
+                        "avg_launch_ms": k["ms"] / k["calls"], "launches": k["calls"],
+                        "kernel_launches": k["launches"], "algorithmic_bytes_per_launch": k["bytes"] / k["calls"]}
             tf = os.path.join(ROOT, "profiles", f"traffic_{dom}_{args.workload}.json")
             if os.path.exists(tf):
                 with open(tf) as f:
@@ -178,7 +178,8 @@ def main():
             ach = k["bytes"] / (k["ms"] * 1e-3)
             roofline = {"bound": "hbm", "achieved": ach / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                         "frac": ach / HBM_PEAK, "traffic": None, "kernel": dom,
-                        "avg_launch_ms": k["ms"] / k["launches"], "launches": k["launches"]}
+                        "avg_launch_ms": k["ms"] / k["calls"], "launches": k["calls"],
+                        "kernel_launches": k["launches"]}
         log(f"[bench] dominant kernel {dom}: {roofline['achieved']:.1f} {roofline['unit']} "
             f"({100 * roofline['frac']:.1f}% of peak)")
         # whole-plan bound: each kernel kind at max(FLOPs / its MFMA peak, bytes / HBM peak)
@@ -292,7 +293,9 @@ def traffic_from(fetch_csv, write_csv, copy_fetch_csv, copy_write_csv, last=0):
 
 def bench_grouped(args):
     """512 uneven random 3-regular networks (float32, scalar output) in lockstep
-    grouped launches; a step is one contraction of every instance."""
+    grouped launches; a step is one contraction of every instance. Each precision
+    variant is timed and checked against the float64 oracle; the fastest variant
+    within the accuracy bar is reported."""
     import numpy as np
     import torch
 
@@ -305,52 +308,66 @@ def bench_grouped(args):
     plans = [Plan(n["inputs"], n["output"], n["size_dict"], p) for n, _, p in insts]
     arrs = [[torch.from_numpy(x).to(dev) for x in a] for _, a, _ in insts]
     flops = sum(2.0 * p.info()["flops_per_slice"] for p in plans)
-    t0 = time.perf_counter()
-    G = GroupedContraction(plans, np.float32, strip=True)
-    log(f"[bench] {len(plans)} instances, {flops:.3e} FLOP per step; grouped executor built in "
-        f"{time.perf_counter() - t0:.2f} s")
-    for w in range(args.warmup):
-        G.run(arrs)
-        torch.cuda.synchronize()
-        log(f"[bench] warmup {w + 1}/{args.warmup} done")
-    stream = torch.cuda.current_stream(dev)
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    torch.cuda.synchronize()
-    e0.record(stream)
-    for _ in range(args.steps):
-        outs, exps = G.run(arrs)
-    e1.record(stream)
-    torch.cuda.synchronize()
-    ms = e0.elapsed_time(e1) / args.steps
-    _, _, rep = G.run(arrs, report=True)
-    log(f"[bench] {args.steps} steps: {ms:.1f} ms/step, {flops / ms / 1e9:.2f} TFLOP/s, {rep['n_launches']} launches, "
-        f"{rep['batch']} groups")
-    vals = [(float(o.cpu()), e) for o, e in zip(outs, exps)]
-    # accuracy against the float64 oracle on every instance the CPU finishes quickly
+    # float64 oracle (and the same plans in float32 on the CPU) on the instances the CPU finishes quickly
     cheap = [i for i, c in enumerate(doc["log10_mults"]) if c <= 8.0]
-    errs, errs32 = [], []
     t0 = time.perf_counter()
+    refs = {}
     for i in cheap:
         net, a, path = insts[i]
         m64, e64 = O.contract(net, [x.astype(np.float64) for x in a], path, mode="stripped")
         m32, e32 = O.contract(net, a, path, mode="stripped")
-        ref = float(m64)
-        m, e = vals[i]
-        errs.append(abs(m * 10 ** (e - e64) - ref) / abs(ref))
-        errs32.append(abs(float(m32) * 10 ** (e32 - e64) - ref) / abs(ref))
-    cpu_s = time.perf_counter() - t0
-    log(f"[bench] float64 oracle on {len(cheap)} instances: max rel err {max(errs):.2e} "
-        f"(CPU float32 oracle on the same plans: {max(errs32):.2e})")
+        refs[i] = (float(m64), e64, abs(float(m32) * 10 ** (e32 - e64) - float(m64)) / abs(float(m64)))
+    oracle_s = time.perf_counter() - t0
+    precs = ["native", "bf16x3", "fp16x2"] if args.precision == "auto" else [args.precision]
+    variants, best = {}, None
+    stream = torch.cuda.current_stream(dev)
+    for prec in precs:
+        t0 = time.perf_counter()
+        G = GroupedContraction(plans, np.float32, strip=True, precision=prec)
+        build_s = time.perf_counter() - t0
+        for w in range(args.warmup):
+            G.run(arrs)
+            torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        e0.record(stream)
+        for _ in range(args.steps):
+            outs, exps = G.run(arrs)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / args.steps
+        _, _, rep = G.run(arrs, report=True)
+        vals = [(float(o.cpu()), e) for o, e in zip(outs, exps)]
+        errs = []
+        for i, (ref, e64, err32) in refs.items():
+            m, e = vals[i]
+            errs.append(abs(m * 10 ** (e - e64) - ref) / abs(ref))
+        # the scalar of some instances comes out of cancellation, where float32
+        # itself is off by ~1e-4: the bar is the median at 1e-5 and the worst case
+        # within twice the worst error of the same plans in float32 on the CPU
+        ok = bool(np.median(errs) <= 1e-5 and max(errs) <= 2 * max(v[2] for v in refs.values()))
+        variants[prec] = {"ms": ms, "tflops": flops / ms / 1e9, "n_launches": rep["n_launches"], "groups": rep["batch"],
+                          "workspace_bytes": rep["workspace_bytes"], "build_s": build_s,
+                          "max_rel_err_vs_f64": max(errs), "median_rel_err_vs_f64": float(np.median(errs)),
+                          "within_bar": ok}
+        log(f"[bench] {prec}: {ms:.1f} ms/step, {flops / ms / 1e9:.2f} TFLOP/s, {rep['n_launches']} launches, "
+            f"max rel err {max(errs):.2e} (median {np.median(errs):.1e}), within bar: {ok}")
+        if ok and (best is None or ms < variants[best]["ms"]):
+            best = prec
+        del G
+        torch.cuda.synchronize()
+    best = best or precs[0]
+    r = variants[best]
     plain = {}
     if args.plain_loop:
         t0 = time.perf_counter()
         for p, a in zip(plans, arrs):
-            p.run(a, 0, strip=True)
+            p.run(a, 0, strip=True, precision=best)
             p.release()
         torch.cuda.synchronize()
         pms = (time.perf_counter() - t0) * 1e3
-        plain = {"per_instance_loop_ms": pms, "speedup_vs_per_instance_loop": pms / ms}
-        log(f"[bench] per-instance executor loop: {pms:.1f} ms -> {pms / ms:.2f}x")
+        plain = {"per_instance_loop_ms": pms, "speedup_vs_per_instance_loop": pms / r["ms"]}
+        log(f"[bench] per-instance executor loop: {pms:.1f} ms -> {pms / r['ms']:.2f}x")
     cpu = None
     if not args.skip_cpu:
         # CPU oracle (float32, the executed arithmetic) on the cheapest instances, about 10 s of work
@@ -368,18 +385,18 @@ def bench_grouped(args):
                "sample": f"the {done} cheapest instances (float32, stripped), oracle CPU restatement, {dt:.1f} s",
                "instances_per_s": done / dt}
     line = {
-        "metric": "grouped contraction TFLOP/s of uneven instances", "value": round(flops / ms / 1e9, 3),
-        "unit": "TFLOP/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 3),
+        "metric": "grouped contraction TFLOP/s of uneven instances", "value": round(r["tflops"], 3),
+        "unit": "TFLOP/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(r["ms"], 3),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
         "data": "synthetic (seeded random 3-regular graphs, N(0,1) entries)",
         "config": {"workload": args.workload, "instances": len(plans), "vertices": "even in [40, 120]",
                    "bond_dims": [2, 3, 4], "paths": "random greedy, seed = instance id", "strip": True,
-                   "flops_per_step": flops},
-        "instances_per_s": round(len(plans) / (ms * 1e-3), 2), "n_launches": rep["n_launches"],
-        "groups": rep["batch"], "workspace_bytes": rep["workspace_bytes"],
-        "accuracy": {"instances_checked": len(cheap), "max_rel_err_vs_f64": max(errs),
-                     "median_rel_err_vs_f64": float(np.median(errs)),
-                     "cpu_f32_max_rel_err_vs_f64": max(errs32), "oracle_seconds": cpu_s},
+                   "precision": best, "flops_per_step": flops},
+        "instances_per_s": round(len(plans) / (r["ms"] * 1e-3), 2),
+        "accuracy": {"instances_checked": len(cheap), "oracle_seconds": oracle_s,
+                     "bar": "median rel err <= 1e-5 and max rel err <= 2 x the float32 CPU oracle's max error",
+                     "cpu_f32_max_rel_err_vs_f64": max(v[2] for v in refs.values())},
+        "variants": variants,
         "cpu_baseline": cpu,
     }
     line.update(plain)

@@ -136,12 +136,14 @@ void atn_executor_destroy(atn_executor* ex);
 int atn_executor_run(atn_executor* ex, const void* const* d_inputs, void* d_out, void* stream, atn_report* report);
 
 /* Per-kernel accounting of one eager execution with HIP events around every
- * launch on `stream` (used for roofline reporting; no reference counterpart).
- * stats[k] for kernel kind k: name, launches, device ms, algorithmic FLOPs and
- * HBM bytes (operands read + results written). *n_stats <= max_stats kinds. */
+ * executor op on `stream` (used for roofline reporting; no reference
+ * counterpart). stats[k] for kernel kind k (the kind of an op's last kernel):
+ * name, ops, kernel launches, device ms, algorithmic FLOPs and HBM bytes
+ * (operands read + results written). *n_stats <= max_stats kinds. */
 typedef struct {
   char name[32];
-  size_t launches;
+  size_t calls;     /* executor ops (one GEMM, one permute, ...) */
+  size_t launches;  /* kernel launches they made (a split-precision GEMM makes several) */
   double ms;
   double flops;
   double bytes;
@@ -194,7 +196,7 @@ int atn_gemm(int dtype, int precision, int kernel, size_t batch, size_t m, size_
  * kernel kind. inputs[p] is the d_inputs array of plan p; outputs[p] its output.
  * With strip != 0 every step result is renormalised by a power of two and
  * log10_exponents[p] receives the exponent (result = output * 10^e). */
-int atn_grouped_execute(const atn_plan* const* plans, size_t n_plans, int dtype, int strip,
+int atn_grouped_execute(const atn_plan* const* plans, size_t n_plans, int dtype, int precision, int strip,
                         const void* const* const* inputs, void* const* outputs, double* log10_exponents,
                         void* stream, atn_report* report);
 
@@ -204,7 +206,8 @@ int atn_grouped_execute(const atn_plan* const* plans, size_t n_plans, int dtype,
  * output pointers. Replaces repeated CompiledContraction::execute /
  * execute_stripped calls (src/compiled.rs:369-381) over many plans. */
 typedef struct atn_grouped atn_grouped;
-int atn_grouped_create(const atn_plan* const* plans, size_t n_plans, int dtype, int strip, atn_grouped** out);
+int atn_grouped_create(const atn_plan* const* plans, size_t n_plans, int dtype, int precision, int strip,
+                       atn_grouped** out);
 void atn_grouped_destroy(atn_grouped* g);
 int atn_grouped_run(atn_grouped* g, const void* const* const* inputs, void* const* outputs, double* log10_exponents,
                     void* stream, atn_report* report);

@@ -120,3 +120,20 @@ def test_grouped_executor_reused_with_new_inputs(cuda):
         for (net, arrays, path), m, e in zip(insts, outs, exps):
             ref = O.contract(net, [x * scale for x in arrays], path)
             assert abs(float(m.cpu()) * 10.0 ** e - ref) <= 1e-12 * abs(ref)
+
+
+@pytest.mark.parametrize("precision", ["bf16x3", "fp16x2"])
+def test_grouped_split_precision(cuda, precision):
+    """Split products on the steps launched on their own (a chain of large
+    steps), within 2e-5 of the float64 oracle; the small instances stay FP32."""
+    insts = [_large_chain(np.float32, 4), _large_chain(np.float32, 5)]
+    insts += W.uneven_instances(6, seed=13, n_lo=8, n_hi=20, ntrials=1)
+    for strip in (False, True):
+        outs, exps = contract_grouped(_plans(insts), [a for _, a, _ in insts], strip=strip, precision=precision)
+        for (net, arrays, path), m, e in zip(insts, outs, exps):
+            ref = O.contract(net, [a.astype(np.float64) for a in arrays], path)
+            got = np.asarray(m, dtype=np.float64) * 10.0 ** e
+            cpu32 = O.contract(net, arrays, path).astype(np.float64)
+            scale = np.abs(ref).max()
+            bound = max(2e-5 * scale, 4 * np.abs(cpu32 - ref).max())
+            assert np.abs(got - ref).max() <= bound, (precision, strip, np.abs(got - ref).max() / scale)
