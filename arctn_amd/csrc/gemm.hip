@@ -1005,17 +1005,26 @@ __global__ __launch_bounds__(256, 2) void gemm_presplit_kernel(GemmArgs g, PreSp
             ah[i] = *reinterpret_cast<const V*>(&ahi[swz<BK>(wm0 + i * 16 + cl, kg)]);
             al[i] = *reinterpret_cast<const V*>(&alo[swz<BK>(wm0 + i * 16 + cl, kg)]);
           }
+          V bh[TN], bl[TN];
 #pragma unroll
           for (int j = 0; j < TN; ++j) {
-            V bh = *reinterpret_cast<const V*>(&bhi[swz<BK>(wn0 + j * 16 + cl, kg)]);
-            V bl = *reinterpret_cast<const V*>(&blo[swz<BK>(wn0 + j * 16 + cl, kg)]);
-#pragma unroll
-            for (int i = 0; i < TM; ++i) {
-              acc[i][j] = Half<F16>::mfma(al[i], bh, acc[i][j]);
-              acc[i][j] = Half<F16>::mfma(ah[i], bl, acc[i][j]);
-              acc[i][j] = Half<F16>::mfma(ah[i], bh, acc[i][j]);
-            }
+            bh[j] = *reinterpret_cast<const V*>(&bhi[swz<BK>(wn0 + j * 16 + cl, kg)]);
+            bl[j] = *reinterpret_cast<const V*>(&blo[swz<BK>(wn0 + j * 16 + cl, kg)]);
           }
+          // the three products of one accumulator are TM*TN instructions apart,
+          // so consecutive MFMAs never wait on each other's result
+#pragma unroll
+          for (int i = 0; i < TM; ++i)
+#pragma unroll
+            for (int j = 0; j < TN; ++j) acc[i][j] = Half<F16>::mfma(al[i], bh[j], acc[i][j]);
+#pragma unroll
+          for (int i = 0; i < TM; ++i)
+#pragma unroll
+            for (int j = 0; j < TN; ++j) acc[i][j] = Half<F16>::mfma(ah[i], bl[j], acc[i][j]);
+#pragma unroll
+          for (int i = 0; i < TM; ++i)
+#pragma unroll
+            for (int j = 0; j < TN; ++j) acc[i][j] = Half<F16>::mfma(ah[i], bh[j], acc[i][j]);
         }
         if constexpr (NBUF == 2) {
           if (kt + 1 < nk) store(buf ^ 1);
