@@ -141,7 +141,8 @@ void Executor::gen_inputs(bool body, size_t nb, std::vector<Ref>& slots, std::ve
   }
 }
 
-void Executor::gen_gemm(const PairStep& st, size_t nbo, bool va, bool vb, const Ref& a, const Ref& b, const Ref& c) {
+void Executor::gen_gemm(const PairStep& st, size_t nbo, bool va, bool vb, const Ref& a, const Ref& b, const Ref& c,
+                        const Ref* cmax) {
   GemmArgs g{};
   g.m = (int64_t)st.m, g.k = (int64_t)st.k, g.n = (int64_t)st.n;
   g.lda = g.k, g.ldb = g.n, g.ldc = g.n;
@@ -174,12 +175,15 @@ void Executor::gen_gemm(const PairStep& st, size_t nbo, bool va, bool vb, const 
                    GEMM_MFMA_R64, GEMM_MFMA_R128})  // autotune candidates
     scr_bytes = std::max(scr_bytes, gemm_scratch_bytes(dt, prec, kern, g));
   const int sb = scratch_buf_;
+  const Ref mref = cmax ? *cmax : Ref{};
+  if (cmax) use(mref);
   emit(
       [=](const Ctx& cx) {
         GemmArgs q = g;
         q.a = cx.ptr(a), q.b = cx.ptr(b), q.c = cx.ptr(c);
         q.scratch = cx.ptr(scr);
         q.scratch_bytes = bufs_[sb].bytes;
+        if (mref.kind != Ref::NONE) q.cmax = (unsigned long long*)cx.ptr(mref);
         launch_gemm(dt, prec, gemm_choice_[idx], q, cx.s);
       },
       flops, bytes);
@@ -226,16 +230,22 @@ void Executor::gen_steps(bool body, size_t nb, std::vector<Ref>& slots, std::vec
     Ref a = chain(slots[st.li], shapes[st.li], st.sum_a, st.a_perm, nbA);
     Ref b = chain(slots[st.ri], shapes[st.ri], st.sum_b, st.b_perm, nbB);
     Ref c{Ref::WS, newbuf(nbC * st.bt * st.m * st.n * es)};
-    gen_gemm(st, nbC, va, vb, a, b, c);
     if (o_.strip) {
+      // the GEMM folds the max of its result into mb; the rescale then runs only
+      // when that max leaves [2^-STRIP_KEEP, 2^STRIP_KEEP] (always after the last step)
+      Ref mb{Ref::WS, maxbits_buf_};
+      use(mb);
+      emit([=](const Ctx& cx) { launch_fill_zero(cx.ptr(mb), 8, cx.s); });
+      gen_gemm(st, nbC, va, vb, a, b, c, &mb);
       size_t numel = nbC * st.bt * st.m * st.n;
-      Ref mb{Ref::WS, maxbits_buf_}, ex{Ref::WS, expo_buf_};
+      Ref ex{Ref::WS, expo_buf_};
+      const int keep = si + 1 == P.cp.steps.size() ? 0 : STRIP_KEEP;
       use(c);
       emit([=](const Ctx& cx) {
-        launch_fill_zero(cx.ptr(mb), 8, cx.s);
-        launch_maxabs(dt, cx.ptr(c), numel, (uint64_t*)cx.ptr(mb), cx.s);
-        launch_scale_pow2(dt, cx.ptr(c), numel, (const uint64_t*)cx.ptr(mb), (int64_t*)cx.ptr(ex), cx.s);
+        launch_scale_pow2(dt, cx.ptr(c), numel, (const uint64_t*)cx.ptr(mb), (int64_t*)cx.ptr(ex), cx.s, keep);
       });
+    } else {
+      gen_gemm(st, nbC, va, vb, a, b, c);
     }
     slots[st.out] = c;
     shapes[st.out] = st.result_shape;

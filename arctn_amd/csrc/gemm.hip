@@ -54,6 +54,41 @@ template <> struct CM<double2> {
   static __device__ double2 conj(double2 a) { return make_double2(a.x, -a.y); }
 };
 
+// Result max-magnitude tracking for the strip (GemmArgs::cmax): each thread
+// folds max |re|, |im| of the values it stores, one atomicMax per wave at the
+// end. Non-negative doubles order like their bit patterns; NaN never wins.
+// Grid bound of kernels that track the result max (persistent loops over tiles).
+constexpr unsigned TRACK_WORKGROUPS = 8192;
+
+// The running max is kept in the value's own precision (float compares for
+// float results) and widened to double once, when published.
+__device__ inline void fold_max(float& mx, float v) {
+  const float a = fabsf(v);
+  mx = a > mx ? a : mx;
+}
+__device__ inline void fold_max(double& mx, double v) {
+  const double a = fabs(v);
+  mx = a > mx ? a : mx;
+}
+__device__ inline void fold_max(float& mx, float2 v) { fold_max(mx, v.x), fold_max(mx, v.y); }
+__device__ inline void fold_max(double& mx, double2 v) { fold_max(mx, v.x), fold_max(mx, v.y); }
+template <class T> struct MaxOf { using R = float; };
+template <> struct MaxOf<double> { using R = double; };
+template <> struct MaxOf<double2> { using R = double; };
+// every lane of the wave calls this
+__device__ inline void publish_max(double mx, unsigned long long* cmax) {
+  for (int o = 32; o > 0; o >>= 1) {
+    const double t = __shfl_down(mx, o, 64);
+    mx = t > mx ? t : mx;
+  }
+  // one slot for the whole grid: read it first and skip the atomic unless this
+  // wave's max is larger (the slot only grows, so a stale read is harmless)
+  if ((threadIdx.x & 63) == 0 && mx > 0.0) {
+    const unsigned long long b = (unsigned long long)__double_as_longlong(mx);
+    if (b > __hip_atomic_load(cmax, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) atomicMax(cmax, b);
+  }
+}
+
 // ------------------------------------------------------------- simple GEMM --
 // One thread per C element, k accumulated in order. Used for skinny and tiny
 // steps, which are bandwidth- or launch-bound. OPA/OPB = 1 reads the operand as
@@ -142,6 +177,7 @@ __global__ __launch_bounds__(256) void gemm_stream_kernel(GemmArgs g) {
   using W = typename Vec<T, V>::W;
   static_assert(sizeof(W) == V * sizeof(T), "vector width");
   const uint64_t nz = (uint64_t)g.nbo * g.nbi;
+  typename MaxOf<T>::R mx = 0;
   for (uint64_t z = blockIdx.y; z < nz; z += gridDim.y) {
     const uint64_t zo = z / g.nbi, zi = z - zo * g.nbi;
     const T* A = (const T*)g.a + zo * g.sao + zi * g.sai;
@@ -184,9 +220,15 @@ __global__ __launch_bounds__(256) void gemm_stream_kernel(GemmArgs g) {
       }
 #pragma unroll
       for (int i = 0; i < MM; ++i)
-        if (i < g.m) *reinterpret_cast<W*>(C + i * g.ldc + j0) = *reinterpret_cast<const W*>(acc[i]);
+        if (i < g.m) {
+          *reinterpret_cast<W*>(C + i * g.ldc + j0) = *reinterpret_cast<const W*>(acc[i]);
+          if (g.cmax)
+#pragma unroll
+            for (int v = 0; v < V; ++v) fold_max(mx, acc[i][v]);
+        }
     }
   }
+  if (g.cmax) publish_max((double)mx, g.cmax);
 }
 
 // Small n (n <= NN): one thread per output row i keeps all n accumulators;
@@ -334,6 +376,7 @@ static size_t scratch_avail(const GemmArgs& g) { return g.scratch && g.scratch_b
 template <class T, int TM, int TN>
 __global__ __launch_bounds__(256) void gemm_splitk_kernel(GemmArgs g, SplitK sk) {
   __shared__ T red[4][TM * TN];
+  typename MaxOf<T>::R mx = 0;
   const int64_t q0 = (int64_t)blockIdx.y * sk.k_chunk;
   const int64_t q1 = q0 + sk.k_chunk < g.k ? q0 + sk.k_chunk : g.k;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -377,6 +420,7 @@ __global__ __launch_bounds__(256) void gemm_splitk_kernel(GemmArgs g, SplitK sk)
       if (i0 + a < g.m && j0 + b < g.n) {
         if (sk.n_split == 1) {
           ((T*)g.c)[zo * g.sco + zi * g.sci + (i0 + a) * g.ldc + j0 + b] = v;
+          if (g.cmax) fold_max(mx, v);
         } else {
           const int64_t n_out = (int64_t)g.nbo * g.nbi * g.m * g.n;
           ((T*)sk.part)[blockIdx.y * n_out + (z * g.m + i0 + a) * g.n + j0 + b] = v;
@@ -385,18 +429,22 @@ __global__ __launch_bounds__(256) void gemm_splitk_kernel(GemmArgs g, SplitK sk)
     }
     __syncthreads();
   }
+  if (g.cmax && sk.n_split == 1) publish_max((double)mx, g.cmax);
 }
 
 template <class T>
 __global__ __launch_bounds__(256) void splitk_reduce_kernel(GemmArgs g, SplitK sk) {
   const int64_t n_out = (int64_t)g.nbo * g.nbi * g.m * g.n;
+  typename MaxOf<T>::R mx = 0;
   for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n_out; e += (int64_t)gridDim.x * blockDim.x) {
     T v = ((const T*)sk.part)[e];
     for (int64_t sp = 1; sp < sk.n_split; ++sp) v = cadd(v, ((const T*)sk.part)[sp * n_out + e]);
     const int64_t mn = g.m * g.n, z = e / mn, r = e - z * mn, i = r / g.n, j = r - i * g.n;
     const int64_t zo = z / g.nbi, zi = z - zo * g.nbi;
     ((T*)g.c)[zo * g.sco + zi * g.sci + i * g.ldc + j] = v;
+    if (g.cmax) fold_max(mx, v);
   }
+  if (g.cmax) publish_max((double)mx, g.cmax);
 }
 
 // Tile order: column tile fastest. (An XCD-aware order - XCD j taking the
@@ -428,7 +476,9 @@ template <> struct Mfma<double> {
 //   B frag lane l = B[l/16][l%16]; output register r of lane l holds
 //   D[4*(l/16) + r][l%16] for the f32 instruction and D[4*r + l/16][l%16] for
 //   the f64 instruction.
-template <class F, bool CPLX, int BM, int BN, int BK>
+// TRACK: fold the result max into g.cmax (a separate instantiation, so the
+// plain kernel carries no extra live register or epilogue work).
+template <class F, bool CPLX, int BM, int BN, int BK, bool TRACK>
 __global__ __launch_bounds__(256) void gemm_mfma_kernel(GemmArgs g) {
   constexpr int WM = BM / 2, WN = BN / 2;
   constexpr int TM = WM / 16, TN = WN / 16;
@@ -451,6 +501,7 @@ __global__ __launch_bounds__(256) void gemm_mfma_kernel(GemmArgs g) {
   const int wm0 = (wave >> 1) * WM, wn0 = (wave & 1) * WN;
   const uint64_t nz = (uint64_t)g.nbo * g.nbi;
   const uint64_t ntile = (uint64_t)tiles_m * tiles_n;
+  F mx = 0;
   // k range of this blockIdx.z (real units; kc is a multiple of BK)
   const int64_t kbeg = g.kc ? (int64_t)blockIdx.z * g.kc * (CPLX ? 2 : 1) : 0;
   const int64_t kend = g.kc && kbeg + g.kc * (CPLX ? 2 : 1) < K ? kbeg + g.kc * (CPLX ? 2 : 1) : K;
@@ -557,10 +608,14 @@ __global__ __launch_bounds__(256) void gemm_mfma_kernel(GemmArgs g) {
 #pragma unroll
           for (int r = 0; r < 4; ++r) {
             int64_t row = m0 + wm0 + i * 16 + Mfma<F>::row(lane, r), col = n0 + wn0 + j * 16 + cl;
-            if (row < M && col < N) C[row * ldo + col] = acc[i][j][r];
+            if (row < M && col < N) {
+              C[row * ldo + col] = acc[i][j][r];
+              if constexpr (TRACK) fold_max(mx, acc[i][j][r]);
+            }
           }
     }
   }
+  if constexpr (TRACK) publish_max((double)mx, g.cmax);
 }
 
 // ------------------------------------------------------ split-precision GEMM --
@@ -618,6 +673,7 @@ __global__ __launch_bounds__(256, 2) void gemm_split_kernel(GemmArgs g) {
   H* blo = bhi + BN * SK;
 
   float sa = 1.f, sb = 1.f, unscale = 1.f;
+  float mx = 0.f;
   if constexpr (F16) {
     const unsigned long long* mb = (const unsigned long long*)g.scratch;
     int ea = pow2_shift(mb[0]), eb = pow2_shift(mb[1]);
@@ -762,10 +818,14 @@ __global__ __launch_bounds__(256, 2) void gemm_split_kernel(GemmArgs g) {
 #pragma unroll
           for (int r = 0; r < 4; ++r) {
             int64_t row = m0 + wm0 + i * 16 + 4 * (lane >> 4) + r, col = n0 + wn0 + j * 16 + cl;
-            if (row < M && col < N) C[row * ldc + col] = acc[i][j][r] * unscale;
+            if (row < M && col < N) {
+              C[row * ldc + col] = acc[i][j][r] * unscale;
+              if (g.cmax) fold_max(mx, acc[i][j][r] * unscale);
+            }
           }
     }
   }
+  if (g.cmax) publish_max((double)mx, g.cmax);
 }
 
 constexpr size_t SPLIT_LDS = 4 * 128 * (64 + 8) * 2;
@@ -911,6 +971,7 @@ __global__ __launch_bounds__(256, 2) void gemm_presplit_kernel(GemmArgs g, PreSp
   H* lds = reinterpret_cast<H*>(smem_split);  // [NBUF buffers][ahi, alo, bhi, blo][TILE]
 
   float unscale = 1.f;
+  float mx = 0.f;
   if constexpr (F16) {
     const unsigned long long* mb = (const unsigned long long*)g.scratch;
     unscale = ldexpf(1.f, -(pow2_shift(mb[0]) + pow2_shift(mb[1])));
@@ -1046,10 +1107,14 @@ __global__ __launch_bounds__(256, 2) void gemm_presplit_kernel(GemmArgs g, PreSp
             const int lrow = wn0 + j * 16 + cl;  // LDS row of B' = tile column
             const int64_t row = m0 + wm0 + i * 16 + 4 * (lane >> 4) + r;
             const int64_t col = CPLX ? n0 + 2 * (lrow & 63) + (lrow >> 6) : n0 + lrow;
-            if (row < M && col < N) C[row * ldc + col] = acc[i][j][r] * unscale;
+            if (row < M && col < N) {
+              C[row * ldc + col] = acc[i][j][r] * unscale;
+              if (g.cmax) fold_max(mx, acc[i][j][r] * unscale);
+            }
           }
     }
   }
+  if (g.cmax) publish_max((double)mx, g.cmax);
 }
 
 template <class F>
@@ -1092,8 +1157,12 @@ static void run_mfma(int dt, const GemmArgs& g0, hipStream_t s) {
   int64_t N = CPLX ? 2 * g.n : g.n;
   uint64_t tiles = (uint64_t)((g.m + BM - 1) / BM) * (uint64_t)((N + BN - 1) / BN);
   dim3 grid((unsigned)std::min<uint64_t>(tiles, 1u << 20), zgrid(g), (unsigned)ks.n_split);
+  // tracking the result max: a persistent grid keeps the per-wave publishes few
+  if (g.cmax) grid.x = std::min<unsigned>(grid.x, std::max<unsigned>(1, TRACK_WORKGROUPS / (grid.y * grid.z)));
   g_last_kind = sizeof(F) == 4 ? K_GEMM_MFMA_F32 : K_GEMM_MFMA_F64;
-  LAUNCH((gemm_mfma_kernel<F, CPLX, BM, BN, BK>), grid, dim3(256), 0, s, g);
+  // with split k the reduce kernel tracks the result max
+  if (g.cmax && !g.part) LAUNCH((gemm_mfma_kernel<F, CPLX, BM, BN, BK, true>), grid, dim3(256), 0, s, g);
+  else LAUNCH((gemm_mfma_kernel<F, CPLX, BM, BN, BK, false>), grid, dim3(256), 0, s, g);
   if (ks.n_split > 1) {
     SplitK sk{};
     sk.n_split = ks.n_split;
@@ -1133,6 +1202,7 @@ template <class T, int MM, int V>
 static void run_stream_t(const GemmArgs& g, hipStream_t s) {
   const int64_t cols = (g.n + V - 1) / V;
   unsigned gx = (unsigned)std::min<int64_t>((cols + 255) / 256, 65536);
+  if (g.cmax) gx = std::min<unsigned>(gx, std::max<unsigned>(1, TRACK_WORKGROUPS / zgrid(g)));
   dim3 grid(gx, zgrid(g));
   if ((uint64_t)grid.y * gx > (1u << 22)) grid.y = std::max<unsigned>(1, (1u << 22) / gx);
   LAUNCH((gemm_stream_kernel<T, MM, V>), grid, dim3(256), 0, s, g);
@@ -1191,15 +1261,16 @@ static bool run_stream(int dt, const GemmArgs& g, hipStream_t s) {
   return true;
 }
 
-static bool run_skinny(int dt, const GemmArgs& g, hipStream_t s) {
-  if (run_stream(dt, g, s)) return true;
+// 0: not a skinny shape; 1: streaming kernel (folds the result max); 2: smallm / smalln kernel
+static int run_skinny(int dt, const GemmArgs& g, hipStream_t s) {
+  if (run_stream(dt, g, s)) return 1;
   if (g.m <= 32 && g.n >= 64) {
     g_last_kind = K_GEMM_SMALL;
     if (g.m <= 4) run_skinny_t<true, 4>(dt, g, s);
     else if (g.m <= 8) run_skinny_t<true, 8>(dt, g, s);
     else if (g.m <= 16) run_skinny_t<true, 16>(dt, g, s);
     else run_skinny_t<true, 32>(dt, g, s);
-    return true;
+    return 2;
   }
   if (g.n <= 32 && g.m >= 64) {
     g_last_kind = K_GEMM_SMALL;
@@ -1207,9 +1278,9 @@ static bool run_skinny(int dt, const GemmArgs& g, hipStream_t s) {
     else if (g.n <= 8) run_skinny_t<false, 8>(dt, g, s);
     else if (g.n <= 16) run_skinny_t<false, 16>(dt, g, s);
     else run_skinny_t<false, 32>(dt, g, s);
-    return true;
+    return 2;
   }
-  return false;
+  return 0;
 }
 
 
@@ -1296,14 +1367,36 @@ static int auto_kernel(int dt, int prec, const GemmArgs& g) {
   return GEMM_SIMPLE;
 }
 
+static int launch_gemm_kernels(int dt, int prec, int kernel, const GemmArgs& g, hipStream_t s);
+
+// Kernels that fold the result max into their epilogue (GemmArgs::cmax).
+static bool folds_max(int kernel_run) { return kernel_run != GEMM_SIMPLE && kernel_run != -2; }
+
 int launch_gemm(int dt, int prec, int kernel, const GemmArgs& g, hipStream_t s) {
+  if (g.cmax) {
+    const bool contiguous = g.ldc == g.n && (g.nbi == 1 || g.sci == g.m * g.n) &&
+                            (g.nbo == 1 || g.sco == g.nbi * g.m * g.n);
+    if (!contiguous) throw Error("GEMM result max tracking needs a contiguous result");
+  }
+  const int used = launch_gemm_kernels(dt, prec, kernel, g, s);
+  if (g.cmax && !folds_max(used)) {
+    const int last = g_last_kind;
+    launch_maxabs(dt, g.c, (size_t)(g.nbo * g.nbi * g.m * g.n), (uint64_t*)g.cmax, s);
+    g_last_kind = last;
+  }
+  return used < 0 ? GEMM_SKINNY : used;
+}
+
+static int launch_gemm_kernels(int dt, int prec, int kernel, const GemmArgs& g, hipStream_t s) {
   if (kernel == GEMM_AUTO) kernel = auto_kernel(dt, prec, g);
   if (kernel == GEMM_SPLITK) {
     run_splitk(dt, g, s);
     return GEMM_SPLITK;
   }
   if (kernel == GEMM_SKINNY) {
-    if (run_skinny(dt, g, s)) return GEMM_SKINNY;
+    const int r = run_skinny(dt, g, s);
+    if (r == 1) return GEMM_SKINNY;
+    if (r == 2) return -2;  // skinny kernel without the folded max
     kernel = GEMM_SIMPLE;
   }
   if (kernel == GEMM_SIMPLE) {
@@ -1321,6 +1414,7 @@ int launch_gemm(int dt, int prec, int kernel, const GemmArgs& g, hipStream_t s) 
     int64_t N = dt == C64 ? 2 * g.n : g.n;
     uint64_t tiles = (uint64_t)((g.m + 127) / 128) * (uint64_t)((N + 127) / 128);
     dim3 grid((unsigned)std::min<uint64_t>(tiles, 1u << 20), zgrid(g));
+    if (g.cmax) grid.x = std::min<unsigned>(grid.x, std::max<unsigned>(1, TRACK_WORKGROUPS / grid.y));
     if (prec == FP16X2) {
       if (!g.scratch) throw Error("FP16X2 GEMM needs 16 bytes of device scratch");
       // operand max magnitudes -> per-GEMM power-of-two scales

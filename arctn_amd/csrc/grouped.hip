@@ -50,7 +50,8 @@ struct GGemmDesc {
 struct GStripDesc {
   GRef c;
   uint64_t numel, block0;
-  uint32_t plan;  // index of the plan within the group (strip slots)
+  uint32_t plan;   // index of the plan within the group (strip slots)
+  uint32_t final;  // the plan's last step: always normalise (else only outside 2^+-STRIP_KEEP)
 };
 
 constexpr unsigned GRID_CAP = 4096;  // persistent grid: 16 workgroups per CU
@@ -197,6 +198,7 @@ __global__ __launch_bounds__(256) void grouped_scale_kernel(const GStripDesc* __
     if (!(m > 0.0) || !isfinite(m)) continue;
     int ex;
     frexp(m, &ex);
+    if (!p.final && ex >= -STRIP_KEEP && ex <= STRIP_KEEP) continue;
     if (vb == p.block0 && threadIdx.x == 0) expo[p.plan] += ex;
     uint64_t e = (vb - p.block0) * 256ull + threadIdx.x;
     if (e >= p.numel) continue;
@@ -242,7 +244,9 @@ constexpr uint32_t ARENA = 0, INPUT0 = 1, OUTPUT = 0xffffffffu;
 struct Single {  // an op launched on its own
   enum Kind { PERMUTE, COPY, GEMM, SUM, DIAG, TRACE, STRIP } kind = PERMUTE;
   GRef in{}, in2{}, out{};
-  uint32_t plan = 0;  // STRIP: index of the plan within its group
+  uint32_t plan = 0;   // GEMM, STRIP: index of the plan within its group
+  bool fold = false;   // GEMM: fold the result max into the plan's strip slot
+  bool final = false;  // STRIP: the plan's last step (always normalise)
   PermPlan perm;
   GemmArgs gemm{};
   size_t A = 0, d = 0, Bm = 0, C = 0;  // SUM: [A, d, C]; DIAG/TRACE: [A, d, Bm, d, C]; COPY: A bytes
@@ -392,13 +396,15 @@ PlanProg build_plan(const Plan& P, int dt, bool strip, size_t n_rounds) {
       R.gemms.push_back(
           GGemmDesc{a, b, c, (int64_t)st.m, (int64_t)st.k, (int64_t)st.n, (int64_t)st.bt, 0, lanes_for((int64_t)st.k)});
     }
-    if (strip && cnum >= LARGE_NUMEL) {
+    const bool last = si + 1 == C.steps.size();
+    if (strip && cnum >= LARGE_NUMEL) {  // the GEMM (launched on its own) folds the max
+      R.mid.back().fold = true;
       Single sg;
       sg.kind = Single::STRIP;
-      sg.in = c, sg.A = cnum;
+      sg.in = c, sg.A = cnum, sg.final = last;
       R.big_strips.push_back(sg);
     } else if (strip) {
-      R.strips.push_back(GStripDesc{c, cnum, 0, 0});
+      R.strips.push_back(GStripDesc{c, cnum, 0, 0, last ? 1u : 0u});
     }
     slot[st.out] = c;
     shapes[st.out] = st.result_shape;
@@ -541,7 +547,10 @@ GroupedExecutor::GroupedExecutor(std::vector<std::shared_ptr<const Plan>> plans,
           for (GGemmDesc g : R.gemms) g.a = fix(g.a), g.b = fix(g.b), g.c = fix(g.c), gemms.push_back(g);
           for (GStripDesc g : R.strips) g.c = fix(g.c), g.plan = (uint32_t)i, strips.push_back(g);
           for (Single sg : R.pre) sg.in = fix(sg.in), sg.out = fix(sg.out), L.pre.push_back(sg);
-          for (Single sg : R.mid) sg.in = fix(sg.in), sg.in2 = fix(sg.in2), sg.out = fix(sg.out), L.mid.push_back(sg);
+          for (Single sg : R.mid) {
+            sg.in = fix(sg.in), sg.in2 = fix(sg.in2), sg.out = fix(sg.out), sg.plan = (uint32_t)i;
+            L.mid.push_back(sg);
+          }
           for (Single sg : R.big_strips) sg.in = fix(sg.in), sg.plan = (uint32_t)i, L.big_strips.push_back(sg);
         }
         uint64_t b = 0;
@@ -637,6 +646,7 @@ void GroupedExecutor::run(const std::vector<const void* const*>& inputs, const s
             g.a = at(sg.in), g.b = at(sg.in2), g.c = at(sg.out);
             g.scratch = dev + G.gemm_scratch_off;
             g.scratch_bytes = G.gemm_scratch;
+            if (sg.fold) g.cmax = slots + sg.plan;
             launch_gemm(dt, prec_, GEMM_AUTO, g, s);
             break;
           }
@@ -650,6 +660,12 @@ void GroupedExecutor::run(const std::vector<const void* const*>& inputs, const s
       gdispatch(dt, [&](auto tag) {
         using T = decltype(tag);
         for (auto& L : G.launches) {
+          const bool strips = L.strip_blocks || !L.big_strips.empty();
+          if (strips) {  // strip slots zeroed before the GEMMs that fold their max into them
+            hipLaunchKernelGGL(grouped_zero_kernel, dim3((unsigned)((ng + 255) / 256)), dim3(256), 0, s, slots, (uint64_t)ng);
+            ATN_HIP(hipGetLastError());
+            ++g_launch_count;
+          }
           for (auto& sg : L.pre) run_single(sg);
           if (L.perm_blocks) {
             g_last_kind = K_PERMUTE_GENERIC;
@@ -667,17 +683,15 @@ void GroupedExecutor::run(const std::vector<const void* const*>& inputs, const s
             ATN_HIP(hipGetLastError());
             ++g_launch_count;
           }
-          if (L.strip_blocks || !L.big_strips.empty()) {
+          if (strips) {
             g_last_kind = K_STRIP;
-            hipLaunchKernelGGL(grouped_zero_kernel, dim3((unsigned)((ng + 255) / 256)), dim3(256), 0, s, slots, (uint64_t)ng);
-            ++g_launch_count;
             if (L.strip_blocks) {
               hipLaunchKernelGGL(grouped_maxabs_kernel<T>, grid(L.strip_blocks), dim3(256), 0, s,
                                  reinterpret_cast<const GStripDesc*>(dev + L.strip_off), L.n_strip, L.strip_blocks, dbases,
                                  slots);
               ++g_launch_count;
             }
-            for (auto& sg : L.big_strips) launch_maxabs(dt, at(sg.in), sg.A, (uint64_t*)(slots + sg.plan), s);
+            // (the big results' maxima were folded in by their GEMMs)
             if (L.strip_blocks) {
               hipLaunchKernelGGL(grouped_scale_kernel<T>, grid(L.strip_blocks), dim3(256), 0, s,
                                  reinterpret_cast<const GStripDesc*>(dev + L.strip_off), L.n_strip, L.strip_blocks, dbases,
@@ -685,7 +699,8 @@ void GroupedExecutor::run(const std::vector<const void* const*>& inputs, const s
               ++g_launch_count;
             }
             for (auto& sg : L.big_strips)
-              launch_scale_pow2(dt, at(sg.in), sg.A, (const uint64_t*)(slots + sg.plan), (int64_t*)(expo + sg.plan), s);
+              launch_scale_pow2(dt, at(sg.in), sg.A, (const uint64_t*)(slots + sg.plan), (int64_t*)(expo + sg.plan), s,
+                                sg.final ? 0 : STRIP_KEEP);
             ATN_HIP(hipGetLastError());
           }
         }

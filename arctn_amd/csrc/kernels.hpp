@@ -89,7 +89,12 @@ void launch_diag_scatter(int es, bool trace, size_t A, size_t d, size_t Bm, size
 // ---- strip (src/compiled.rs:412-430), power-of-two variant ----------------
 // maxbits: one uint64 slot (non-negative double bits); expo: int64 running exponent.
 void launch_maxabs(int dt, const void* x, size_t numel, uint64_t* maxbits, hipStream_t s);
-void launch_scale_pow2(int dt, void* x, size_t numel, const uint64_t* maxbits, int64_t* expo, hipStream_t s);
+// keep > 0: leave x (and expo) unchanged while the max's exponent e lies in
+// [-keep, keep] — the power-of-two scale is only needed against over/underflow.
+void launch_scale_pow2(int dt, void* x, size_t numel, const uint64_t* maxbits, int64_t* expo, hipStream_t s,
+                       int keep = 0);
+// Exponent bound below which intermediate results of a stripped run are left unscaled.
+constexpr int STRIP_KEEP = 32;
 
 // ---- GEMM -------------------------------------------------------------------
 struct GemmArgs {
@@ -109,6 +114,10 @@ struct GemmArgs {
   // all of k) and the partial-sum buffer when k is split.
   int64_t kc = 0;
   void* part = nullptr;
+  // Optional (zeroed) slot receiving max |re|, |im| over C as double bits: the
+  // strip's max pass folded into the GEMM epilogue (launch_gemm adds a separate
+  // pass after the kernels that do not fold it). C must be contiguous.
+  unsigned long long* cmax = nullptr;
 };
 // GEMM_PRESPLIT: split-precision MFMA GEMM whose operands are split into
 // 16-bit (hi, lo) arrays by a separate pass (GEMM_MFMA splits them while staging

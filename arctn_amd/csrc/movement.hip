@@ -676,24 +676,26 @@ __global__ void maxabs_kernel(const T* __restrict__ x, uint64_t n, unsigned long
 }
 
 template <class T>
-__global__ void scale_pow2_kernel(T* __restrict__ x, uint64_t n, const unsigned long long* maxbits, long long* expo);
+__global__ void scale_pow2_kernel(T* __restrict__ x, uint64_t n, const unsigned long long* maxbits, long long* expo, int keep);
 template <>
-__global__ void scale_pow2_kernel<float2>(float2* __restrict__ x, uint64_t n, const unsigned long long* maxbits, long long* expo) {
+__global__ void scale_pow2_kernel<float2>(float2* __restrict__ x, uint64_t n, const unsigned long long* maxbits, long long* expo, int keep) {
   double m = __longlong_as_double((long long)*maxbits);
   if (!(m > 0.0) || !isfinite(m)) return;
   int e;
   frexp(m, &e);
+  if (keep > 0 && e >= -keep && e <= keep) return;
   if (blockIdx.x == 0 && threadIdx.x == 0) *expo += e;
   float s = (float)ldexp(1.0, -e);
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
     x[i] = make_float2(x[i].x * s, x[i].y * s);
 }
 template <>
-__global__ void scale_pow2_kernel<double2>(double2* __restrict__ x, uint64_t n, const unsigned long long* maxbits, long long* expo) {
+__global__ void scale_pow2_kernel<double2>(double2* __restrict__ x, uint64_t n, const unsigned long long* maxbits, long long* expo, int keep) {
   double m = __longlong_as_double((long long)*maxbits);
   if (!(m > 0.0) || !isfinite(m)) return;
   int e;
   frexp(m, &e);
+  if (keep > 0 && e >= -keep && e <= keep) return;
   if (blockIdx.x == 0 && threadIdx.x == 0) *expo += e;
   double s = ldexp(1.0, -e);
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
@@ -701,21 +703,23 @@ __global__ void scale_pow2_kernel<double2>(double2* __restrict__ x, uint64_t n, 
 }
 
 template <>
-__global__ void scale_pow2_kernel<float>(float* __restrict__ x, uint64_t n, const unsigned long long* maxbits, long long* expo) {
+__global__ void scale_pow2_kernel<float>(float* __restrict__ x, uint64_t n, const unsigned long long* maxbits, long long* expo, int keep) {
   double m = __longlong_as_double((long long)*maxbits);
   if (!(m > 0.0) || !isfinite(m)) return;
   int e;
   frexp(m, &e);
+  if (keep > 0 && e >= -keep && e <= keep) return;
   if (blockIdx.x == 0 && threadIdx.x == 0) *expo += e;
   float s = (float)ldexp(1.0, -e);
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) x[i] *= s;
 }
 template <>
-__global__ void scale_pow2_kernel<double>(double* __restrict__ x, uint64_t n, const unsigned long long* maxbits, long long* expo) {
+__global__ void scale_pow2_kernel<double>(double* __restrict__ x, uint64_t n, const unsigned long long* maxbits, long long* expo, int keep) {
   double m = __longlong_as_double((long long)*maxbits);
   if (!(m > 0.0) || !isfinite(m)) return;
   int e;
   frexp(m, &e);
+  if (keep > 0 && e >= -keep && e <= keep) return;
   if (blockIdx.x == 0 && threadIdx.x == 0) *expo += e;
   double s = ldexp(1.0, -e);
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) x[i] *= s;
@@ -728,12 +732,12 @@ void launch_maxabs(int dt, const void* x, size_t numel, uint64_t* maxbits, hipSt
     LAUNCH(maxabs_kernel<T>, grid_for(numel, 256, 2048), 256, 0, s, (const T*)x, (uint64_t)numel, (unsigned long long*)maxbits);
   });
 }
-void launch_scale_pow2(int dt, void* x, size_t numel, const uint64_t* maxbits, int64_t* expo, hipStream_t s) {
+void launch_scale_pow2(int dt, void* x, size_t numel, const uint64_t* maxbits, int64_t* expo, hipStream_t s, int keep) {
   dispatch_dt(dt, [&](auto tag) {
     using T = decltype(tag);
     g_last_kind = K_STRIP;
     LAUNCH(scale_pow2_kernel<T>, grid_for(numel, 256), 256, 0, s, (T*)x, (uint64_t)numel, (const unsigned long long*)maxbits,
-           (long long*)expo);
+           (long long*)expo, keep);
   });
 }
 
