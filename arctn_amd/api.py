@@ -161,18 +161,20 @@ class Plan:
             _lib.lib().atn_free(p)
 
     def executor(self, dtype_code, precision="native", batched=False, use_graph=False, autotune=False, strip=False,
-                 max_batch=0, workspace_limit=0, slice_range=None):
+                 max_batch=0, workspace_limit=0, slice_range=None, gather=True):
         """``slice_range=(begin, end)``: the batched form over global slices
-        [begin, end) only (their partial sum), e.g. one device's share."""
+        [begin, end) only (their partial sum), e.g. one device's share.
+        ``gather=False`` permutes every GEMM B operand into [k, n] order
+        instead of reading it through gather tables."""
         if precision not in _lib.PREC:
             raise ValueError(f"unknown precision {precision!r}; use one of {sorted(_lib.PREC)}")
         sb, se = (0, 0) if slice_range is None else (int(slice_range[0]), int(slice_range[1]))
         key = (dtype_code, precision, bool(batched), bool(use_graph), bool(autotune), bool(strip), max_batch,
-               workspace_limit, sb, se)
+               workspace_limit, sb, se, bool(gather))
         ex = self._executors.get(key)
         if ex is None:
             o = _lib.ExecOptions(dtype_code, _lib.PREC[precision], int(batched), int(use_graph), int(autotune),
-                                 int(strip), max_batch, workspace_limit, sb, se)
+                                 int(strip), max_batch, workspace_limit, sb, se, int(not gather))
             ex = _Executor(self._h, o)
             self._executors[key] = ex
         return ex

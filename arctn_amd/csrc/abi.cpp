@@ -65,6 +65,7 @@ static ExecOptions to_opts(const atn_exec_options* o) {
   e.workspace_limit = o->workspace_limit;
   e.slice_begin = o->slice_begin;
   e.slice_end = o->slice_end;
+  e.gather = o->permute_b == 0;
   return e;
 }
 

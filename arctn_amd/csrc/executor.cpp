@@ -141,8 +141,43 @@ void Executor::gen_inputs(bool body, size_t nb, std::vector<Ref>& slots, std::ve
   }
 }
 
+bool Executor::add_gather(const std::vector<size_t>& shape, const std::vector<size_t>& perm, size_t n_con, int64_t& row,
+                          int64_t& col) {
+  const size_t r = shape.size();
+  std::vector<int64_t> st(r, 1);
+  for (size_t a = r; a-- > 1;) st[a - 1] = st[a] * (int64_t)shape[a];
+  if (r && st[0] * (int64_t)shape[0] > INT32_MAX) return false;
+  auto table = [&](size_t d0, size_t d1, std::vector<int32_t>& out) {
+    size_t len = 1;
+    for (size_t d = d0; d < d1; ++d) len *= shape[perm[d]];
+    out.assign(len, 0);
+    for (size_t e = 0; e < len; ++e) {  // row-major decode over output axes d0..d1-1
+      size_t rem = e;
+      int64_t off = 0;
+      for (size_t d = d1; d-- > d0;) {
+        const size_t dim = shape[perm[d]];
+        off += (int64_t)(rem % dim) * st[perm[d]];
+        rem /= dim;
+      }
+      out[e] = (int32_t)off;
+    }
+  };
+  std::vector<int32_t> rows, cols;
+  size_t nrow = 1, ncol = 1;
+  for (size_t d = 0; d < n_con; ++d) nrow *= shape[perm[d]];
+  for (size_t d = n_con; d < r; ++d) ncol *= shape[perm[d]];
+  if (const_host_.size() + nrow + ncol > ((size_t)64 << 20)) return false;  // 256 MB of tables at most
+  table(0, n_con, rows);
+  table(n_con, r, cols);
+  row = (int64_t)const_host_.size();
+  const_host_.insert(const_host_.end(), rows.begin(), rows.end());
+  col = (int64_t)const_host_.size();
+  const_host_.insert(const_host_.end(), cols.begin(), cols.end());
+  return true;
+}
+
 void Executor::gen_gemm(const PairStep& st, size_t nbo, bool va, bool vb, const Ref& a, const Ref& b, const Ref& c,
-                        const Ref* cmax) {
+                        const Ref* cmax, int64_t brow, int64_t bcol) {
   GemmArgs g{};
   g.m = (int64_t)st.m, g.k = (int64_t)st.k, g.n = (int64_t)st.n;
   g.lda = g.k, g.ldb = g.n, g.ldc = g.n;
@@ -160,7 +195,7 @@ void Executor::gen_gemm(const PairStep& st, size_t nbo, bool va, bool vb, const 
   }
   int idx = (int)gemm_choice_.size();
   gemm_choice_.push_back(GEMM_AUTO);
-  gemm_sites_.push_back(GemmSite{g, a, b, c});
+  gemm_sites_.push_back(GemmSite{g, a, b, c, brow, bcol});
   int dt = o_.dtype, prec = o_.precision;
   use(a);
   use(b);
@@ -184,6 +219,7 @@ void Executor::gen_gemm(const PairStep& st, size_t nbo, bool va, bool vb, const 
         q.scratch = cx.ptr(scr);
         q.scratch_bytes = bufs_[sb].bytes;
         if (mref.kind != Ref::NONE) q.cmax = (unsigned long long*)cx.ptr(mref);
+        if (bcol >= 0) q.brow = consts_ + brow, q.bcol = consts_ + bcol;
         launch_gemm(dt, prec, gemm_choice_[idx], q, cx.s);
       },
       flops, bytes);
@@ -228,7 +264,25 @@ void Executor::gen_steps(bool body, size_t nb, std::vector<Ref>& slots, std::vec
       return cur;
     };
     Ref a = chain(slots[st.li], shapes[st.li], st.sum_a, st.a_perm, nbA);
-    Ref b = chain(slots[st.ri], shapes[st.ri], st.sum_b, st.b_perm, nbB);
+    // B read through gather tables instead of being permuted into [con, free_b]
+    // (no batch legs, no summed legs): the GEMM kernels address b[p][j] as
+    // row_off[p] + col_off[j] of the unpermuted operand.
+    int64_t brow = -1, bcol = -1;
+    Ref b;
+    if (o_.gather && st.bt == 1 && st.sum_b.empty() && plan_permute(shapes[st.ri], st.b_perm, 1, es).kind != 0) {
+      size_t n_con = 0;  // right legs shared with the left operand and absent from the result
+      for (size_t d = 0; d < st.b_perm.size(); ++d) {
+        const Leg l = st.right_legs[st.b_perm[d]];
+        const bool in_left = std::count(st.left_legs.begin(), st.left_legs.end(), l) > 0;
+        const bool in_res = std::count(st.result_legs.begin(), st.result_legs.end(), l) > 0;
+        if (in_left && !in_res) n_con = d + 1;
+      }
+      if (add_gather(shapes[st.ri], st.b_perm, n_con, brow, bcol)) {
+        b = slots[st.ri];
+        use(b);
+      }
+    }
+    if (bcol < 0) b = chain(slots[st.ri], shapes[st.ri], st.sum_b, st.b_perm, nbB);
     Ref c{Ref::WS, newbuf(nbC * st.bt * st.m * st.n * es)};
     if (o_.strip) {
       // the GEMM folds the max of its result into mb; the rescale then runs only
@@ -236,7 +290,7 @@ void Executor::gen_steps(bool body, size_t nb, std::vector<Ref>& slots, std::vec
       Ref mb{Ref::WS, maxbits_buf_};
       use(mb);
       emit([=](const Ctx& cx) { launch_fill_zero(cx.ptr(mb), 8, cx.s); });
-      gen_gemm(st, nbC, va, vb, a, b, c, &mb);
+      gen_gemm(st, nbC, va, vb, a, b, c, &mb, brow, bcol);
       size_t numel = nbC * st.bt * st.m * st.n;
       Ref ex{Ref::WS, expo_buf_};
       const int keep = si + 1 == P.cp.steps.size() ? 0 : STRIP_KEEP;
@@ -245,7 +299,7 @@ void Executor::gen_steps(bool body, size_t nb, std::vector<Ref>& slots, std::vec
         launch_scale_pow2(dt, cx.ptr(c), numel, (const uint64_t*)cx.ptr(mb), (int64_t*)cx.ptr(ex), cx.s, keep);
       });
     } else {
-      gen_gemm(st, nbC, va, vb, a, b, c);
+      gen_gemm(st, nbC, va, vb, a, b, c, nullptr, brow, bcol);
     }
     slots[st.out] = c;
     shapes[st.out] = st.result_shape;
@@ -301,6 +355,7 @@ void Executor::build(size_t B) {
   pre_.clear(), body_.clear(), tail_.clear(), post_.clear();
   gemm_choice_.clear();
   gemm_sites_.clear();
+  const_host_.clear();
   body_ids_.clear();
   replay_ = record_ = false;
   t_ = 0;
@@ -472,6 +527,10 @@ Executor::Executor(std::shared_ptr<const Plan> plan, const ExecOptions& o) : P_(
   }
   if (ws_bytes_ > limit) throw Error(cat("workspace of ", ws_bytes_, " bytes exceeds the limit of ", limit, " bytes"));
   ATN_HIP(hipMalloc(&ws_, std::max<size_t>(ws_bytes_, 256)));
+  if (!const_host_.empty()) {
+    ATN_HIP(hipMalloc((void**)&consts_, const_host_.size() * sizeof(int32_t)));
+    ATN_HIP(hipMemcpy(consts_, const_host_.data(), const_host_.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+  }
   if (o_.use_graph) ATN_HIP(hipStreamCreateWithFlags(&cap_stream_, hipStreamNonBlocking));
 }
 
@@ -480,6 +539,7 @@ Executor::~Executor() {
   if (graph_) (void)hipGraphDestroy(graph_);
   if (cap_stream_) (void)hipStreamDestroy(cap_stream_);
   if (ws_) (void)hipFree(ws_);
+  if (consts_) (void)hipFree(consts_);
 }
 
 void Executor::run_ops(const std::vector<Op>& ops, const Ctx& c) {
@@ -589,6 +649,7 @@ void Executor::autotune(const Ctx& c, hipStream_t s) {
     GemmArgs q = site.g;
     q.a = c.ptr(site.a), q.b = c.ptr(site.b), q.c = c.ptr(site.c);
     q.scratch = c.ws + offs_[scratch_buf_];
+    if (site.bcol >= 0) q.brow = consts_ + site.brow, q.bcol = consts_ + site.bcol;
     q.scratch_bytes = bufs_[scratch_buf_].bytes;
     float best_ms = 1e30f;
     int choice = GEMM_AUTO;

@@ -21,6 +21,8 @@ struct ExecOptions {
   size_t workspace_limit = 0;
   // global slice range [slice_begin, slice_end) of the batched form; 0, 0 = all slices
   size_t slice_begin = 0, slice_end = 0;
+  // read B operands through gather tables instead of permuting them (where possible)
+  bool gather = true;
 };
 
 struct Report {
@@ -90,9 +92,10 @@ class Executor {
   void gen_steps(bool body, size_t nb, std::vector<Ref>& slots, std::vector<std::vector<size_t>>& shapes);
   void gen_inputs(bool body, size_t nb, std::vector<Ref>& slots, std::vector<std::vector<size_t>>& shapes);
   Ref gen_finalize(size_t nb, const Ref& src, bool to_output);
-  // cmax: optional zeroed slot receiving the result's max magnitude (strip)
+  // cmax: optional zeroed slot receiving the result's max magnitude (strip);
+  // brow / bcol: gather tables of B in consts_ (-1: B is a row-major [k, n] operand)
   void gen_gemm(const PairStep& st, size_t nb_outer, bool va, bool vb, const Ref& a, const Ref& b, const Ref& c,
-                const Ref* cmax = nullptr);
+                const Ref* cmax = nullptr, int64_t brow = -1, int64_t bcol = -1);
   void run_ops(const std::vector<Op>& ops, const Ctx& c);
   void enqueue(const Ctx& c, const std::function<void(const Op&, const Ctx&)>& visit = nullptr);
   void autotune(const Ctx& c, hipStream_t s);
@@ -100,7 +103,16 @@ class Executor {
   struct GemmSite {
     GemmArgs g;
     Ref a, b, c;
+    int64_t brow = -1, bcol = -1;  // gather tables of B in consts_ (element offsets), or -1
   };
+  // Gather tables (int32 element offsets) of B operands read without a permute.
+  std::vector<int32_t> const_host_;
+  int32_t* consts_ = nullptr;
+  // Offsets of the [con][free] view of an operand of `shape` permuted by `perm`
+  // (con = the first n_con output axes), appended to const_host_; false when the
+  // tables would be too large.
+  bool add_gather(const std::vector<size_t>& shape, const std::vector<size_t>& perm, size_t n_con, int64_t& row,
+                  int64_t& col);
 
   std::shared_ptr<const Plan> P_;
   ExecOptions o_;

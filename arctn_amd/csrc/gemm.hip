@@ -89,6 +89,12 @@ __device__ inline void publish_max(double mx, unsigned long long* cmax) {
   }
 }
 
+// B element (p, j) of one batch entry: row-major with ldb, or through the gather
+// tables of an unpermuted operand (GemmArgs::brow / bcol).
+__device__ inline int64_t b_index(const GemmArgs& g, int64_t p, int64_t j) {
+  return g.bcol ? (int64_t)g.brow[p] + (int64_t)g.bcol[j] : p * g.ldb + j;
+}
+
 // ------------------------------------------------------------- simple GEMM --
 // One thread per C element, k accumulated in order. Used for skinny and tiny
 // steps, which are bandwidth- or launch-bound. OPA/OPB = 1 reads the operand as
@@ -107,7 +113,7 @@ __global__ __launch_bounds__(256) void gemm_simple_kernel(GemmArgs g) {
       T acc = CM<T>::zero();
       for (int64_t p = 0; p < g.k; ++p) {
         T a = OPA ? CM<T>::conj(A[p * g.lda + i]) : A[i * g.lda + p];
-        T b = OPB ? CM<T>::conj(B[j * g.ldb + p]) : B[p * g.ldb + j];
+        T b = OPB ? CM<T>::conj(B[j * g.ldb + p]) : B[b_index(g, p, j)];
         acc = CM<T>::mac(acc, a, b);
       }
       C[i * g.ldc + j] = acc;
@@ -194,7 +200,7 @@ __global__ __launch_bounds__(256) void gemm_stream_kernel(GemmArgs g) {
       for (; p + KU <= g.k; p += KU) {
         W b[KU];
 #pragma unroll
-        for (int u = 0; u < KU; ++u) b[u] = *reinterpret_cast<const W*>(B + (p + u) * g.ldb + j0);
+        for (int u = 0; u < KU; ++u) b[u] = *reinterpret_cast<const W*>(B + b_index(g, p + u, j0));
 #pragma unroll
         for (int u = 0; u < KU; ++u) {
           const T* bu = reinterpret_cast<const T*>(&b[u]);
@@ -208,7 +214,7 @@ __global__ __launch_bounds__(256) void gemm_stream_kernel(GemmArgs g) {
         }
       }
       for (; p < g.k; ++p) {
-        W bw = *reinterpret_cast<const W*>(B + p * g.ldb + j0);
+        W bw = *reinterpret_cast<const W*>(B + b_index(g, p, j0));
         const T* bu = reinterpret_cast<const T*>(&bw);
 #pragma unroll
         for (int i = 0; i < MM; ++i)
@@ -398,7 +404,7 @@ __global__ __launch_bounds__(256) void gemm_splitk_kernel(GemmArgs g, SplitK sk)
 #pragma unroll
       for (int a = 0; a < TM; ++a) av[a] = i0 + a < g.m ? A[(i0 + a) * g.lda + q] : CM<T>::zero();
 #pragma unroll
-      for (int b = 0; b < TN; ++b) bv[b] = j0 + b < g.n ? B[q * g.ldb + j0 + b] : CM<T>::zero();
+      for (int b = 0; b < TN; ++b) bv[b] = j0 + b < g.n ? B[b_index(g, q, j0 + b)] : CM<T>::zero();
 #pragma unroll
       for (int a = 0; a < TM; ++a)
 #pragma unroll
@@ -538,15 +544,16 @@ __global__ __launch_bounds__(256) void gemm_mfma_kernel(GemmArgs g) {
             int e = tid + 256 * u, j = e % (BN / 2), p = e / (BN / 2);
             int64_t gp = k0 / 2 + p, gj = n0 / 2 + j;
             bool ok = gp < kend / 2 && gj < g.n;
-            rb[2 * u] = ok ? B[(gp * ldb + gj) * 2] : (F)0;
-            rb[2 * u + 1] = ok ? B[(gp * ldb + gj) * 2 + 1] : (F)0;
+            const int64_t bi = ok ? b_index(g, gp, gj) * 2 : 0;
+            rb[2 * u] = ok ? B[bi] : (F)0;
+            rb[2 * u + 1] = ok ? B[bi + 1] : (F)0;
           }
         } else {
 #pragma unroll
           for (int u = 0; u < B_PER; ++u) {
             int e = tid + 256 * u, col = e % BN, row = e / BN;
             int64_t gr = k0 + row, gc = n0 + col;
-            rb[u] = (gr < kend && gc < N) ? B[gr * ldb + gc] : (F)0;
+            rb[u] = (gr < kend && gc < N) ? B[b_index(g, gr, gc)] : (F)0;
           }
         }
       };
@@ -725,23 +732,23 @@ __global__ __launch_bounds__(256, 2) void gemm_split_kernel(GemmArgs g) {
             int q = tid + 256 * u, j = q % (BN / 2), pp = q / (BN / 2);
             int64_t gp = k0 / 2 + 2 * pp, gj = n0 / 2 + j;
             bool ok0 = gj < g.n && gp < g.k, ok1 = gj < g.n && gp + 1 < g.k;
-            const float* b0 = B + (gp * ldb + gj) * 2;
+            const float* b0 = ok0 ? B + b_index(g, gp, gj) * 2 : B;
+            const float* b1 = ok1 ? B + b_index(g, gp + 1, gj) * 2 : B;
             rb[u].x = ok0 ? b0[0] : 0.f;
             rb[u].y = ok0 ? b0[1] : 0.f;
-            rb[u].z = ok1 ? b0[2 * ldb] : 0.f;
-            rb[u].w = ok1 ? b0[2 * ldb + 1] : 0.f;
+            rb[u].z = ok1 ? b1[0] : 0.f;
+            rb[u].w = ok1 ? b1[1] : 0.f;
           }
         } else {
 #pragma unroll
           for (int u = 0; u < B_Q; ++u) {  // 4 consecutive k rows of one column
             int q = tid + 256 * u, col = q % BN, r4 = (q / BN) * 4;
             int64_t gr = k0 + r4, gc = n0 + col;
-            const float* b0 = B + gr * ldb + gc;
             bool okc = gc < N;
-            rb[u].x = (okc && gr < K) ? b0[0] : 0.f;
-            rb[u].y = (okc && gr + 1 < K) ? b0[ldb] : 0.f;
-            rb[u].z = (okc && gr + 2 < K) ? b0[2 * ldb] : 0.f;
-            rb[u].w = (okc && gr + 3 < K) ? b0[3 * ldb] : 0.f;
+            rb[u].x = (okc && gr < K) ? B[b_index(g, gr, gc)] : 0.f;
+            rb[u].y = (okc && gr + 1 < K) ? B[b_index(g, gr + 1, gc)] : 0.f;
+            rb[u].z = (okc && gr + 2 < K) ? B[b_index(g, gr + 2, gc)] : 0.f;
+            rb[u].w = (okc && gr + 3 < K) ? B[b_index(g, gr + 3, gc)] : 0.f;
           }
         }
       };
@@ -899,7 +906,7 @@ __global__ __launch_bounds__(256) void split_bt_kernel(GemmArgs g, PreSplit ps) 
       const int kk = t / TN + 4 * s, nn = t % TN;
       const int64_t gk = k0 + kk, gn = n0 + nn;
       E v{};
-      if (gk < g.k && gn < g.n) v = src[gk * g.ldb + gn];
+      if (gk < g.k && gn < g.n) v = src[b_index(g, gk, gn)];
       tile[kk][nn] = v;
     }
     __syncthreads();
@@ -1210,6 +1217,7 @@ static void run_stream_t(const GemmArgs& g, hipStream_t s) {
 
 // 16-byte column vectors when every row start of B and C is 16-byte aligned.
 static int stream_width(int dt, const GemmArgs& g) {
+  if (g.bcol) return 1;  // gathered columns are not contiguous
   const int es = (int)dtype_size(dt);
   int v = 16 / es;
   auto ok = [&](int w) {
@@ -1264,6 +1272,7 @@ static bool run_stream(int dt, const GemmArgs& g, hipStream_t s) {
 // 0: not a skinny shape; 1: streaming kernel (folds the result max); 2: smallm / smalln kernel
 static int run_skinny(int dt, const GemmArgs& g, hipStream_t s) {
   if (run_stream(dt, g, s)) return 1;
+  if (g.bcol) return 0;  // the smallm / smalln kernels read B row-major only
   if (g.m <= 32 && g.n >= 64) {
     g_last_kind = K_GEMM_SMALL;
     if (g.m <= 4) run_skinny_t<true, 4>(dt, g, s);

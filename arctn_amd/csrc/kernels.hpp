@@ -118,6 +118,12 @@ struct GemmArgs {
   // strip's max pass folded into the GEMM epilogue (launch_gemm adds a separate
   // pass after the kernels that do not fold it). C must be contiguous.
   unsigned long long* cmax = nullptr;
+  // Optional gather tables: B element (p, j) of a batch entry sits at
+  // brow[p] + bcol[j] (elements) instead of p * ldb + j, so B can be read from
+  // an operand that was never permuted into [k, n] order. Supported by the
+  // simple, streaming, split-K, MFMA and split-precision kernels.
+  const int32_t* brow = nullptr;
+  const int32_t* bcol = nullptr;
 };
 // GEMM_PRESPLIT: split-precision MFMA GEMM whose operands are split into
 // 16-bit (hi, lo) arrays by a separate pass (GEMM_MFMA splits them while staging

@@ -208,6 +208,14 @@ def main():
                  "plain_rel_err": abs(complex(p_out.cpu()) - ref) / abs(ref) if ref is not None else None,
                  "speedup_vs_plain": rep["elapsed_ms"] / ms}
         log(f"[bench] plain per-slice FP32 path: {rep['elapsed_ms']:.1f} ms -> speedup {rep['elapsed_ms'] / ms:.2f}x")
+        plan.release()
+        # the same per-slice path with every B operand permuted as the reference lays it out
+        plan.run(x64, 2, batched=False, gather=False)
+        torch.cuda.synchronize()
+        _, rep_p = plan.run(x64, 2, batched=False, gather=False, report=True)
+        plain["plain_fp32_permuted_ms"] = rep_p["elapsed_ms"]
+        plain["speedup_vs_plain_permuted"] = rep_p["elapsed_ms"] / ms
+        log(f"[bench] plain per-slice FP32 path with permuted B operands: {rep_p['elapsed_ms']:.1f} ms")
 
     cpu = None
     if not args.skip_cpu:

@@ -93,6 +93,8 @@ typedef struct {
   size_t workspace_limit;  /* bytes; 0 = 60% of free device memory */
   size_t slice_begin;   /* batched form: run global slices [slice_begin, slice_end) only and */
   size_t slice_end;     /* return their partial sum (the share of one device); 0, 0 = all */
+  int permute_b;        /* 1: always permute GEMM B operands into [k, n] order; 0: read them
+                           through gather tables where the layout allows (no copy) */
 } atn_exec_options;
 
 typedef struct {

@@ -232,3 +232,39 @@ def test_autotuned_batched_graph_forms(cuda, precision):
         got = contract_sliced(net["inputs"], net["output"], net["size_dict"], arrays, sliced, ssa_path=path,
                               batched=True, use_graph=graph, autotune=True, precision=precision)
         assert abs(complex(got) - want) / abs(want) < 1e-5, graph
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.complex128, np.complex64])
+def test_gather_tables_match_permuted_operands(cuda, dtype):
+    """B operands read through gather tables (no permute) give the results of the
+    permuted operands: random networks in every form, and a sliced circuit in every
+    precision (against the complex128 oracle)."""
+    import torch
+
+    tol = 1e-12 if np.dtype(dtype) in (np.dtype(np.float64), np.dtype(np.complex128)) else 1e-5
+    for seed in range(10):
+        net, path = _random_case(seed)
+        arrays = W.random_arrays(net, dtype, seed)
+        want = O.contract(net, [a.astype(np.complex128) for a in arrays], path, "naive")
+        scale = max(1.0, np.abs(want).max())
+        plan = Plan(net["inputs"], net["output"], net["size_dict"], path, sorted(net["size_dict"])[:1])
+        dev = [torch.from_numpy(a).cuda() for a in arrays]
+        code = {np.dtype(np.float64): 1, np.dtype(np.complex128): 3, np.dtype(np.complex64): 2}[np.dtype(dtype)]
+        for mode in MODES:
+            g1, _ = plan.run(dev, code, gather=True, **mode)
+            g0, _ = plan.run(dev, code, gather=False, **mode)
+            g1, g0 = g1.cpu().numpy(), g0.cpu().numpy()
+            assert np.abs(g1 - g0).max() <= tol * scale, (seed, mode)
+            assert np.abs(g1 - want).max() <= max(tol, 1e-4 if tol > 1e-6 else tol) * scale * 10, (seed, mode)
+    if dtype != np.complex64:
+        return
+    net, arrays, _ = W.random_circuit_amplitude(4, 4, 8, 0)
+    path = P.greedy(net)
+    sliced = P.find_slices_to_size(net, path, 2 ** 7)
+    want = complex(O.contract(net, [a.astype(np.complex128) for a in arrays], path, "compiled"))
+    plan = Plan(net["inputs"], net["output"], net["size_dict"], path, sliced)
+    dev = [torch.from_numpy(a).cuda() for a in arrays]
+    for prec in ("native", "bf16x3", "fp16x2"):
+        for gather in (True, False):
+            got, _ = plan.run(dev, 2, batched=True, use_graph=True, precision=prec, gather=gather)
+            assert abs(complex(got.cpu()) - want) / abs(want) < 1e-5, (prec, gather)
