@@ -142,7 +142,7 @@ void Executor::gen_inputs(bool body, size_t nb, std::vector<Ref>& slots, std::ve
 }
 
 bool Executor::add_gather(const std::vector<size_t>& shape, const std::vector<size_t>& perm, size_t n_con, int64_t& row,
-                          int64_t& col) {
+                          int64_t& col, int32_t& col_run) {
   const size_t r = shape.size();
   std::vector<int64_t> st(r, 1);
   for (size_t a = r; a-- > 1;) st[a - 1] = st[a] * (int64_t)shape[a];
@@ -169,6 +169,13 @@ bool Executor::add_gather(const std::vector<size_t>& shape, const std::vector<si
   if (const_host_.size() + nrow + ncol > ((size_t)64 << 20)) return false;  // 256 MB of tables at most
   table(0, n_con, rows);
   table(n_con, r, cols);
+  // innermost free output axes that are also the innermost input axes: runs of
+  // consecutive offsets starting at multiples of the run length
+  col_run = 1;
+  for (size_t d = r, a = r; d-- > n_con && a-- > 0;) {
+    if (perm[d] != a) break;
+    col_run *= (int32_t)shape[a];
+  }
   row = (int64_t)const_host_.size();
   const_host_.insert(const_host_.end(), rows.begin(), rows.end());
   col = (int64_t)const_host_.size();
@@ -177,7 +184,7 @@ bool Executor::add_gather(const std::vector<size_t>& shape, const std::vector<si
 }
 
 void Executor::gen_gemm(const PairStep& st, size_t nbo, bool va, bool vb, const Ref& a, const Ref& b, const Ref& c,
-                        const Ref* cmax, int64_t brow, int64_t bcol) {
+                        const Ref* cmax, int64_t brow, int64_t bcol, int32_t bcol_run) {
   GemmArgs g{};
   g.m = (int64_t)st.m, g.k = (int64_t)st.k, g.n = (int64_t)st.n;
   g.lda = g.k, g.ldb = g.n, g.ldc = g.n;
@@ -195,7 +202,8 @@ void Executor::gen_gemm(const PairStep& st, size_t nbo, bool va, bool vb, const 
   }
   int idx = (int)gemm_choice_.size();
   gemm_choice_.push_back(GEMM_AUTO);
-  gemm_sites_.push_back(GemmSite{g, a, b, c, brow, bcol});
+  g.bcol_run = bcol_run;
+  gemm_sites_.push_back(GemmSite{g, a, b, c, brow, bcol, bcol_run});
   int dt = o_.dtype, prec = o_.precision;
   use(a);
   use(b);
@@ -268,6 +276,7 @@ void Executor::gen_steps(bool body, size_t nb, std::vector<Ref>& slots, std::vec
     // (no batch legs, no summed legs): the GEMM kernels address b[p][j] as
     // row_off[p] + col_off[j] of the unpermuted operand.
     int64_t brow = -1, bcol = -1;
+    int32_t bcol_run = 1;
     Ref b;
     if (o_.gather && st.bt == 1 && st.sum_b.empty() && plan_permute(shapes[st.ri], st.b_perm, 1, es).kind != 0) {
       size_t n_con = 0;  // right legs shared with the left operand and absent from the result
@@ -277,7 +286,7 @@ void Executor::gen_steps(bool body, size_t nb, std::vector<Ref>& slots, std::vec
         const bool in_res = std::count(st.result_legs.begin(), st.result_legs.end(), l) > 0;
         if (in_left && !in_res) n_con = d + 1;
       }
-      if (add_gather(shapes[st.ri], st.b_perm, n_con, brow, bcol)) {
+      if (add_gather(shapes[st.ri], st.b_perm, n_con, brow, bcol, bcol_run)) {
         b = slots[st.ri];
         use(b);
       }
@@ -290,7 +299,7 @@ void Executor::gen_steps(bool body, size_t nb, std::vector<Ref>& slots, std::vec
       Ref mb{Ref::WS, maxbits_buf_};
       use(mb);
       emit([=](const Ctx& cx) { launch_fill_zero(cx.ptr(mb), 8, cx.s); });
-      gen_gemm(st, nbC, va, vb, a, b, c, &mb, brow, bcol);
+      gen_gemm(st, nbC, va, vb, a, b, c, &mb, brow, bcol, bcol_run);
       size_t numel = nbC * st.bt * st.m * st.n;
       Ref ex{Ref::WS, expo_buf_};
       const int keep = si + 1 == P.cp.steps.size() ? 0 : STRIP_KEEP;
@@ -299,7 +308,7 @@ void Executor::gen_steps(bool body, size_t nb, std::vector<Ref>& slots, std::vec
         launch_scale_pow2(dt, cx.ptr(c), numel, (const uint64_t*)cx.ptr(mb), (int64_t*)cx.ptr(ex), cx.s, keep);
       });
     } else {
-      gen_gemm(st, nbC, va, vb, a, b, c, nullptr, brow, bcol);
+      gen_gemm(st, nbC, va, vb, a, b, c, nullptr, brow, bcol, bcol_run);
     }
     slots[st.out] = c;
     shapes[st.out] = st.result_shape;

@@ -95,7 +95,7 @@ class Executor {
   // cmax: optional zeroed slot receiving the result's max magnitude (strip);
   // brow / bcol: gather tables of B in consts_ (-1: B is a row-major [k, n] operand)
   void gen_gemm(const PairStep& st, size_t nb_outer, bool va, bool vb, const Ref& a, const Ref& b, const Ref& c,
-                const Ref* cmax = nullptr, int64_t brow = -1, int64_t bcol = -1);
+                const Ref* cmax = nullptr, int64_t brow = -1, int64_t bcol = -1, int32_t bcol_run = 1);
   void run_ops(const std::vector<Op>& ops, const Ctx& c);
   void enqueue(const Ctx& c, const std::function<void(const Op&, const Ctx&)>& visit = nullptr);
   void autotune(const Ctx& c, hipStream_t s);
@@ -104,6 +104,7 @@ class Executor {
     GemmArgs g;
     Ref a, b, c;
     int64_t brow = -1, bcol = -1;  // gather tables of B in consts_ (element offsets), or -1
+    int32_t bcol_run = 1;
   };
   // Gather tables (int32 element offsets) of B operands read without a permute.
   std::vector<int32_t> const_host_;
@@ -112,7 +113,7 @@ class Executor {
   // (con = the first n_con output axes), appended to const_host_; false when the
   // tables would be too large.
   bool add_gather(const std::vector<size_t>& shape, const std::vector<size_t>& perm, size_t n_con, int64_t& row,
-                  int64_t& col);
+                  int64_t& col, int32_t& col_run);
 
   std::shared_ptr<const Plan> P_;
   ExecOptions o_;

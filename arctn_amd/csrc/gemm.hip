@@ -94,6 +94,9 @@ __device__ inline void publish_max(double mx, unsigned long long* cmax) {
 __device__ inline int64_t b_index(const GemmArgs& g, int64_t p, int64_t j) {
   return g.bcol ? (int64_t)g.brow[p] + (int64_t)g.bcol[j] : p * g.ldb + j;
 }
+// The two halves of b_index, for loaders whose column stays fixed across k tiles.
+__device__ inline int64_t b_row(const GemmArgs& g, int64_t p) { return g.bcol ? (int64_t)g.brow[p] : p * g.ldb; }
+__device__ inline int64_t b_col(const GemmArgs& g, int64_t j) { return g.bcol ? (int64_t)g.bcol[j] : j; }
 
 // ------------------------------------------------------------- simple GEMM --
 // One thread per C element, k accumulated in order. Used for skinny and tiny
@@ -492,6 +495,7 @@ __global__ __launch_bounds__(256) void gemm_mfma_kernel(GemmArgs g) {
   constexpr int SA = BM + PAD, SB = BN + PAD;
   constexpr int A_PER = BM * BK / 256;
   constexpr int B_PER = CPLX ? (BK / 2) * (BN / 2) / 256 : BK * BN / 256;
+  static_assert(256 % (CPLX ? BN / 2 : BN) == 0, "a thread's B loads share one column");
   static_assert(A_PER >= 1 && B_PER >= 1, "tile too small for 256 threads");
   using V = typename Mfma<F>::V;
   __shared__ F As[BK * SA];
@@ -531,6 +535,10 @@ __global__ __launch_bounds__(256) void gemm_mfma_kernel(GemmArgs g) {
 
       F ra[A_PER];
       F rb[CPLX ? 2 * B_PER : B_PER];
+      // every B element a thread loads lies in one column (256 is a multiple of BN)
+      const int64_t bcn = CPLX ? n0 / 2 + tid % (BN / 2) : n0 + tid % BN;
+      const bool bcn_ok = bcn < (CPLX ? g.n : N);
+      const int64_t bco = bcn_ok ? b_col(g, bcn) : 0;
       auto load = [&](int64_t k0) {
 #pragma unroll
         for (int u = 0; u < A_PER; ++u) {
@@ -541,19 +549,19 @@ __global__ __launch_bounds__(256) void gemm_mfma_kernel(GemmArgs g) {
         if constexpr (CPLX) {
 #pragma unroll
           for (int u = 0; u < B_PER; ++u) {
-            int e = tid + 256 * u, j = e % (BN / 2), p = e / (BN / 2);
-            int64_t gp = k0 / 2 + p, gj = n0 / 2 + j;
-            bool ok = gp < kend / 2 && gj < g.n;
-            const int64_t bi = ok ? b_index(g, gp, gj) * 2 : 0;
+            int e = tid + 256 * u, p = e / (BN / 2);
+            int64_t gp = k0 / 2 + p;
+            bool ok = gp < kend / 2 && bcn_ok;
+            const int64_t bi = ok ? (b_row(g, gp) + bco) * 2 : 0;
             rb[2 * u] = ok ? B[bi] : (F)0;
             rb[2 * u + 1] = ok ? B[bi + 1] : (F)0;
           }
         } else {
 #pragma unroll
           for (int u = 0; u < B_PER; ++u) {
-            int e = tid + 256 * u, col = e % BN, row = e / BN;
-            int64_t gr = k0 + row, gc = n0 + col;
-            rb[u] = (gr < kend && gc < N) ? B[b_index(g, gr, gc)] : (F)0;
+            int e = tid + 256 * u, row = e / BN;
+            int64_t gr = k0 + row;
+            rb[u] = (gr < kend && bcn_ok) ? B[b_row(g, gr) + bco] : (F)0;
           }
         }
       };
@@ -695,6 +703,7 @@ __global__ __launch_bounds__(256, 2) void gemm_split_kernel(GemmArgs g) {
   constexpr int A_Q = BM * BK / 4 / 256;  // float4 groups of A per thread (8)
   // B items per thread: complex -> (complex row pair, complex column); real -> (4 k rows, column)
   constexpr int B_Q = CPLX ? (BK / 4) * (BN / 2) / 256 : (BK / 4) * BN / 256;  // 4 / 8
+  static_assert(256 % (CPLX ? BN / 2 : BN) == 0, "a thread's B loads share one column");
 
   for (uint64_t z = blockIdx.y; z < nz; z += gridDim.y) {
     uint64_t zo = z / g.nbi, zi = z - zo * g.nbi;
@@ -711,6 +720,10 @@ __global__ __launch_bounds__(256, 2) void gemm_split_kernel(GemmArgs g) {
         for (int j = 0; j < TN; ++j) acc[i][j] = v4f{0, 0, 0, 0};
       float4 ra[A_Q];
       float4 rb[B_Q];
+      // every B element a thread loads lies in one column (256 is a multiple of BN)
+      const int64_t bcn = CPLX ? n0 / 2 + tid % (BN / 2) : n0 + tid % BN;
+      const bool bcn_ok = bcn < (CPLX ? g.n : N);
+      const int64_t bco = bcn_ok ? b_col(g, bcn) : 0;
       auto load = [&](int64_t k0) {
 #pragma unroll
         for (int u = 0; u < A_Q; ++u) {
@@ -729,11 +742,11 @@ __global__ __launch_bounds__(256, 2) void gemm_split_kernel(GemmArgs g) {
         if constexpr (CPLX) {
 #pragma unroll
           for (int u = 0; u < B_Q; ++u) {  // complex rows p, p+1 of one complex column j
-            int q = tid + 256 * u, j = q % (BN / 2), pp = q / (BN / 2);
-            int64_t gp = k0 / 2 + 2 * pp, gj = n0 / 2 + j;
-            bool ok0 = gj < g.n && gp < g.k, ok1 = gj < g.n && gp + 1 < g.k;
-            const float* b0 = ok0 ? B + b_index(g, gp, gj) * 2 : B;
-            const float* b1 = ok1 ? B + b_index(g, gp + 1, gj) * 2 : B;
+            int q = tid + 256 * u, pp = q / (BN / 2);
+            int64_t gp = k0 / 2 + 2 * pp;
+            bool ok0 = bcn_ok && gp < g.k, ok1 = bcn_ok && gp + 1 < g.k;
+            const float* b0 = ok0 ? B + (b_row(g, gp) + bco) * 2 : B;
+            const float* b1 = ok1 ? B + (b_row(g, gp + 1) + bco) * 2 : B;
             rb[u].x = ok0 ? b0[0] : 0.f;
             rb[u].y = ok0 ? b0[1] : 0.f;
             rb[u].z = ok1 ? b1[0] : 0.f;
@@ -742,13 +755,13 @@ __global__ __launch_bounds__(256, 2) void gemm_split_kernel(GemmArgs g) {
         } else {
 #pragma unroll
           for (int u = 0; u < B_Q; ++u) {  // 4 consecutive k rows of one column
-            int q = tid + 256 * u, col = q % BN, r4 = (q / BN) * 4;
-            int64_t gr = k0 + r4, gc = n0 + col;
-            bool okc = gc < N;
-            rb[u].x = (okc && gr < K) ? B[b_index(g, gr, gc)] : 0.f;
-            rb[u].y = (okc && gr + 1 < K) ? B[b_index(g, gr + 1, gc)] : 0.f;
-            rb[u].z = (okc && gr + 2 < K) ? B[b_index(g, gr + 2, gc)] : 0.f;
-            rb[u].w = (okc && gr + 3 < K) ? B[b_index(g, gr + 3, gc)] : 0.f;
+            int q = tid + 256 * u, r4 = (q / BN) * 4;
+            int64_t gr = k0 + r4;
+            bool okc = bcn_ok;
+            rb[u].x = (okc && gr < K) ? B[b_row(g, gr) + bco] : 0.f;
+            rb[u].y = (okc && gr + 1 < K) ? B[b_row(g, gr + 1) + bco] : 0.f;
+            rb[u].z = (okc && gr + 2 < K) ? B[b_row(g, gr + 2) + bco] : 0.f;
+            rb[u].w = (okc && gr + 3 < K) ? B[b_row(g, gr + 3) + bco] : 0.f;
           }
         }
       };
@@ -901,12 +914,14 @@ __global__ __launch_bounds__(256) void split_bt_kernel(GemmArgs g, PreSplit ps) 
     const int64_t mat = b / (tk * tn), r = b - mat * tk * tn, k0 = (r / tn) * TK, n0 = (r - (r / tn) * tn) * TN;
     const int64_t zo = mat / inner, zi = mat - zo * inner;
     const E* src = (const E*)g.b + zo * g.sbo + zi * g.sbi;
+    const int64_t gcn = n0 + t % TN;  // the thread's column in every load below
+    const int64_t gco = gcn < g.n ? b_col(g, gcn) : 0;
 #pragma unroll
     for (int s = 0; s < TK / 4; ++s) {
       const int kk = t / TN + 4 * s, nn = t % TN;
-      const int64_t gk = k0 + kk, gn = n0 + nn;
+      const int64_t gk = k0 + kk;
       E v{};
-      if (gk < g.k && gn < g.n) v = src[b_index(g, gk, gn)];
+      if (gk < g.k && gcn < g.n) v = src[b_row(g, gk) + gco];
       tile[kk][nn] = v;
     }
     __syncthreads();
@@ -1217,12 +1232,13 @@ static void run_stream_t(const GemmArgs& g, hipStream_t s) {
 
 // 16-byte column vectors when every row start of B and C is 16-byte aligned.
 static int stream_width(int dt, const GemmArgs& g) {
-  if (g.bcol) return 1;  // gathered columns are not contiguous
   const int es = (int)dtype_size(dt);
   int v = 16 / es;
   auto ok = [&](int w) {
     const int64_t b = (int64_t)w * es;
-    return g.n % w == 0 && g.ldb % w == 0 && g.ldc % w == 0 && g.sbo % w == 0 && g.sbi % w == 0 && g.sco % w == 0 &&
+    // gathered columns: only within runs of consecutive offsets
+    const bool cols = g.bcol ? g.bcol_run % w == 0 : g.ldb % w == 0;
+    return g.n % w == 0 && cols && g.ldc % w == 0 && g.sbo % w == 0 && g.sbi % w == 0 && g.sco % w == 0 &&
            g.sci % w == 0 && (uintptr_t)g.b % b == 0 && (uintptr_t)g.c % b == 0;
   };
   while (v > 1 && !ok(v)) v /= 2;

@@ -124,6 +124,9 @@ struct GemmArgs {
   // simple, streaming, split-K, MFMA and split-precision kernels.
   const int32_t* brow = nullptr;
   const int32_t* bcol = nullptr;
+  // bcol holds runs of this many consecutive offsets, each starting at a multiple
+  // of the run length (vector loads stay within a run)
+  int32_t bcol_run = 1;
 };
 // GEMM_PRESPLIT: split-precision MFMA GEMM whose operands are split into
 // 16-bit (hi, lo) arrays by a separate pass (GEMM_MFMA splits them while staging

@@ -131,7 +131,9 @@ def main():
             + (f"{err:.2e}" if err is not None else "n/a"))
         plan.release()
     key = lambda c: c[0] + ("+autotune" if c[1] else "")
-    ok = [c for c in cands if trial[key(c)]["rel_err"] is not None and trial[key(c)]["rel_err"] <= 1e-5] or cands
+    # half the 1e-5 bar as the selection margin: a form's error varies a little with
+    # the autotuner's per-run kernel choices, and the timed run must stay within 1e-5
+    ok = [c for c in cands if trial[key(c)]["rel_err"] is not None and trial[key(c)]["rel_err"] <= 5e-6] or cands
     prec, tune = min(ok, key=lambda c: trial[key(c)]["ms"])
     form["precision"], form["autotune"] = prec, tune
     log(f"[bench] measuring form {key((prec, tune))} ({trial[key((prec, tune))]['launches']} launches per contraction)")
