@@ -37,7 +37,7 @@ class ExecOptions(C.Structure):
     _fields_ = [("dtype", C.c_int), ("precision", C.c_int), ("batched", C.c_int), ("use_graph", C.c_int),
                 ("autotune", C.c_int), ("strip", C.c_int), ("max_batch", C.c_size_t),
                 ("workspace_limit", C.c_size_t), ("slice_begin", C.c_size_t), ("slice_end", C.c_size_t),
-                ("permute_b", C.c_int)]
+                ("permute_b", C.c_int), ("gemm_kernel", C.c_int)]
 
 
 class Report(C.Structure):

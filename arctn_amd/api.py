@@ -161,20 +161,22 @@ class Plan:
             _lib.lib().atn_free(p)
 
     def executor(self, dtype_code, precision="native", batched=False, use_graph=False, autotune=False, strip=False,
-                 max_batch=0, workspace_limit=0, slice_range=None, gather=True):
+                 max_batch=0, workspace_limit=0, slice_range=None, gather=True, gemm_kernel=None):
         """``slice_range=(begin, end)``: the batched form over global slices
         [begin, end) only (their partial sum), e.g. one device's share.
         ``gather=False`` permutes every GEMM B operand into [k, n] order
-        instead of reading it through gather tables."""
+        instead of reading it through gather tables. ``gemm_kernel=k`` runs
+        every GEMM step on kernel code k of ``atn_gemm`` (None: chosen per step)."""
         if precision not in _lib.PREC:
             raise ValueError(f"unknown precision {precision!r}; use one of {sorted(_lib.PREC)}")
         sb, se = (0, 0) if slice_range is None else (int(slice_range[0]), int(slice_range[1]))
         key = (dtype_code, precision, bool(batched), bool(use_graph), bool(autotune), bool(strip), max_batch,
-               workspace_limit, sb, se, bool(gather))
+               workspace_limit, sb, se, bool(gather), gemm_kernel)
         ex = self._executors.get(key)
         if ex is None:
             o = _lib.ExecOptions(dtype_code, _lib.PREC[precision], int(batched), int(use_graph), int(autotune),
-                                 int(strip), max_batch, workspace_limit, sb, se, int(not gather))
+                                 int(strip), max_batch, workspace_limit, sb, se, int(not gather),
+                                 0 if gemm_kernel is None else int(gemm_kernel) + 1)
             ex = _Executor(self._h, o)
             self._executors[key] = ex
         return ex

@@ -66,6 +66,8 @@ static ExecOptions to_opts(const atn_exec_options* o) {
   e.slice_begin = o->slice_begin;
   e.slice_end = o->slice_end;
   e.gather = o->permute_b == 0;
+  if (o->gemm_kernel < 0 || o->gemm_kernel > 9) throw Error("unknown GEMM kernel");
+  e.gemm_kernel = o->gemm_kernel - 1;
   return e;
 }
 
@@ -304,7 +306,7 @@ int atn_gemm(int dtype, int precision, int kernel, size_t batch, size_t m, size_
     g.lda = g.k, g.ldb = g.n, g.ldc = g.n;
     g.nbo = (int64_t)batch, g.nbi = 1;
     g.sao = (int64_t)stride_a, g.sbo = (int64_t)stride_b, g.sco = (int64_t)stride_c;
-    if (kernel < -1 || kernel > 7) throw Error("unknown GEMM kernel");
+    if (kernel < -1 || kernel > 8) throw Error("unknown GEMM kernel");
     if (precision < 0 || precision > 3) throw Error("unsupported precision code");
     hipStream_t s = static_cast<hipStream_t>(stream);
     void* scr = nullptr;

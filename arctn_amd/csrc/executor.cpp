@@ -184,7 +184,8 @@ bool Executor::add_gather(const std::vector<size_t>& shape, const std::vector<si
 }
 
 void Executor::gen_gemm(const PairStep& st, size_t nbo, bool va, bool vb, const Ref& a, const Ref& b, const Ref& c,
-                        const Ref* cmax, int64_t brow, int64_t bcol, int32_t bcol_run) {
+                        const Ref* cmax, int64_t brow, int64_t bcol, int32_t bcol_run, const Ref* amax,
+                        const Ref* bmax) {
   GemmArgs g{};
   g.m = (int64_t)st.m, g.k = (int64_t)st.k, g.n = (int64_t)st.n;
   g.lda = g.k, g.ldb = g.n, g.ldc = g.n;
@@ -201,9 +202,12 @@ void Executor::gen_gemm(const PairStep& st, size_t nbo, bool va, bool vb, const 
     g.sai = g.m * g.k, g.sci = g.m * g.n;
   }
   int idx = (int)gemm_choice_.size();
-  gemm_choice_.push_back(GEMM_AUTO);
+  gemm_choice_.push_back(o_.gemm_kernel);
   g.bcol_run = bcol_run;
-  gemm_sites_.push_back(GemmSite{g, a, b, c, brow, bcol, bcol_run});
+  const Ref am = amax ? *amax : Ref{}, bm = bmax ? *bmax : Ref{};
+  if (amax) use(am);
+  if (bmax) use(bm);
+  gemm_sites_.push_back(GemmSite{g, a, b, c, brow, bcol, bcol_run, am, bm});
   int dt = o_.dtype, prec = o_.precision;
   use(a);
   use(b);
@@ -215,7 +219,7 @@ void Executor::gen_gemm(const PairStep& st, size_t nbo, bool va, bool vb, const 
   Ref scr{Ref::WS, scratch_buf_};
   size_t& scr_bytes = bufs_[scratch_buf_].bytes;
   for (int kern : {GEMM_AUTO, GEMM_SIMPLE, GEMM_SKINNY, GEMM_MFMA, GEMM_SPLITK, GEMM_PRESPLIT, GEMM_MFMA_R32,
-                   GEMM_MFMA_R64, GEMM_MFMA_R128})  // autotune candidates
+                   GEMM_MFMA_R64, GEMM_MFMA_R128, GEMM_PRESPLIT_A})  // autotune candidates
     scr_bytes = std::max(scr_bytes, gemm_scratch_bytes(dt, prec, kern, g));
   const int sb = scratch_buf_;
   const Ref mref = cmax ? *cmax : Ref{};
@@ -227,6 +231,8 @@ void Executor::gen_gemm(const PairStep& st, size_t nbo, bool va, bool vb, const 
         q.scratch = cx.ptr(scr);
         q.scratch_bytes = bufs_[sb].bytes;
         if (mref.kind != Ref::NONE) q.cmax = (unsigned long long*)cx.ptr(mref);
+        if (am.kind != Ref::NONE) q.amax = (const unsigned long long*)cx.ptr(am);
+        if (bm.kind != Ref::NONE) q.bmax = (const unsigned long long*)cx.ptr(bm);
         if (bcol >= 0) q.brow = consts_ + brow, q.bcol = consts_ + bcol;
         launch_gemm(dt, prec, gemm_choice_[idx], q, cx.s);
       },
@@ -293,13 +299,17 @@ void Executor::gen_steps(bool body, size_t nb, std::vector<Ref>& slots, std::vec
     }
     if (bcol < 0) b = chain(slots[st.ri], shapes[st.ri], st.sum_b, st.b_perm, nbB);
     Ref c{Ref::WS, newbuf(nbC * st.bt * st.m * st.n * es)};
+    // operand max magnitudes folded by the producing GEMMs (a permute or a gather
+    // keeps the max; a summed leg does not)
+    const Ref* am = st.sum_a.empty() && slot_max_[st.li].kind != Ref::NONE ? &slot_max_[st.li] : nullptr;
+    const Ref* bm = st.sum_b.empty() && slot_max_[st.ri].kind != Ref::NONE ? &slot_max_[st.ri] : nullptr;
     if (o_.strip) {
       // the GEMM folds the max of its result into mb; the rescale then runs only
       // when that max leaves [2^-STRIP_KEEP, 2^STRIP_KEEP] (always after the last step)
       Ref mb{Ref::WS, maxbits_buf_};
       use(mb);
       emit([=](const Ctx& cx) { launch_fill_zero(cx.ptr(mb), 8, cx.s); });
-      gen_gemm(st, nbC, va, vb, a, b, c, &mb, brow, bcol, bcol_run);
+      gen_gemm(st, nbC, va, vb, a, b, c, &mb, brow, bcol, bcol_run, am, bm);
       size_t numel = nbC * st.bt * st.m * st.n;
       Ref ex{Ref::WS, expo_buf_};
       const int keep = si + 1 == P.cp.steps.size() ? 0 : STRIP_KEEP;
@@ -307,8 +317,14 @@ void Executor::gen_steps(bool body, size_t nb, std::vector<Ref>& slots, std::vec
       emit([=](const Ctx& cx) {
         launch_scale_pow2(dt, cx.ptr(c), numel, (const uint64_t*)cx.ptr(mb), (int64_t*)cx.ptr(ex), cx.s, keep);
       });
+    } else if (want_max_[st.out]) {
+      Ref mc{Ref::WS, newbuf(8)};
+      use(mc);
+      emit([=](const Ctx& cx) { launch_fill_zero(cx.ptr(mc), 8, cx.s); });
+      gen_gemm(st, nbC, va, vb, a, b, c, &mc, brow, bcol, bcol_run, am, bm);
+      slot_max_[st.out] = mc;
     } else {
-      gen_gemm(st, nbC, va, vb, a, b, c, nullptr, brow, bcol, bcol_run);
+      gen_gemm(st, nbC, va, vb, a, b, c, nullptr, brow, bcol, bcol_run, am, bm);
     }
     slots[st.out] = c;
     shapes[st.out] = st.result_shape;
@@ -383,6 +399,14 @@ void Executor::build(size_t B) {
   std::vector<Ref> slots(nslot);
   std::vector<std::vector<size_t>> shapes(nslot);
   ScatterArgs sc = make_scatter(P);
+  slot_max_.assign(nslot, Ref{});
+  want_max_.assign(nslot, 0);
+  if (o_.precision == FP16X2 && !o_.strip)
+    for (const PairStep& st : P.cp.steps)
+      if (split_products_apply(dt, o_.precision, (int64_t)st.m, (int64_t)st.k, (int64_t)st.n)) {
+        if (st.sum_a.empty()) want_max_[st.li] = 1;
+        if (st.sum_b.empty()) want_max_[st.ri] = 1;
+      }
   if (o_.batched) {
     gen_inputs(false, 1, slots, shapes);
     gen_steps(false, 1, slots, shapes);
@@ -592,7 +616,7 @@ void Executor::profile(const void* const* inputs, void* out, hipStream_t s, std:
   c.inputs = inputs;
   c.out = out;
   c.s = s;
-  if (o_.autotune && !tuned_) autotune(c, s);
+  if (o_.autotune && o_.gemm_kernel == GEMM_AUTO && !tuned_) autotune(c, s);
   struct Rec {
     hipEvent_t a, b;
     int kind;
@@ -659,21 +683,24 @@ void Executor::autotune(const Ctx& c, hipStream_t s) {
     q.a = c.ptr(site.a), q.b = c.ptr(site.b), q.c = c.ptr(site.c);
     q.scratch = c.ws + offs_[scratch_buf_];
     if (site.bcol >= 0) q.brow = consts_ + site.brow, q.bcol = consts_ + site.bcol;
+    if (site.amax.kind != Ref::NONE) q.amax = (const unsigned long long*)c.ptr(site.amax);
+    if (site.bmax.kind != Ref::NONE) q.bmax = (const unsigned long long*)c.ptr(site.bmax);
     q.scratch_bytes = bufs_[scratch_buf_].bytes;
     float best_ms = 1e30f;
     int choice = GEMM_AUTO;
     const bool split_prec = o_.precision != NATIVE;
     const double macs = (double)q.nbo * (double)q.nbi * (double)q.m * (double)q.n * (double)q.k;
     const double outs = (double)q.nbo * (double)q.nbi * (double)q.m * (double)q.n;
-    for (int cand : {GEMM_SIMPLE, GEMM_SKINNY, GEMM_SPLITK, GEMM_PRESPLIT, GEMM_MFMA, GEMM_MFMA_R32, GEMM_MFMA_R64,
-                     GEMM_MFMA_R128}) {
+    for (int cand : {GEMM_SIMPLE, GEMM_SKINNY, GEMM_SPLITK, GEMM_PRESPLIT, GEMM_PRESPLIT_A, GEMM_MFMA, GEMM_MFMA_R32,
+                     GEMM_MFMA_R64, GEMM_MFMA_R128}) {
       // candidates that cannot win are not timed (the one-thread-per-output kernel
       // on large products, split-K with many outputs, row tiles far from m)
       if (cand == GEMM_SIMPLE && macs > (double)(1 << 26)) continue;
       if (cand == GEMM_SKINNY && q.m > 64 && q.n > 32) continue;
       if (cand == GEMM_SPLITK && (q.k < 256 || outs > (double)(1 << 16))) continue;
       // split products (GEMM_MFMA splits while staging) only where they can pay off
-      if ((cand == GEMM_PRESPLIT || cand == GEMM_MFMA) && (!split_prec || q.m < 64)) continue;
+      if ((cand == GEMM_PRESPLIT || cand == GEMM_PRESPLIT_A || cand == GEMM_MFMA) && (!split_prec || q.m < 64))
+        continue;
       if (cand == GEMM_MFMA_R32 && q.m > 64) continue;
       if (cand == GEMM_MFMA_R64 && q.m > 256) continue;
       if (cand == GEMM_MFMA_R128 && q.m < 64) continue;
@@ -702,7 +729,7 @@ void Executor::run(const void* const* inputs, void* out, hipStream_t s, Report* 
   c.inputs = inputs;
   c.out = out;
   c.s = s;
-  if (o_.autotune && !tuned_) autotune(c, s);
+  if (o_.autotune && o_.gemm_kernel == GEMM_AUTO && !tuned_) autotune(c, s);
   hipEvent_t e0 = nullptr, e1 = nullptr;
   if (rep) {
     ATN_HIP(hipEventCreate(&e0));

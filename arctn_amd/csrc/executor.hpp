@@ -23,6 +23,8 @@ struct ExecOptions {
   size_t slice_begin = 0, slice_end = 0;
   // read B operands through gather tables instead of permuting them (where possible)
   bool gather = true;
+  // GEMM kernel of every step (GemmKernel); GEMM_AUTO = chosen per step
+  int gemm_kernel = -1;
 };
 
 struct Report {
@@ -92,10 +94,13 @@ class Executor {
   void gen_steps(bool body, size_t nb, std::vector<Ref>& slots, std::vector<std::vector<size_t>>& shapes);
   void gen_inputs(bool body, size_t nb, std::vector<Ref>& slots, std::vector<std::vector<size_t>>& shapes);
   Ref gen_finalize(size_t nb, const Ref& src, bool to_output);
-  // cmax: optional zeroed slot receiving the result's max magnitude (strip);
-  // brow / bcol: gather tables of B in consts_ (-1: B is a row-major [k, n] operand)
+  // cmax: optional zeroed slot receiving the result's max magnitude (strip, or
+  // the FP16X2 scale of a later product); brow / bcol: gather tables of B in
+  // consts_ (-1: B is a row-major [k, n] operand); amax / bmax: slots holding the
+  // operands' max magnitudes (FP16X2), or null
   void gen_gemm(const PairStep& st, size_t nb_outer, bool va, bool vb, const Ref& a, const Ref& b, const Ref& c,
-                const Ref* cmax = nullptr, int64_t brow = -1, int64_t bcol = -1, int32_t bcol_run = 1);
+                const Ref* cmax = nullptr, int64_t brow = -1, int64_t bcol = -1, int32_t bcol_run = 1,
+                const Ref* amax = nullptr, const Ref* bmax = nullptr);
   void run_ops(const std::vector<Op>& ops, const Ctx& c);
   void enqueue(const Ctx& c, const std::function<void(const Op&, const Ctx&)>& visit = nullptr);
   void autotune(const Ctx& c, hipStream_t s);
@@ -105,7 +110,13 @@ class Executor {
     Ref a, b, c;
     int64_t brow = -1, bcol = -1;  // gather tables of B in consts_ (element offsets), or -1
     int32_t bcol_run = 1;
+    Ref amax, bmax;  // operand max magnitudes (FP16X2), or NONE
   };
+  // FP16X2: step results whose max magnitude a later split product reads as its
+  // operand scale (folded into the producing GEMM instead of a separate pass),
+  // and the slot holding it per plan slot
+  std::vector<char> want_max_;
+  std::vector<Ref> slot_max_;
   // Gather tables (int32 element offsets) of B operands read without a permute.
   std::vector<int32_t> const_host_;
   int32_t* consts_ = nullptr;

@@ -504,7 +504,6 @@ __global__ __launch_bounds__(256) void gemm_mfma_kernel(GemmArgs g) {
   const int64_t M = g.m, N = CPLX ? 2 * g.n : g.n, K = CPLX ? 2 * g.k : g.k;
   const int64_t lda = CPLX ? 2 * g.lda : g.lda;  // real units
   const int64_t ldc = CPLX ? 2 * g.ldc : g.ldc;
-  const int64_t ldb = g.ldb;                      // complex units when CPLX
   const int tiles_n = (int)((N + BN - 1) / BN);
   const int tiles_m = (int)((M + BM - 1) / BM);
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -690,12 +689,11 @@ __global__ __launch_bounds__(256, 2) void gemm_split_kernel(GemmArgs g) {
   float sa = 1.f, sb = 1.f, unscale = 1.f;
   float mx = 0.f;
   if constexpr (F16) {
-    const unsigned long long* mb = (const unsigned long long*)g.scratch;
-    int ea = pow2_shift(mb[0]), eb = pow2_shift(mb[1]);
+    int ea = pow2_shift(*g.amax), eb = pow2_shift(*g.bmax);
     sa = ldexpf(1.f, ea), sb = ldexpf(1.f, eb), unscale = ldexpf(1.f, -(ea + eb));
   }
   const int64_t M = g.m, N = CPLX ? 2 * g.n : g.n, K = CPLX ? 2 * g.k : g.k;
-  const int64_t lda = CPLX ? 2 * g.lda : g.lda, ldc = CPLX ? 2 * g.ldc : g.ldc, ldb = g.ldb;
+  const int64_t lda = CPLX ? 2 * g.lda : g.lda, ldc = CPLX ? 2 * g.ldc : g.ldc;
   const int tiles_n = (int)((N + BN - 1) / BN), tiles_m = (int)((M + BM - 1) / BM);
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm0 = (wave >> 1) * WM, wn0 = (wave & 1) * WN;
@@ -849,10 +847,10 @@ __global__ __launch_bounds__(256, 2) void gemm_split_kernel(GemmArgs g) {
 }
 
 constexpr size_t SPLIT_LDS = 4 * 128 * (64 + 8) * 2;
-// Measured on gfx950 (complex64 1024x512x8192 x64, BF16x3): BK 64 / one buffer
-// 302 TFLOP/s algorithmic, BK 32 / two buffers 277.
-constexpr int PRESPLIT_BK = 64, PRESPLIT_NBUF = 1;
-constexpr size_t PRESPLIT_LDS = PRESPLIT_NBUF * 4 * 128 * PRESPLIT_BK * 2;
+// Measured on gfx950 (complex64 1024x512x8192 x64, BF16x3): BK 64 with one LDS
+// buffer (two workgroups per CU) 302 TFLOP/s algorithmic, BK 32 with two 277.
+constexpr int PRESPLIT_BK = 64;
+constexpr size_t PRESPLIT_LDS = 4 * 128 * PRESPLIT_BK * 2;
 
 // ------------------------------------------------------------------ dispatch --
 // ------------------------------------------------ pre-split operand GEMM --
@@ -881,7 +879,7 @@ __global__ __launch_bounds__(256) void split_a_kernel(GemmArgs g, PreSplit ps) {
   const int64_t inner = g.sai ? g.nbi : 1, nm = n_mats(g.sao, g.sai, g.nbo, g.nbi);
   const int64_t per_row = kp / 4, total = nm * g.m * per_row;
   float sa = 1.f;
-  if constexpr (F16) sa = ldexpf(1.f, pow2_shift(((const unsigned long long*)g.scratch)[0]));
+  if constexpr (F16) sa = ldexpf(1.f, pow2_shift(*g.amax));
   for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
     const int64_t mat = e / (g.m * per_row), r = e - mat * g.m * per_row, row = r / per_row, c = (r - row * per_row) * 4;
     const int64_t zo = mat / inner, zi = mat - zo * inner;
@@ -908,7 +906,7 @@ __global__ __launch_bounds__(256) void split_bt_kernel(GemmArgs g, PreSplit ps) 
   const int64_t kp = ps.kp, inner = g.sbi ? g.nbi : 1, nm = n_mats(g.sbo, g.sbi, g.nbo, g.nbi);
   const int64_t tk = (kp / EH + TK - 1) / TK, tn = (g.n + TN - 1) / TN, total = nm * tk * tn;
   float sb = 1.f;
-  if constexpr (F16) sb = ldexpf(1.f, pow2_shift(((const unsigned long long*)g.scratch)[1]));
+  if constexpr (F16) sb = ldexpf(1.f, pow2_shift(*g.bmax));
   const int t = threadIdx.x;
   for (int64_t b = blockIdx.x; b < total; b += gridDim.x) {
     const int64_t mat = b / (tk * tn), r = b - mat * tk * tn, k0 = (r / tn) * TK, n0 = (r - (r / tn) * tn) * TN;
@@ -963,15 +961,16 @@ __device__ inline uint4 neg_im(uint4 v) { return make_uint4(v.x ^ 0x80000000u, v
 __device__ inline uint32_t swap16(uint32_t x) { return (x >> 16) | (x << 16); }
 __device__ inline uint4 swap_pairs(uint4 v) { return make_uint4(swap16(v.x), swap16(v.y), swap16(v.z), swap16(v.w)); }
 
-// LDS tiles are 128 rows of BK halves (no padding), NBUF buffers. The
+// LDS tiles are 128 rows of BK halves (no padding), one buffer (64 KiB, two
+// workgroups per CU); the next k tile waits in registers for a barrier. The
 // 16-byte chunk c of row r sits at chunk c ^ f(r) (f(r) = (r >> 1) & 7 for 8
 // chunks per row, (r >> 2) & 3 for 4): the 16 rows of one MFMA fragment read
-// and the rows of one 16-lane store each cover all 64 banks once. With two
-// buffers the next tile is stored into the other buffer while the waves still
-// compute on this one (one barrier per k tile); with one buffer (half the LDS,
-// two workgroups per CU) the store waits for a barrier.
+// and the rows of one 16-lane store each cover all 64 banks once.
 // Complex B: LDS row r (< 64) holds (re, -im) of compact row r and produces
 // output column 2r (real part); row 64 + r holds (im, re) and produces 2r + 1.
+// BRAW: B is not pre-split; the FP32 (or complex64) B is read through the gather
+// tables and split while staging, so a B element costs one HBM read per 128-row
+// tile of A instead of a separate read-split-write pass plus the 16-bit reads.
 template <int BK>
 __device__ inline int swz(int row, int ch) {
   constexpr int CH = BK / 8;  // 16-byte chunks per row
@@ -979,7 +978,7 @@ __device__ inline int swz(int row, int ch) {
   return row * BK + (((ch >> 3) ^ f) << 3);
 }
 
-template <bool CPLX, bool F16, int BK, int NBUF>
+template <bool CPLX, bool F16, int BK, bool BRAW>
 __global__ __launch_bounds__(256, 2) void gemm_presplit_kernel(GemmArgs g, PreSplit ps) {
   using H = typename Half<F16>::H;
   using V = typename Half<F16>::V;
@@ -989,14 +988,22 @@ __global__ __launch_bounds__(256, 2) void gemm_presplit_kernel(GemmArgs g, PreSp
   constexpr int A_Q = BM * CH / 256;                    // 16-byte A chunks per thread
   constexpr int B_Q = (CPLX ? BN / 2 : BN) * CH / 256;  // 16-byte B chunks per thread
   constexpr int TILE = 128 * BK;                        // halves per LDS array
+  // raw B: chunk q of a k tile is (tile column q % BC, k group q / BC), so the
+  // lanes of a wave read consecutive columns of one B row; a chunk is BE rows
+  constexpr int BC = CPLX ? BN / 2 : BN, BE = CPLX ? 4 : 8;
+  static_assert(256 % BC == 0, "a thread's raw B loads share one column");
+  using BT = typename std::conditional<CPLX, float2, float>::type;
   extern __shared__ __align__(16) char smem_split[];
-  H* lds = reinterpret_cast<H*>(smem_split);  // [NBUF buffers][ahi, alo, bhi, blo][TILE]
+  H* const ahi = reinterpret_cast<H*>(smem_split);  // [ahi, alo, bhi, blo][TILE]
+  H* const alo = ahi + TILE;
+  H* const bhi = alo + TILE;
+  H* const blo = bhi + TILE;
 
-  float unscale = 1.f;
+  float unscale = 1.f, sb = 1.f;
   float mx = 0.f;
   if constexpr (F16) {
-    const unsigned long long* mb = (const unsigned long long*)g.scratch;
-    unscale = ldexpf(1.f, -(pow2_shift(mb[0]) + pow2_shift(mb[1])));
+    unscale = ldexpf(1.f, -(pow2_shift(*g.amax) + pow2_shift(*g.bmax)));
+    sb = ldexpf(1.f, pow2_shift(*g.bmax));
   }
   const int64_t M = g.m, N = CPLX ? 2 * g.n : g.n, kp = ps.kp;
   const int64_t ldc = CPLX ? 2 * g.ldc : g.ldc;
@@ -1012,8 +1019,9 @@ __global__ __launch_bounds__(256, 2) void gemm_presplit_kernel(GemmArgs g, PreSp
     const int64_t bmat = (g.sbo ? zo : 0) * binner + (g.sbi ? zi : 0);
     const H* Ah = (const H*)ps.ahi + amat * M * kp;
     const H* Al = (const H*)ps.alo + amat * M * kp;
-    const H* Bh = (const H*)ps.bhi + bmat * g.n * kp;
-    const H* Bl = (const H*)ps.blo + bmat * g.n * kp;
+    const H* Bh = BRAW ? nullptr : (const H*)ps.bhi + bmat * g.n * kp;
+    const H* Bl = BRAW ? nullptr : (const H*)ps.blo + bmat * g.n * kp;
+    const BT* Braw = (const BT*)g.b + zo * g.sbo + zi * g.sbi;
     float* C = (float*)g.c + (zo * g.sco + zi * g.sci) * (CPLX ? 2 : 1);
     for (uint64_t t = blockIdx.x; t < ntile; t += gridDim.x) {
       int64_t m0, n0;
@@ -1023,7 +1031,12 @@ __global__ __launch_bounds__(256, 2) void gemm_presplit_kernel(GemmArgs g, PreSp
       for (int i = 0; i < TM; ++i)
 #pragma unroll
         for (int j = 0; j < TN; ++j) acc[i][j] = v4f{0, 0, 0, 0};
-      uint4 rah[A_Q], ral[A_Q], rbh[B_Q], rbl[B_Q];
+      uint4 rah[A_Q], ral[A_Q], rbh[BRAW ? 1 : B_Q], rbl[BRAW ? 1 : B_Q];
+      BT rbr[BRAW ? B_Q * BE : 1];
+      // raw B: the thread's one column, its gather offset hoisted out of the k loop
+      const int64_t bcn = (CPLX ? n0 / 2 : n0) + tid % BC;
+      const bool bcn_ok = BRAW && bcn < g.n;
+      const int64_t bco = bcn_ok ? b_col(g, bcn) : 0;
       auto load = [&](int64_t k0) {
 #pragma unroll
         for (int u = 0; u < A_Q; ++u) {
@@ -1033,20 +1046,31 @@ __global__ __launch_bounds__(256, 2) void gemm_presplit_kernel(GemmArgs g, PreSp
           rah[u] = ok ? *reinterpret_cast<const uint4*>(Ah + gr * kp + gc) : make_uint4(0, 0, 0, 0);
           ral[u] = ok ? *reinterpret_cast<const uint4*>(Al + gr * kp + gc) : make_uint4(0, 0, 0, 0);
         }
+        if constexpr (BRAW) {
+          const BT* bp = Braw + bco;
 #pragma unroll
-        for (int u = 0; u < B_Q; ++u) {
-          const int q = tid + 256 * u, row = q / CH, ch = (q % CH) * 8;
-          const int64_t gr = (CPLX ? n0 / 2 : n0) + row, gc = k0 + ch;
-          const bool ok = gr < g.n && gc < kp;
-          rbh[u] = ok ? *reinterpret_cast<const uint4*>(Bh + gr * kp + gc) : make_uint4(0, 0, 0, 0);
-          rbl[u] = ok ? *reinterpret_cast<const uint4*>(Bl + gr * kp + gc) : make_uint4(0, 0, 0, 0);
+          for (int u = 0; u < B_Q; ++u) {
+            // first B row of the chunk (complex rows for complex B; k0 is even). BC is
+            // a multiple of the wave size, so the row is the same on every lane of a
+            // wave and its gather offset is one scalar load.
+            const int kc = __builtin_amdgcn_readfirstlane((tid + 256 * u) / BC);
+            const int64_t r0 = (k0 + kc * 8) / (CPLX ? 2 : 1);
+#pragma unroll
+            for (int e = 0; e < BE; ++e)
+              rbr[u * BE + e] = bcn_ok && r0 + e < g.k ? bp[b_row(g, r0 + e)] : BT{};
+          }
+        } else {
+#pragma unroll
+          for (int u = 0; u < B_Q; ++u) {
+            const int q = tid + 256 * u, row = q / CH, ch = (q % CH) * 8;
+            const int64_t gr = (CPLX ? n0 / 2 : n0) + row, gc = k0 + ch;
+            const bool ok = gr < g.n && gc < kp;
+            rbh[u] = ok ? *reinterpret_cast<const uint4*>(Bh + gr * kp + gc) : make_uint4(0, 0, 0, 0);
+            rbl[u] = ok ? *reinterpret_cast<const uint4*>(Bl + gr * kp + gc) : make_uint4(0, 0, 0, 0);
+          }
         }
       };
-      auto store = [&](int buf) {
-        H* ahi = lds + buf * 4 * TILE;
-        H* alo = ahi + TILE;
-        H* bhi = alo + TILE;
-        H* blo = bhi + TILE;
+      auto store = [&]() {
 #pragma unroll
         for (int u = 0; u < A_Q; ++u) {
           const int q = tid + 256 * u, row = q / CH, ch = (q % CH) * 8;
@@ -1055,30 +1079,44 @@ __global__ __launch_bounds__(256, 2) void gemm_presplit_kernel(GemmArgs g, PreSp
         }
 #pragma unroll
         for (int u = 0; u < B_Q; ++u) {
-          const int q = tid + 256 * u, row = q / CH, ch = (q % CH) * 8;
-          if constexpr (CPLX) {
-            *reinterpret_cast<uint4*>(bhi + swz<BK>(row, ch)) = neg_im(rbh[u]);
-            *reinterpret_cast<uint4*>(blo + swz<BK>(row, ch)) = neg_im(rbl[u]);
-            *reinterpret_cast<uint4*>(bhi + swz<BK>(row + 64, ch)) = swap_pairs(rbh[u]);
-            *reinterpret_cast<uint4*>(blo + swz<BK>(row + 64, ch)) = swap_pairs(rbl[u]);
+          const int q = tid + 256 * u;
+          int row, ch;
+          uint4 vh, vl;
+          if constexpr (BRAW) {
+            row = q % BC, ch = (q / BC) * 8;
+            H h[8], l[8];
+#pragma unroll
+            for (int e = 0; e < BE; ++e) {
+              if constexpr (CPLX) {
+                split16<F16>(rbr[u * BE + e].x * sb, h[2 * e], l[2 * e]);
+                split16<F16>(rbr[u * BE + e].y * sb, h[2 * e + 1], l[2 * e + 1]);
+              } else {
+                split16<F16>(rbr[u * BE + e] * sb, h[e], l[e]);
+              }
+            }
+            vh = *reinterpret_cast<const uint4*>(h), vl = *reinterpret_cast<const uint4*>(l);
           } else {
-            *reinterpret_cast<uint4*>(bhi + swz<BK>(row, ch)) = rbh[u];
-            *reinterpret_cast<uint4*>(blo + swz<BK>(row, ch)) = rbl[u];
+            row = q / CH, ch = (q % CH) * 8;
+            vh = rbh[u], vl = rbl[u];
+          }
+          if constexpr (CPLX) {
+            *reinterpret_cast<uint4*>(bhi + swz<BK>(row, ch)) = neg_im(vh);
+            *reinterpret_cast<uint4*>(blo + swz<BK>(row, ch)) = neg_im(vl);
+            *reinterpret_cast<uint4*>(bhi + swz<BK>(row + 64, ch)) = swap_pairs(vh);
+            *reinterpret_cast<uint4*>(blo + swz<BK>(row + 64, ch)) = swap_pairs(vl);
+          } else {
+            *reinterpret_cast<uint4*>(bhi + swz<BK>(row, ch)) = vh;
+            *reinterpret_cast<uint4*>(blo + swz<BK>(row, ch)) = vl;
           }
         }
       };
       const int64_t nk = (kp + BK - 1) / BK;
       const int cl = lane & 15;
       load(0);
-      store(0);
+      store();
       __syncthreads();
       for (int64_t kt = 0; kt < nk; ++kt) {
-        const int buf = NBUF == 2 ? (int)(kt & 1) : 0;
         if (kt + 1 < nk) load((kt + 1) * BK);
-        const H* ahi = lds + buf * 4 * TILE;
-        const H* alo = ahi + TILE;
-        const H* bhi = alo + TILE;
-        const H* blo = bhi + TILE;
 #pragma unroll
         for (int ks = 0; ks < BK; ks += 32) {
           const int kg = ks + (lane >> 4) * 8;
@@ -1109,15 +1147,10 @@ __global__ __launch_bounds__(256, 2) void gemm_presplit_kernel(GemmArgs g, PreSp
 #pragma unroll
             for (int j = 0; j < TN; ++j) acc[i][j] = Half<F16>::mfma(ah[i], bh[j], acc[i][j]);
         }
-        if constexpr (NBUF == 2) {
-          if (kt + 1 < nk) store(buf ^ 1);
+        __syncthreads();
+        if (kt + 1 < nk) {
+          store();
           __syncthreads();
-        } else {
-          __syncthreads();
-          if (kt + 1 < nk) {
-            store(0);
-            __syncthreads();
-          }
         }
       }
 #pragma unroll
@@ -1381,10 +1414,11 @@ static int native_mfma_kernel(int dt, int64_t m) {
 static int auto_kernel(int dt, int prec, const GemmArgs& g) {
   const bool split_prec = (prec == BF16X3 || prec == FP16X2) && (dt == F32 || dt == C64);
   if (split_prec && split_worth(dt, g)) {
-    // Splitting inside the kernel repeats B's conversion once per 128-row tile
-    // of A (and A's once per column tile and batch entry): with three or more
-    // row tiles one separate splitting pass is cheaper.
-    return (g.m + 127) / 128 >= 3 ? GEMM_PRESPLIT : GEMM_MFMA;
+    // A (the small operand of these steps) split once by a pass; B read in its
+    // own precision and split while staging: measured on gfx950 at least as fast
+    // as splitting both operands by passes (up to 1.6x on short-k steps, where
+    // the passes' extra read and write of B dominate) or both while staging.
+    return GEMM_PRESPLIT_A;
   }
   if (mfma_worth(dt, g)) return native_mfma_kernel(dt, g.m);
   if (splitk_worth(g)) return GEMM_SPLITK;
@@ -1412,8 +1446,10 @@ int launch_gemm(int dt, int prec, int kernel, const GemmArgs& g, hipStream_t s) 
   return used < 0 ? GEMM_SKINNY : used;
 }
 
-static int launch_gemm_kernels(int dt, int prec, int kernel, const GemmArgs& g, hipStream_t s) {
+static int launch_gemm_kernels(int dt, int prec, int kernel, const GemmArgs& g0, hipStream_t s) {
+  GemmArgs g = g0;
   if (kernel == GEMM_AUTO) kernel = auto_kernel(dt, prec, g);
+  if (g.k == 0) kernel = GEMM_SIMPLE;  // empty sums: every kernel but this one assumes k tiles
   if (kernel == GEMM_SPLITK) {
     run_splitk(dt, g, s);
     return GEMM_SPLITK;
@@ -1430,24 +1466,28 @@ static int launch_gemm_kernels(int dt, int prec, int kernel, const GemmArgs& g, 
   }
   if (prec == C3M) throw Error("precision c3m is not available in this build");
   const bool split_prec = (prec == BF16X3 || prec == FP16X2) && (dt == F32 || dt == C64);
-  if (kernel == GEMM_PRESPLIT && !split_prec) kernel = GEMM_MFMA;
+  if ((kernel == GEMM_PRESPLIT || kernel == GEMM_PRESPLIT_A) && !split_prec) kernel = GEMM_MFMA;
   // split precision runs on GEMM_MFMA (split while staging) and GEMM_PRESPLIT;
   // the explicit row-tile kernels are the FP32 / FP64 MFMA kernels in any mode
-  if (split_prec && (kernel == GEMM_MFMA || kernel == GEMM_PRESPLIT)) {
+  if (split_prec && (kernel == GEMM_MFMA || kernel == GEMM_PRESPLIT || kernel == GEMM_PRESPLIT_A)) {
     const PreSplitSizes psz = presplit_sizes(dt, g);
-    const bool pre = kernel == GEMM_PRESPLIT && g.scratch && g.scratch_bytes >= 256 + psz.total;
+    const bool raw_b = kernel == GEMM_PRESPLIT_A;
+    const bool pre = kernel != GEMM_MFMA && g.scratch &&
+                     g.scratch_bytes >= 256 + (raw_b ? 2 * psz.a_half : psz.total);
     int64_t N = dt == C64 ? 2 * g.n : g.n;
     uint64_t tiles = (uint64_t)((g.m + 127) / 128) * (uint64_t)((N + 127) / 128);
     dim3 grid((unsigned)std::min<uint64_t>(tiles, 1u << 20), zgrid(g));
     if (g.cmax) grid.x = std::min<unsigned>(grid.x, std::max<unsigned>(1, TRACK_WORKGROUPS / grid.y));
     if (prec == FP16X2) {
       if (!g.scratch) throw Error("FP16X2 GEMM needs 16 bytes of device scratch");
-      // operand max magnitudes -> per-GEMM power-of-two scales
+      // operand max magnitudes -> per-GEMM power-of-two scales (measured here
+      // unless the caller supplies them)
       int64_t na = (g.sao ? g.nbo : 1) * (g.sai ? g.nbi : 1) * g.m * g.k;
       int64_t nb = (g.sbo ? g.nbo : 1) * (g.sbi ? g.nbi : 1) * g.k * g.n;
-      launch_fill_zero(g.scratch, 16, s);
-      launch_maxabs(dt, g.a, (size_t)na, (uint64_t*)g.scratch, s);
-      launch_maxabs(dt, g.b, (size_t)nb, (uint64_t*)g.scratch + 1, s);
+      unsigned long long* sc = (unsigned long long*)g.scratch;
+      if (!g.amax || !g.bmax) launch_fill_zero(g.scratch, 16, s);
+      if (!g.amax) launch_maxabs(dt, g.a, (size_t)na, (uint64_t*)sc, s), g.amax = sc;
+      if (!g.bmax) launch_maxabs(dt, g.b, (size_t)nb, (uint64_t*)sc + 1, s), g.bmax = sc + 1;
       g_last_kind = K_GEMM_MFMA_FP16X2;
     } else {
       g_last_kind = K_GEMM_MFMA_BF16X3;
@@ -1456,28 +1496,37 @@ static int launch_gemm_kernels(int dt, int prec, int kernel, const GemmArgs& g, 
     if (pre) {  // operands split once into 16-bit (hi, lo) arrays, then the copy-only GEMM
       PreSplit ps;
       char* base = (char*)g.scratch + 256;
-      ps.ahi = base, ps.alo = base + psz.a_half, ps.bhi = base + 2 * psz.a_half, ps.blo = base + 2 * psz.a_half + psz.b_half;
+      ps.ahi = base, ps.alo = base + psz.a_half;
+      ps.bhi = raw_b ? nullptr : base + 2 * psz.a_half, ps.blo = raw_b ? nullptr : base + 2 * psz.a_half + psz.b_half;
       ps.kp = psz.kp;
       const int64_t na = n_mats(g.sao, g.sai, g.nbo, g.nbi) * g.m * psz.kp / 4;
       const int64_t tiles_b = n_mats(g.sbo, g.sbi, g.nbo, g.nbi) * ((psz.kp / (cx ? 2 : 1) + 31) / 32) * ((g.n + 63) / 64);
       const dim3 ga((unsigned)std::min<int64_t>((na + 255) / 256, 65536)), gb((unsigned)std::min<int64_t>(tiles_b, 65536));
-#define PRESPLIT_LAUNCH(C_, F_)                                                                            \
+#define PRESPLIT_LAUNCH(C_, F_, R_)                                                                        \
   do {                                                                                                     \
     static bool attr_set = false;                                                                          \
     if (!attr_set) {                                                                                       \
-      ATN_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&gemm_presplit_kernel<C_, F_, PRESPLIT_BK, PRESPLIT_NBUF>), \
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)PRESPLIT_LDS));         \
+      ATN_HIP(hipFuncSetAttribute(                                                                         \
+          reinterpret_cast<const void*>(&gemm_presplit_kernel<C_, F_, PRESPLIT_BK, R_>),    \
+          hipFuncAttributeMaxDynamicSharedMemorySize, (int)PRESPLIT_LDS));                                 \
       attr_set = true;                                                                                     \
     }                                                                                                      \
     LAUNCH((split_a_kernel<C_, F_>), ga, dim3(256), 0, s, g, ps);                                         \
-    LAUNCH((split_bt_kernel<C_, F_>), gb, dim3(256), 0, s, g, ps);                                        \
+    if (!R_) LAUNCH((split_bt_kernel<C_, F_>), gb, dim3(256), 0, s, g, ps);                               \
     g_last_kind = F_ ? K_GEMM_MFMA_FP16X2 : K_GEMM_MFMA_BF16X3;                                           \
-    LAUNCH((gemm_presplit_kernel<C_, F_, PRESPLIT_BK, PRESPLIT_NBUF>), grid, dim3(256), PRESPLIT_LDS, s, g, ps);         \
+    LAUNCH((gemm_presplit_kernel<C_, F_, PRESPLIT_BK, R_>), grid, dim3(256), PRESPLIT_LDS, s, g, ps); \
   } while (0)
-      if (cx && f16) PRESPLIT_LAUNCH(true, true);
-      else if (cx) PRESPLIT_LAUNCH(true, false);
-      else if (f16) PRESPLIT_LAUNCH(false, true);
-      else PRESPLIT_LAUNCH(false, false);
+      if (raw_b) {
+        if (cx && f16) PRESPLIT_LAUNCH(true, true, true);
+        else if (cx) PRESPLIT_LAUNCH(true, false, true);
+        else if (f16) PRESPLIT_LAUNCH(false, true, true);
+        else PRESPLIT_LAUNCH(false, false, true);
+        return GEMM_PRESPLIT_A;
+      }
+      if (cx && f16) PRESPLIT_LAUNCH(true, true, false);
+      else if (cx) PRESPLIT_LAUNCH(true, false, false);
+      else if (f16) PRESPLIT_LAUNCH(false, true, false);
+      else PRESPLIT_LAUNCH(false, false, false);
 #undef PRESPLIT_LAUNCH
       return GEMM_PRESPLIT;
     }
@@ -1536,6 +1585,12 @@ static int launch_gemm_kernels(int dt, int prec, int kernel, const GemmArgs& g, 
   return GEMM_MFMA;
 }
 
+bool split_products_apply(int dt, int prec, int64_t m, int64_t k, int64_t n) {
+  GemmArgs g{};
+  g.m = m, g.k = k, g.n = n;
+  return (prec == BF16X3 || prec == FP16X2) && (dt == F32 || dt == C64) && split_worth(dt, g);
+}
+
 size_t gemm_scratch_bytes(int dt, int prec, int kernel, const GemmArgs& g) {
   if (kernel == GEMM_AUTO) kernel = auto_kernel(dt, prec, g);
   KSplit ks;
@@ -1543,8 +1598,10 @@ size_t gemm_scratch_bytes(int dt, int prec, int kernel, const GemmArgs& g) {
   else if (kernel == GEMM_MFMA || kernel >= GEMM_MFMA_R32)
     ks = plan_mfma_split(dt, prec, g, MAX_PARTIAL_BYTES, mfma_rows_for(dt, kernel, g.m));
   size_t part = ks.n_split > 1 ? (size_t)ks.n_split * dtype_size(dt) * (size_t)(g.nbo * g.nbi * g.m * g.n) : 0;
-  if (kernel == GEMM_PRESPLIT && (prec == BF16X3 || prec == FP16X2) && (dt == F32 || dt == C64))
-    part = presplit_sizes(dt, g).total;
+  if ((prec == BF16X3 || prec == FP16X2) && (dt == F32 || dt == C64)) {
+    if (kernel == GEMM_PRESPLIT) part = presplit_sizes(dt, g).total;
+    if (kernel == GEMM_PRESPLIT_A) part = 2 * presplit_sizes(dt, g).a_half;
+  }
   return 256 + part;
 }
 

@@ -127,6 +127,11 @@ struct GemmArgs {
   // bcol holds runs of this many consecutive offsets, each starting at a multiple
   // of the run length (vector loads stay within a run)
   int32_t bcol_run = 1;
+  // FP16X2: max |re|, |im| of A / of B as double bits when already known (folded
+  // into the epilogue of the GEMM that produced the operand); null = measured by
+  // a pass before the GEMM. An upper bound is enough: it only sets the scale.
+  const unsigned long long* amax = nullptr;
+  const unsigned long long* bmax = nullptr;
 };
 // GEMM_PRESPLIT: split-precision MFMA GEMM whose operands are split into
 // 16-bit (hi, lo) arrays by a separate pass (GEMM_MFMA splits them while staging
@@ -135,14 +140,19 @@ struct GemmArgs {
 // whatever the precision mode (GEMM_MFMA picks the row tile from m, or splits
 // the operands while staging under split precision; FP64 tiles have at most
 // 64 rows).
+// GEMM_PRESPLIT_A: the GEMM_PRESPLIT kernel with only A split by a pass; B is
+// read in its own precision through the gather tables and split while staging.
 enum GemmKernel {
   GEMM_AUTO = -1, GEMM_SIMPLE = 0, GEMM_MFMA = 1, GEMM_SKINNY = 2, GEMM_SPLITK = 3, GEMM_PRESPLIT = 4,
-  GEMM_MFMA_R32 = 5, GEMM_MFMA_R64 = 6, GEMM_MFMA_R128 = 7
+  GEMM_MFMA_R32 = 5, GEMM_MFMA_R64 = 6, GEMM_MFMA_R128 = 7, GEMM_PRESPLIT_A = 8
 };
 // Returns the kernel used.
 int launch_gemm(int dt, int prec, int kernel, const GemmArgs& g, hipStream_t s);
 // Device scratch launch_gemm can use for this GEMM (FP16X2 scales, split-K partials).
 size_t gemm_scratch_bytes(int dt, int prec, int kernel, const GemmArgs& g);
+// Whether the heuristic choice for an m x k x n product (element units) runs
+// split-precision products under precision prec.
+bool split_products_apply(int dt, int prec, int64_t m, int64_t k, int64_t n);
 // Conjugating variants for the adjoint: opA/opB: 0 = N, 1 = conj-transpose (as stored).
 void launch_gemm_adjoint(int dt, int opa, int opb, const GemmArgs& g, hipStream_t s);
 

@@ -95,6 +95,8 @@ typedef struct {
   size_t slice_end;     /* return their partial sum (the share of one device); 0, 0 = all */
   int permute_b;        /* 1: always permute GEMM B operands into [k, n] order; 0: read them
                            through gather tables where the layout allows (no copy) */
+  int gemm_kernel;      /* 0: GEMM kernel chosen per step (heuristic, or timed with autotune);
+                           k + 1: atn_gemm kernel code k at every step (overrides autotune) */
 } atn_exec_options;
 
 typedef struct {

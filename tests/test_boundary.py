@@ -130,6 +130,9 @@ def test_executor_options_validated_before_device_work():
     o = _lib.ExecOptions(1, 2, 0, 0, 0, 0, 0, 0)  # split precision on float64
     rc = L.atn_executor_create(p.handle, C.byref(o), C.byref(h))
     assert rc != 0 and b"split precision" in L.atn_last_error()
+    o = _lib.ExecOptions(1, 0, 1, 0, 0, 0, 0, 0, 0, 0, 0, 10)  # GEMM kernel code 9 does not exist
+    rc = L.atn_executor_create(p.handle, C.byref(o), C.byref(h))
+    assert rc != 0 and b"GEMM kernel" in L.atn_last_error()
 
 
 def test_linear_and_ssa_paths_round_trip():

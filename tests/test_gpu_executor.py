@@ -268,3 +268,36 @@ def test_gather_tables_match_permuted_operands(cuda, dtype):
         for gather in (True, False):
             got, _ = plan.run(dev, 2, batched=True, use_graph=True, precision=prec, gather=gather)
             assert abs(complex(got.cpu()) - want) / abs(want) < 1e-5, (prec, gather)
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.complex64])
+def test_forced_gemm_kernels_with_gather_tables(cuda, dtype):
+    """Every GEMM kernel forced on every step, with B read through gather tables
+    and permuted, in each precision: random networks (dims up to 8, so most steps
+    leave partial tiles) and a sliced circuit against the complex128 oracle."""
+    import torch
+
+    code = {np.dtype(np.float32): 0, np.dtype(np.complex64): 2}[np.dtype(dtype)]
+    cases = []
+    for seed in range(6):
+        net, path = _random_case(seed, dims=(3, 5, 8))
+        cases.append((net, W.random_arrays(net, dtype, seed), path, sorted(net["size_dict"])[:1]))
+    if dtype == np.complex64:
+        net, arrays, _ = W.random_circuit_amplitude(5, 4, 10, 1)
+        path = P.greedy(net)
+        cases.append((net, arrays, path, P.find_slices_to_size(net, path, 2 ** 9)))
+    for net, arrays, path, sliced in cases:
+        want = O.contract(net, [a.astype(np.complex128) for a in arrays], path, "compiled")
+        scale = max(np.abs(want).max(), 1e-30)
+        plan = Plan(net["inputs"], net["output"], net["size_dict"], path, sliced)
+        dev = [torch.from_numpy(a).cuda() for a in arrays]
+        # BF16x3 drops products of 2^-16 relative size: forced on all ~300 steps of
+        # the circuit (the heuristic keeps it to large steps) its error adds up
+        for prec, kernels, tol in (("native", (0, 1, 2, 3, 5, 6, 7, 8), 1e-5),
+                                   ("bf16x3", (1, 4, 8), 3e-4), ("fp16x2", (1, 4, 8), 1e-5)):
+            for kern in kernels:
+                for gather in (True, False):
+                    got, _ = plan.run(dev, code, batched=True, precision=prec, gather=gather, gemm_kernel=kern)
+                    err = np.abs(got.cpu().numpy() - want).max() / scale
+                    assert err < tol, (len(net["inputs"]), prec, kern, gather, err)
+        plan.release()

@@ -91,15 +91,15 @@ PREC_CASES = [(np.float32, 0, 1e-6), (np.float64, 0, 1e-14), (np.complex64, 0, 1
 
 
 @pytest.mark.parametrize("case", range(len(PREC_CASES)))
-@pytest.mark.parametrize("kernel", [0, 1, 2, 3, 4, 5, 6, 7])
+@pytest.mark.parametrize("kernel", [0, 1, 2, 3, 4, 5, 6, 7, 8])
 def test_gemm_against_numpy(cuda, case, kernel):
     """Simple (0), MFMA (1, native or split precision), skinny (2), split-K (3)
-    and pre-split MFMA (4) GEMMs vs complex128 NumPy."""
+    pre-split MFMA (4) and A-pre-split MFMA (8) GEMMs vs complex128 NumPy."""
     import torch
     from arctn_amd import _lib
 
     dtype, prec, eps = PREC_CASES[case]
-    if prec and kernel not in (1, 4):
+    if prec and kernel not in (1, 4, 8):
         pytest.skip("split precision applies to the MFMA kernels")
     if kernel == 7 and dtype in (np.float64, np.complex128):
         pytest.skip("FP64 MFMA tiles have at most 64 rows (7 runs as 6)")
@@ -131,7 +131,7 @@ def test_gemm_broadcast_batch(cuda, dtype):
     a = _rand((m, k), dtype, rng)
     b = _rand((bt, k, n), dtype, rng)
     want = np.matmul(a.astype(np.complex128)[None], b.astype(np.complex128))
-    for kernel, prec in ((0, 0), (1, 0), (3, 0), (1, 2), (1, 3), (4, 2), (4, 3)):
+    for kernel, prec in ((0, 0), (1, 0), (3, 0), (1, 2), (1, 3), (4, 2), (4, 3), (8, 2), (8, 3)):
         ta, tb = torch.from_numpy(a).cuda(), torch.from_numpy(b).cuda()
         tc = torch.empty((bt, m, n), dtype=ta.dtype, device="cuda")
         _lib.check(_lib.lib().atn_gemm(code, prec, kernel, bt, m, k, n, ta.data_ptr(), 0, tb.data_ptr(), k * n,
