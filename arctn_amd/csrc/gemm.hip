@@ -1414,11 +1414,13 @@ static int native_mfma_kernel(int dt, int64_t m) {
 static int auto_kernel(int dt, int prec, const GemmArgs& g) {
   const bool split_prec = (prec == BF16X3 || prec == FP16X2) && (dt == F32 || dt == C64);
   if (split_prec && split_worth(dt, g)) {
-    // A (the small operand of these steps) split once by a pass; B read in its
-    // own precision and split while staging: measured on gfx950 at least as fast
-    // as splitting both operands by passes (up to 1.6x on short-k steps, where
-    // the passes' extra read and write of B dominate) or both while staging.
-    return GEMM_PRESPLIT_A;
+    // One or two 128-row tiles of A: A split by a pass, B read in its own
+    // precision and split while staging (measured on gfx950: up to 1.6x faster
+    // than splitting B by a pass on short-k steps, where that pass's extra read
+    // and write of B dominate). With more row tiles every tile re-reads and
+    // re-splits B, and on operands read through gather tables (short runs of
+    // consecutive columns) one splitting pass over B is faster.
+    return (g.m + 127) / 128 <= 2 ? GEMM_PRESPLIT_A : GEMM_PRESPLIT;
   }
   if (mfma_worth(dt, g)) return native_mfma_kernel(dt, g.m);
   if (splitk_worth(g)) return GEMM_SPLITK;
