@@ -54,6 +54,9 @@ def main():
     ap.add_argument("--skip-ref", action="store_true",
                     help="skip the complex128 reference contraction (rel_err is then null; for profiler runs)")
     ap.add_argument("--cpu-slices", type=int, default=0, help="CPU sample size (0 = three per core)")
+    ap.add_argument("--library-gemm", action="store_true",
+                    help="only time the ROCm BLAS libraries' 16-bit batched GEMM (torch.bmm) on the real product "
+                         "shapes of the circuit's split-precision steps, as a reference for those kernels")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a device copy of 2 GiB (for scaling HBM traffic counters), then exit")
     ap.add_argument("--subset-slices", type=int, default=16,
@@ -84,6 +87,8 @@ def main():
         log("[bench] copied 2 GiB (read 2 GiB, write 2 GiB)")
         return
 
+    if args.library_gemm:
+        return library_gemm()
     if args.workload.startswith("uneven_"):
         return bench_grouped(args)
     if args.workload == "circuit_6x6_c12_o6_t26":
@@ -256,6 +261,34 @@ def main():
 
 # kernels that make up one launch of a split-precision GEMM kind
 SPLIT_KERNELS = ("gemm_presplit_kernel", "split_a_kernel", "split_bt_kernel", "maxabs_kernel", "gemm_split_kernel")
+
+
+def library_gemm():
+    """torch.bmm (ROCm BLAS) in fp16 / bf16, 2 warm-up + 5 timed calls per shape; one JSON line."""
+    import torch
+
+    res = []
+    for b, m, k, n in ((64, 1024, 1024, 16384), (64, 1024, 3072, 16384), (1, 8192, 8192, 8192),
+                       (64, 512, 256, 65536)):
+        for dt in (torch.float16, torch.bfloat16):
+            a = torch.randn(b, m, k, device="cuda", dtype=dt)
+            bb = torch.randn(b, k, n, device="cuda", dtype=dt)
+            for _ in range(2):
+                c = torch.bmm(a, bb)
+            torch.cuda.synchronize()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            for _ in range(5):
+                c = torch.bmm(a, bb)
+            e1.record()
+            torch.cuda.synchronize()
+            ms = e0.elapsed_time(e1) / 5
+            res.append({"shape": [b, m, k, n], "dtype": str(dt).split(".")[-1], "ms": ms,
+                        "tflops": 2.0 * b * m * k * n / ms / 1e9})
+            log(f"[bench] bmm {res[-1]['dtype']} {b}x{m}x{k}x{n}: {ms:.3f} ms {res[-1]['tflops']:.1f} TFLOP/s")
+            del a, bb, c
+            torch.cuda.empty_cache()
+    print(json.dumps({"library_gemm": res}), flush=True)
 
 
 def traffic_from(fetch_csv, write_csv, copy_fetch_csv, copy_write_csv, last=0):
