@@ -142,7 +142,7 @@ void Executor::gen_inputs(bool body, size_t nb, std::vector<Ref>& slots, std::ve
 }
 
 bool Executor::add_gather(const std::vector<size_t>& shape, const std::vector<size_t>& perm, size_t n_con, int64_t& row,
-                          int64_t& col, int32_t& col_run) {
+                          int64_t& col, int32_t& col_run, int32_t& row_run) {
   const size_t r = shape.size();
   std::vector<int64_t> st(r, 1);
   for (size_t a = r; a-- > 1;) st[a - 1] = st[a] * (int64_t)shape[a];
@@ -176,6 +176,12 @@ bool Executor::add_gather(const std::vector<size_t>& shape, const std::vector<si
     if (perm[d] != a) break;
     col_run *= (int32_t)shape[a];
   }
+  // the same for the innermost contracted axes (rows): runs of consecutive row offsets
+  row_run = 1;
+  for (size_t d = n_con, a = r; d-- > 0 && a-- > 0;) {
+    if (perm[d] != a) break;
+    row_run *= (int32_t)shape[a];
+  }
   row = (int64_t)const_host_.size();
   const_host_.insert(const_host_.end(), rows.begin(), rows.end());
   col = (int64_t)const_host_.size();
@@ -185,7 +191,7 @@ bool Executor::add_gather(const std::vector<size_t>& shape, const std::vector<si
 
 void Executor::gen_gemm(const PairStep& st, size_t nbo, bool va, bool vb, const Ref& a, const Ref& b, const Ref& c,
                         const Ref* cmax, int64_t brow, int64_t bcol, int32_t bcol_run, const Ref* amax,
-                        const Ref* bmax) {
+                        const Ref* bmax, int32_t brow_run) {
   GemmArgs g{};
   g.m = (int64_t)st.m, g.k = (int64_t)st.k, g.n = (int64_t)st.n;
   g.lda = g.k, g.ldb = g.n, g.ldc = g.n;
@@ -204,6 +210,7 @@ void Executor::gen_gemm(const PairStep& st, size_t nbo, bool va, bool vb, const 
   int idx = (int)gemm_choice_.size();
   gemm_choice_.push_back(o_.gemm_kernel);
   g.bcol_run = bcol_run;
+  g.brow_run = brow_run;
   const Ref am = amax ? *amax : Ref{}, bm = bmax ? *bmax : Ref{};
   if (amax) use(am);
   if (bmax) use(bm);
@@ -277,27 +284,43 @@ void Executor::gen_steps(bool body, size_t nb, std::vector<Ref>& slots, std::vec
       }
       return cur;
     };
-    Ref a = chain(slots[st.li], shapes[st.li], st.sum_a, st.a_perm, nbA);
     // B read through gather tables instead of being permuted into [con, free_b]
     // (no batch legs, no summed legs): the GEMM kernels address b[p][j] as
-    // row_off[p] + col_off[j] of the unpermuted operand.
-    int64_t brow = -1, bcol = -1;
-    int32_t bcol_run = 1;
-    Ref b;
-    if (o_.gather && st.bt == 1 && st.sum_b.empty() && plan_permute(shapes[st.ri], st.b_perm, 1, es).kind != 0) {
-      size_t n_con = 0;  // right legs shared with the left operand and absent from the result
-      for (size_t d = 0; d < st.b_perm.size(); ++d) {
-        const Leg l = st.right_legs[st.b_perm[d]];
-        const bool in_left = std::count(st.left_legs.begin(), st.left_legs.end(), l) > 0;
-        const bool in_res = std::count(st.result_legs.begin(), st.result_legs.end(), l) > 0;
-        if (in_left && !in_res) n_con = d + 1;
+    // row_off[p] + col_off[j] of the unpermuted operand. The contracted legs are
+    // then taken in B's memory order (A is permuted to match), so where they are
+    // B's innermost legs the rows come in runs of consecutive offsets.
+    std::vector<size_t> a_perm = st.a_perm, b_perm = st.b_perm;
+    size_t n_con = 0;  // right legs shared with the left operand and absent from the result
+    for (size_t d = 0; d < b_perm.size(); ++d) {
+      const Leg l = st.right_legs[b_perm[d]];
+      const bool in_left = std::count(st.left_legs.begin(), st.left_legs.end(), l) > 0;
+      const bool in_res = std::count(st.result_legs.begin(), st.result_legs.end(), l) > 0;
+      if (in_left && !in_res) n_con = d + 1;
+    }
+    const bool gatherable = o_.gather && st.bt == 1 && st.sum_b.empty();
+    if (gatherable && n_con > 1) {
+      std::vector<size_t> bc(b_perm.begin(), b_perm.begin() + (long)n_con);
+      std::sort(bc.begin(), bc.end());
+      std::vector<Leg> la = st.left_legs;  // A's legs once its summed legs are gone
+      for (size_t ax : st.sum_a) la.erase(la.begin() + (long)ax);
+      const size_t nfa = a_perm.size() - n_con;  // a_perm = [free_a, con] without batch legs
+      for (size_t t = 0; t < n_con; ++t) {
+        const Leg l = st.right_legs[bc[t]];
+        a_perm[nfa + t] = (size_t)(std::find(la.begin(), la.end(), l) - la.begin());
+        b_perm[t] = bc[t];
       }
-      if (add_gather(shapes[st.ri], st.b_perm, n_con, brow, bcol, bcol_run)) {
+    }
+    Ref a = chain(slots[st.li], shapes[st.li], st.sum_a, a_perm, nbA);
+    int64_t brow = -1, bcol = -1;
+    int32_t bcol_run = 1, brow_run = 1;
+    Ref b;
+    if (gatherable && plan_permute(shapes[st.ri], b_perm, 1, es).kind != 0) {
+      if (add_gather(shapes[st.ri], b_perm, n_con, brow, bcol, bcol_run, brow_run)) {
         b = slots[st.ri];
         use(b);
       }
     }
-    if (bcol < 0) b = chain(slots[st.ri], shapes[st.ri], st.sum_b, st.b_perm, nbB);
+    if (bcol < 0) b = chain(slots[st.ri], shapes[st.ri], st.sum_b, b_perm, nbB);
     Ref c{Ref::WS, newbuf(nbC * st.bt * st.m * st.n * es)};
     // operand max magnitudes folded by the producing GEMMs (a permute or a gather
     // keeps the max; a summed leg does not)
@@ -309,7 +332,7 @@ void Executor::gen_steps(bool body, size_t nb, std::vector<Ref>& slots, std::vec
       Ref mb{Ref::WS, maxbits_buf_};
       use(mb);
       emit([=](const Ctx& cx) { launch_fill_zero(cx.ptr(mb), 8, cx.s); });
-      gen_gemm(st, nbC, va, vb, a, b, c, &mb, brow, bcol, bcol_run, am, bm);
+      gen_gemm(st, nbC, va, vb, a, b, c, &mb, brow, bcol, bcol_run, am, bm, brow_run);
       size_t numel = nbC * st.bt * st.m * st.n;
       Ref ex{Ref::WS, expo_buf_};
       const int keep = si + 1 == P.cp.steps.size() ? 0 : STRIP_KEEP;
@@ -321,10 +344,10 @@ void Executor::gen_steps(bool body, size_t nb, std::vector<Ref>& slots, std::vec
       Ref mc{Ref::WS, newbuf(8)};
       use(mc);
       emit([=](const Ctx& cx) { launch_fill_zero(cx.ptr(mc), 8, cx.s); });
-      gen_gemm(st, nbC, va, vb, a, b, c, &mc, brow, bcol, bcol_run, am, bm);
+      gen_gemm(st, nbC, va, vb, a, b, c, &mc, brow, bcol, bcol_run, am, bm, brow_run);
       slot_max_[st.out] = mc;
     } else {
-      gen_gemm(st, nbC, va, vb, a, b, c, nullptr, brow, bcol, bcol_run, am, bm);
+      gen_gemm(st, nbC, va, vb, a, b, c, nullptr, brow, bcol, bcol_run, am, bm, brow_run);
     }
     slots[st.out] = c;
     shapes[st.out] = st.result_shape;

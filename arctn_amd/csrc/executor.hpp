@@ -100,7 +100,7 @@ class Executor {
   // operands' max magnitudes (FP16X2), or null
   void gen_gemm(const PairStep& st, size_t nb_outer, bool va, bool vb, const Ref& a, const Ref& b, const Ref& c,
                 const Ref* cmax = nullptr, int64_t brow = -1, int64_t bcol = -1, int32_t bcol_run = 1,
-                const Ref* amax = nullptr, const Ref* bmax = nullptr);
+                const Ref* amax = nullptr, const Ref* bmax = nullptr, int32_t brow_run = 1);
   void run_ops(const std::vector<Op>& ops, const Ctx& c);
   void enqueue(const Ctx& c, const std::function<void(const Op&, const Ctx&)>& visit = nullptr);
   void autotune(const Ctx& c, hipStream_t s);
@@ -124,7 +124,7 @@ class Executor {
   // (con = the first n_con output axes), appended to const_host_; false when the
   // tables would be too large.
   bool add_gather(const std::vector<size_t>& shape, const std::vector<size_t>& perm, size_t n_con, int64_t& row,
-                  int64_t& col, int32_t& col_run);
+                  int64_t& col, int32_t& col_run, int32_t& row_run);
 
   std::shared_ptr<const Plan> P_;
   ExecOptions o_;

@@ -978,8 +978,13 @@ __device__ inline int swz(int row, int ch) {
   return row * BK + (((ch >> 3) ^ f) << 3);
 }
 
-template <bool CPLX, bool F16, int BK, bool BRAW>
+// BMODE: 0 B pre-split, 1 raw B with lanes along columns (a chunk is BE rows of
+// one column, wave-uniform rows: columns contiguous in memory), 2 raw B with
+// lanes along k (chunk q = column q / CH, k chunk q % CH, as for pre-split B: for
+// operands whose contracted legs are innermost in memory, GemmArgs::brow_run).
+template <bool CPLX, bool F16, int BK, int BMODE>
 __global__ __launch_bounds__(256, 2) void gemm_presplit_kernel(GemmArgs g, PreSplit ps) {
+  constexpr bool BRAW = BMODE != 0;
   using H = typename Half<F16>::H;
   using V = typename Half<F16>::V;
   static_assert(BK == 32 || BK == 64, "k tile");
@@ -1033,10 +1038,21 @@ __global__ __launch_bounds__(256, 2) void gemm_presplit_kernel(GemmArgs g, PreSp
         for (int j = 0; j < TN; ++j) acc[i][j] = v4f{0, 0, 0, 0};
       uint4 rah[A_Q], ral[A_Q], rbh[BRAW ? 1 : B_Q], rbl[BRAW ? 1 : B_Q];
       BT rbr[BRAW ? B_Q * BE : 1];
-      // raw B: the thread's one column, its gather offset hoisted out of the k loop
+      // raw B: the thread's column (one per chunk with lanes along k), its gather
+      // offset hoisted out of the k loop
       const int64_t bcn = (CPLX ? n0 / 2 : n0) + tid % BC;
-      const bool bcn_ok = BRAW && bcn < g.n;
+      const bool bcn_ok = BMODE == 1 && bcn < g.n;
       const int64_t bco = bcn_ok ? b_col(g, bcn) : 0;
+      int64_t kco[BMODE == 2 ? B_Q : 1];
+      bool kok[BMODE == 2 ? B_Q : 1];
+      if constexpr (BMODE == 2) {
+#pragma unroll
+        for (int u = 0; u < B_Q; ++u) {
+          const int64_t j = (CPLX ? n0 / 2 : n0) + (tid + 256 * u) / CH;
+          kok[u] = j < g.n;
+          kco[u] = kok[u] ? b_col(g, j) : 0;
+        }
+      }
       auto load = [&](int64_t k0) {
 #pragma unroll
         for (int u = 0; u < A_Q; ++u) {
@@ -1046,7 +1062,17 @@ __global__ __launch_bounds__(256, 2) void gemm_presplit_kernel(GemmArgs g, PreSp
           rah[u] = ok ? *reinterpret_cast<const uint4*>(Ah + gr * kp + gc) : make_uint4(0, 0, 0, 0);
           ral[u] = ok ? *reinterpret_cast<const uint4*>(Al + gr * kp + gc) : make_uint4(0, 0, 0, 0);
         }
-        if constexpr (BRAW) {
+        if constexpr (BMODE == 2) {
+#pragma unroll
+          for (int u = 0; u < B_Q; ++u) {
+            // BE consecutive k of one column: within one run of consecutive row
+            // offsets (brow_run is a multiple of BE, chunks start at multiples of BE)
+            const int64_t r0 = (k0 + ((tid + 256 * u) % CH) * 8) / (CPLX ? 2 : 1);
+            const BT* bp = Braw + kco[u] + (kok[u] && r0 < g.k ? b_row(g, r0) : 0);
+#pragma unroll
+            for (int e = 0; e < BE; ++e) rbr[u * BE + e] = kok[u] && r0 + e < g.k ? bp[e] : BT{};
+          }
+        } else if constexpr (BMODE == 1) {
           const BT* bp = Braw + bco;
 #pragma unroll
           for (int u = 0; u < B_Q; ++u) {
@@ -1083,7 +1109,8 @@ __global__ __launch_bounds__(256, 2) void gemm_presplit_kernel(GemmArgs g, PreSp
           int row, ch;
           uint4 vh, vl;
           if constexpr (BRAW) {
-            row = q % BC, ch = (q / BC) * 8;
+            if constexpr (BMODE == 2) row = q / CH, ch = (q % CH) * 8;
+            else row = q % BC, ch = (q / BC) * 8;
             H h[8], l[8];
 #pragma unroll
             for (int e = 0; e < BE; ++e) {
@@ -1509,26 +1536,31 @@ static int launch_gemm_kernels(int dt, int prec, int kernel, const GemmArgs& g0,
     static bool attr_set = false;                                                                          \
     if (!attr_set) {                                                                                       \
       ATN_HIP(hipFuncSetAttribute(                                                                         \
-          reinterpret_cast<const void*>(&gemm_presplit_kernel<C_, F_, PRESPLIT_BK, R_>),    \
+          reinterpret_cast<const void*>(&gemm_presplit_kernel<C_, F_, PRESPLIT_BK, R_>),                   \
           hipFuncAttributeMaxDynamicSharedMemorySize, (int)PRESPLIT_LDS));                                 \
       attr_set = true;                                                                                     \
     }                                                                                                      \
     LAUNCH((split_a_kernel<C_, F_>), ga, dim3(256), 0, s, g, ps);                                         \
-    if (!R_) LAUNCH((split_bt_kernel<C_, F_>), gb, dim3(256), 0, s, g, ps);                               \
+    if (R_ == 0) LAUNCH((split_bt_kernel<C_, F_>), gb, dim3(256), 0, s, g, ps);                           \
     g_last_kind = F_ ? K_GEMM_MFMA_FP16X2 : K_GEMM_MFMA_BF16X3;                                           \
-    LAUNCH((gemm_presplit_kernel<C_, F_, PRESPLIT_BK, R_>), grid, dim3(256), PRESPLIT_LDS, s, g, ps); \
+    LAUNCH((gemm_presplit_kernel<C_, F_, PRESPLIT_BK, R_>), grid, dim3(256), PRESPLIT_LDS, s, g, ps);     \
+  } while (0)
+#define PRESPLIT_MODES(R_)                                 \
+  do {                                                     \
+    if (cx && f16) PRESPLIT_LAUNCH(true, true, R_);        \
+    else if (cx) PRESPLIT_LAUNCH(true, false, R_);         \
+    else if (f16) PRESPLIT_LAUNCH(false, true, R_);        \
+    else PRESPLIT_LAUNCH(false, false, R_);                \
   } while (0)
       if (raw_b) {
-        if (cx && f16) PRESPLIT_LAUNCH(true, true, true);
-        else if (cx) PRESPLIT_LAUNCH(true, false, true);
-        else if (f16) PRESPLIT_LAUNCH(false, true, true);
-        else PRESPLIT_LAUNCH(false, false, true);
+        // lanes along k where the gathered B has runs of consecutive k offsets
+        // (its contracted legs innermost in memory) and short column runs
+        if (g.bcol && g.brow_run % (cx ? 4 : 8) == 0 && g.bcol_run < 16) PRESPLIT_MODES(2);
+        else PRESPLIT_MODES(1);
         return GEMM_PRESPLIT_A;
       }
-      if (cx && f16) PRESPLIT_LAUNCH(true, true, false);
-      else if (cx) PRESPLIT_LAUNCH(true, false, false);
-      else if (f16) PRESPLIT_LAUNCH(false, true, false);
-      else PRESPLIT_LAUNCH(false, false, false);
+      PRESPLIT_MODES(0);
+#undef PRESPLIT_MODES
 #undef PRESPLIT_LAUNCH
       return GEMM_PRESPLIT;
     }

@@ -127,6 +127,9 @@ struct GemmArgs {
   // bcol holds runs of this many consecutive offsets, each starting at a multiple
   // of the run length (vector loads stay within a run)
   int32_t bcol_run = 1;
+  // brow holds runs of this many consecutive offsets, each starting at a multiple
+  // of the run length (B's contracted legs innermost in memory)
+  int32_t brow_run = 1;
   // FP16X2: max |re|, |im| of A / of B as double bits when already known (folded
   // into the epilogue of the GEMM that produced the operand); null = measured by
   // a pass before the GEMM. An upper bound is enough: it only sets the scale.

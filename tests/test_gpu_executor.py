@@ -301,3 +301,28 @@ def test_forced_gemm_kernels_with_gather_tables(cuda, dtype):
                     err = np.abs(got.cpu().numpy() - want).max() / scale
                     assert err < tol, (len(net["inputs"]), prec, kern, gather, err)
         plan.release()
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.complex64])
+def test_split_gemm_lanes_along_k_for_k_innermost_operands(cuda, dtype):
+    """B whose contracted legs are its innermost legs (rows in runs of consecutive
+    offsets, columns far apart): the B-staging split kernel reads it with lanes
+    along k. Ragged sizes, every precision, against NumPy in complex128."""
+    import torch
+
+    code = {np.dtype(np.float32): 0, np.dtype(np.complex64): 2}[np.dtype(dtype)]
+    rng = np.random.default_rng(11)
+    for dims in ({0: 70, 1: 8, 2: 5, 3: 150}, {0: 130, 1: 4, 2: 2, 3: 97}, {0: 64, 1: 16, 2: 4, 3: 300}):
+        net = {"inputs": [[0, 1, 2], [3, 1, 2]], "output": [0, 3], "size_dict": dims}
+        arrays = [rng.standard_normal(tuple(dims[l] for l in t)).astype(dtype) for t in net["inputs"]]
+        if np.issubdtype(np.dtype(dtype), np.complexfloating):
+            arrays = [(a + 1j * rng.standard_normal(a.shape)).astype(dtype) for a in arrays]
+        want = np.einsum("ikl,jkl->ij", *[a.astype(np.complex128) for a in arrays])
+        plan = Plan(net["inputs"], net["output"], net["size_dict"], [(0, 1)])
+        dev = [torch.from_numpy(a).cuda() for a in arrays]
+        for prec, tol in (("native", 1e-5), ("bf16x3", 2e-4), ("fp16x2", 1e-5)):
+            for kern in (8, 4, 1):
+                got, _ = plan.run(dev, code, batched=True, precision=prec, gemm_kernel=kern)
+                err = np.abs(got.cpu().numpy() - want).max() / np.abs(want).max()
+                assert err < tol, (dims, prec, kern, err)
+        plan.release()
