@@ -290,9 +290,18 @@ void Executor::gen_steps(bool body, size_t nb, std::vector<Ref>& slots, std::vec
     // then taken in B's memory order (A is permuted to match), so where they are
     // B's innermost legs the rows come in runs of consecutive offsets.
     std::vector<size_t> a_perm = st.a_perm, b_perm = st.b_perm;
+    // B may be stored in another axis order than the plan's (only slots read as B
+    // without summed legs are): its permutation is taken over the stored axes
+    const std::vector<size_t>& ob = slot_order_[st.ri];
+    std::vector<Leg> rl(ob.size());  // B's legs in storage order
+    {
+      std::vector<size_t> pos(ob.size());
+      for (size_t d = 0; d < ob.size(); ++d) pos[ob[d]] = d, rl[d] = st.right_legs[ob[d]];
+      for (size_t& x : b_perm) x = pos[x];
+    }
     size_t n_con = 0;  // right legs shared with the left operand and absent from the result
     for (size_t d = 0; d < b_perm.size(); ++d) {
-      const Leg l = st.right_legs[b_perm[d]];
+      const Leg l = rl[b_perm[d]];
       const bool in_left = std::count(st.left_legs.begin(), st.left_legs.end(), l) > 0;
       const bool in_res = std::count(st.result_legs.begin(), st.result_legs.end(), l) > 0;
       if (in_left && !in_res) n_con = d + 1;
@@ -305,9 +314,48 @@ void Executor::gen_steps(bool body, size_t nb, std::vector<Ref>& slots, std::vec
       for (size_t ax : st.sum_a) la.erase(la.begin() + (long)ax);
       const size_t nfa = a_perm.size() - n_con;  // a_perm = [free_a, con] without batch legs
       for (size_t t = 0; t < n_con; ++t) {
-        const Leg l = st.right_legs[bc[t]];
+        const Leg l = rl[bc[t]];
         a_perm[nfa + t] = (size_t)(std::find(la.begin(), la.end(), l) - la.begin());
         b_perm[t] = bc[t];
+      }
+    }
+    // Storage order of the result: [free_a][free_b in B's column order]. When the
+    // result is the B of a later step, B's free legs are ordered for both reads:
+    // the innermost run of them in B's memory (this step's contiguous columns)
+    // stays innermost, and the others go legs contracted by the reader first,
+    // its free legs last, so that its columns come in runs too.
+    std::vector<size_t>& oc = slot_order_[st.out];
+    oc.resize(st.result_legs.size());
+    for (size_t d = 0; d < oc.size(); ++d) oc[d] = d;
+    const long rd = consumer_[st.out].first;
+    if (gatherable && rd >= 0 && consumer_[st.out].second == 1 && P.cp.steps[rd].bt == 1 &&
+        P.cp.steps[rd].sum_b.empty()) {
+      const size_t nfa = a_perm.size() - n_con;
+      std::vector<Leg> res;  // the plan's result order must be [free_a (a_perm)][free_b (b_perm)]
+      std::vector<Leg> la = st.left_legs;
+      for (size_t ax : st.sum_a) la.erase(la.begin() + (long)ax);
+      for (size_t d = 0; d < nfa; ++d) res.push_back(la[a_perm[d]]);
+      std::vector<size_t> fr(b_perm.begin() + (long)n_con, b_perm.end());
+      for (size_t a : fr) res.push_back(rl[a]);
+      if (res == st.result_legs) {
+        std::sort(fr.begin(), fr.end());  // memory order
+        size_t run = 0;
+        while (run < fr.size() && fr[fr.size() - 1 - run] == rl.size() - 1 - run) ++run;
+        const PairStep& cs = P.cp.steps[rd];
+        auto read_con = [&](Leg l) {
+          return std::count(cs.left_legs.begin(), cs.left_legs.end(), l) > 0 &&
+                 std::count(cs.result_legs.begin(), cs.result_legs.end(), l) == 0;
+        };
+        std::vector<size_t> nf;
+        for (size_t t = 0; t + run < fr.size(); ++t)
+          if (read_con(rl[fr[t]])) nf.push_back(fr[t]);
+        for (size_t t = 0; t + run < fr.size(); ++t)
+          if (!read_con(rl[fr[t]])) nf.push_back(fr[t]);
+        for (size_t t = fr.size() - run; t < fr.size(); ++t) nf.push_back(fr[t]);
+        std::copy(nf.begin(), nf.end(), b_perm.begin() + (long)n_con);
+        for (size_t t = 0; t < nf.size(); ++t)
+          oc[nfa + t] = (size_t)(std::find(st.result_legs.begin(), st.result_legs.end(), rl[nf[t]]) -
+                                 st.result_legs.begin());
       }
     }
     Ref a = chain(slots[st.li], shapes[st.li], st.sum_a, a_perm, nbA);
@@ -350,7 +398,8 @@ void Executor::gen_steps(bool body, size_t nb, std::vector<Ref>& slots, std::vec
       gen_gemm(st, nbC, va, vb, a, b, c, nullptr, brow, bcol, bcol_run, am, bm, brow_run);
     }
     slots[st.out] = c;
-    shapes[st.out] = st.result_shape;
+    shapes[st.out].resize(oc.size());
+    for (size_t d = 0; d < oc.size(); ++d) shapes[st.out][d] = st.result_shape[oc[d]];
   }
 }
 
@@ -423,6 +472,19 @@ void Executor::build(size_t B) {
   std::vector<std::vector<size_t>> shapes(nslot);
   ScatterArgs sc = make_scatter(P);
   slot_max_.assign(nslot, Ref{});
+  // inputs are stored in the plan's order (as the steps see them)
+  slot_order_.assign(nslot, {});
+  consumer_.assign(nslot, {-1, 0});
+  for (size_t i = 0; i < P.cp.steps.size(); ++i) {
+    const PairStep& st = P.cp.steps[i];
+    consumer_[st.li] = {(long)i, 0};
+    consumer_[st.ri] = {(long)i, 1};
+    for (auto [slot, rank] : {std::pair<size_t, size_t>{st.li, st.left_legs.size()}, {st.ri, st.right_legs.size()}})
+      if (slot < P.cp.n_inputs) {
+        slot_order_[slot].resize(rank);
+        for (size_t d = 0; d < rank; ++d) slot_order_[slot][d] = d;
+      }
+  }
   want_max_.assign(nslot, 0);
   if (o_.precision == FP16X2 && !o_.strip)
     for (const PairStep& st : P.cp.steps)

@@ -117,6 +117,12 @@ class Executor {
   // and the slot holding it per plan slot
   std::vector<char> want_max_;
   std::vector<Ref> slot_max_;
+  // Storage order of each plan slot's axes (stored axis d holds the plan's axis
+  // slot_order_[s][d]); a step may store its result with the free legs of B in an
+  // order chosen for the step that reads it as B. consumer_[s]: (step, 1 if read
+  // as B) of slot s, or (-1, 0).
+  std::vector<std::vector<size_t>> slot_order_;
+  std::vector<std::pair<long, int>> consumer_;
   // Gather tables (int32 element offsets) of B operands read without a permute.
   std::vector<int32_t> const_host_;
   int32_t* consts_ = nullptr;
