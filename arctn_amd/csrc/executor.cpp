@@ -783,9 +783,12 @@ void Executor::autotune(const Ctx& c, hipStream_t s) {
       if (cand == GEMM_SIMPLE && macs > (double)(1 << 26)) continue;
       if (cand == GEMM_SKINNY && q.m > 64 && q.n > 32) continue;
       if (cand == GEMM_SPLITK && (q.k < 256 || outs > (double)(1 << 16))) continue;
-      // split products (GEMM_MFMA splits while staging) only where they can pay off
-      if ((cand == GEMM_PRESPLIT || cand == GEMM_PRESPLIT_A || cand == GEMM_MFMA) && (!split_prec || q.m < 64))
+      // split products only where they can pay off (the pre-split kernels have
+      // 32-row tiles for short A; GEMM_MFMA splits while staging into 128-row tiles)
+      if ((cand == GEMM_PRESPLIT || cand == GEMM_PRESPLIT_A) &&
+          !split_products_apply(o_.dtype, o_.precision, q.m, q.k, q.n))
         continue;
+      if (cand == GEMM_MFMA && (!split_prec || q.m < 64)) continue;
       if (cand == GEMM_MFMA_R32 && q.m > 64) continue;
       if (cand == GEMM_MFMA_R64 && q.m > 256) continue;
       if (cand == GEMM_MFMA_R128 && q.m < 64) continue;

@@ -57,7 +57,13 @@ template <> struct CM<double2> {
 // Result max-magnitude tracking for the strip (GemmArgs::cmax): each thread
 // folds max |re|, |im| of the values it stores, one atomicMax per wave at the
 // end. Non-negative doubles order like their bit patterns; NaN never wins.
-// Grid bound of kernels that track the result max (persistent loops over tiles).
+// Grid bound of kernels that track the result max where A is one row tile
+// (persistent loops over column tiles). With several row tiles the full grid
+// runs: the workgroups in flight then hold all row tiles of a range of column
+// tiles of one batch entry, which share each B panel in L2, where a capped grid
+// spreads them over many batch entries (complex64 m=1024 k=512 n=8192 x256:
+// 24.2 ms full, 26.9 capped; on one row tile the cap is the faster, 6.6 against
+// 7.5 ms on m=16 k=16 n=262144 x256).
 constexpr unsigned TRACK_WORKGROUPS = 8192;
 
 // The running max is kept in the value's own precision (float compares for
@@ -850,7 +856,8 @@ constexpr size_t SPLIT_LDS = 4 * 128 * (64 + 8) * 2;
 // Measured on gfx950 (complex64 1024x512x8192 x64, BF16x3): BK 64 with one LDS
 // buffer (two workgroups per CU) 302 TFLOP/s algorithmic, BK 32 with two 277.
 constexpr int PRESPLIT_BK = 64;
-constexpr size_t PRESPLIT_LDS = 4 * 128 * PRESPLIT_BK * 2;
+// (hi, lo) halves of a BM x BK tile of A and a 128 x BK tile of B'
+constexpr size_t presplit_lds(int bm) { return 2 * (size_t)(bm + 128) * PRESPLIT_BK * 2; }
 
 // ------------------------------------------------------------------ dispatch --
 // ------------------------------------------------ pre-split operand GEMM --
@@ -982,27 +989,32 @@ __device__ inline int swz(int row, int ch) {
 // one column, wave-uniform rows: columns contiguous in memory), 2 raw B with
 // lanes along k (chunk q = column q / CH, k chunk q % CH, as for pre-split B: for
 // operands whose contracted legs are innermost in memory, GemmArgs::brow_run).
-template <bool CPLX, bool F16, int BK, int BMODE>
-__global__ __launch_bounds__(256, 2) void gemm_presplit_kernel(GemmArgs g, PreSplit ps) {
+// BM: rows of A per tile. 128: four 64x64 wave tiles (two workgroups per CU);
+// 32: four 32x32 wave tiles side by side (40 KiB of LDS, four workgroups per
+// CU) for m <= 32, where B streams from HBM once and the MFMA work per B byte
+// is a quarter of the 128-row tile's.
+template <bool CPLX, bool F16, int BK, int BMODE, int BM>
+__global__ __launch_bounds__(256, BM == 32 ? 4 : 2) void gemm_presplit_kernel(GemmArgs g, PreSplit ps) {
   constexpr bool BRAW = BMODE != 0;
   using H = typename Half<F16>::H;
   using V = typename Half<F16>::V;
   static_assert(BK == 32 || BK == 64, "k tile");
-  constexpr int BM = 128, BN = 128, CH = BK / 8;
-  constexpr int WM = 64, WN = 64, TM = 4, TN = 4;
+  static_assert(BM == 128 || (BM == 32 && BK == 64), "row tile");
+  constexpr int BN = 128, CH = BK / 8;
+  constexpr int WM = BM == 128 ? 64 : 32, WN = WM, TM = WM / 16, TN = WN / 16;
   constexpr int A_Q = BM * CH / 256;                    // 16-byte A chunks per thread
   constexpr int B_Q = (CPLX ? BN / 2 : BN) * CH / 256;  // 16-byte B chunks per thread
-  constexpr int TILE = 128 * BK;                        // halves per LDS array
+  constexpr int TILE_A = BM * BK, TILE_B = BN * BK;     // halves per LDS array
   // raw B: chunk q of a k tile is (tile column q % BC, k group q / BC), so the
   // lanes of a wave read consecutive columns of one B row; a chunk is BE rows
   constexpr int BC = CPLX ? BN / 2 : BN, BE = CPLX ? 4 : 8;
   static_assert(256 % BC == 0, "a thread's raw B loads share one column");
   using BT = typename std::conditional<CPLX, float2, float>::type;
   extern __shared__ __align__(16) char smem_split[];
-  H* const ahi = reinterpret_cast<H*>(smem_split);  // [ahi, alo, bhi, blo][TILE]
-  H* const alo = ahi + TILE;
-  H* const bhi = alo + TILE;
-  H* const blo = bhi + TILE;
+  H* const ahi = reinterpret_cast<H*>(smem_split);  // ahi, alo [TILE_A]; bhi, blo [TILE_B]
+  H* const alo = ahi + TILE_A;
+  H* const bhi = alo + TILE_A;
+  H* const blo = bhi + TILE_B;
 
   float unscale = 1.f, sb = 1.f;
   float mx = 0.f;
@@ -1014,7 +1026,7 @@ __global__ __launch_bounds__(256, 2) void gemm_presplit_kernel(GemmArgs g, PreSp
   const int64_t ldc = CPLX ? 2 * g.ldc : g.ldc;
   const int tiles_n = (int)((N + BN - 1) / BN), tiles_m = (int)((M + BM - 1) / BM);
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int wm0 = (wave >> 1) * WM, wn0 = (wave & 1) * WN;
+  const int wm0 = BM == 128 ? (wave >> 1) * WM : 0, wn0 = BM == 128 ? (wave & 1) * WN : wave * WN;
   const uint64_t nz = (uint64_t)g.nbo * g.nbi, ntile = (uint64_t)tiles_m * tiles_n;
   const int64_t ainner = g.sai ? g.nbi : 1, binner = g.sbi ? g.nbi : 1;
 
@@ -1239,8 +1251,9 @@ static void run_mfma(int dt, const GemmArgs& g0, hipStream_t s) {
   int64_t N = CPLX ? 2 * g.n : g.n;
   uint64_t tiles = (uint64_t)((g.m + BM - 1) / BM) * (uint64_t)((N + BN - 1) / BN);
   dim3 grid((unsigned)std::min<uint64_t>(tiles, 1u << 20), zgrid(g), (unsigned)ks.n_split);
-  // tracking the result max: a persistent grid keeps the per-wave publishes few
-  if (g.cmax) grid.x = std::min<unsigned>(grid.x, std::max<unsigned>(1, TRACK_WORKGROUPS / (grid.y * grid.z)));
+  // tracking the result max over one row tile: a persistent grid
+  if (g.cmax && g.m <= BM)
+    grid.x = std::min<unsigned>(grid.x, std::max<unsigned>(1, TRACK_WORKGROUPS / (grid.y * grid.z)));
   g_last_kind = sizeof(F) == 4 ? K_GEMM_MFMA_F32 : K_GEMM_MFMA_F64;
   // with split k the reduce kernel tracks the result max
   if (g.cmax && !g.part) LAUNCH((gemm_mfma_kernel<F, CPLX, BM, BN, BK, true>), grid, dim3(256), 0, s, g);
@@ -1427,10 +1440,15 @@ static PreSplitSizes presplit_sizes(int dt, const GemmArgs& g) {
 
 // Split-precision products pay off on MFMA-bound shapes only; memory-bound
 // skinny steps run the FP32 kernels (same time, full FP32 accuracy).
+// Split products pay off with 128-row tiles from m = 64 and with 32-row tiles
+// for 8 <= m <= 32 (measured on gfx950, complex64 x256 with contiguous B, BF16x3
+// against the 32-row FP32 MFMA kernel: m=32 k=128 n=131072 9.0 vs 13.0 ms, m=8
+// k=256 n=32768 3.6 vs 5.3 ms; at k=16 the two tie, both bound by the result
+// write).
 static bool split_worth(int dt, const GemmArgs& g) {
   const bool cplx = dtype_complex(dt);
   const int64_t K = cplx ? 2 * g.k : g.k, N = cplx ? 2 * g.n : g.n;
-  return g.m >= 64 && N >= 128 && K >= 64;
+  return (g.m >= 64 || (g.m >= 8 && g.m <= 32)) && N >= 128 && K >= 64;
 }
 
 static int native_mfma_kernel(int dt, int64_t m) {
@@ -1441,12 +1459,13 @@ static int native_mfma_kernel(int dt, int64_t m) {
 static int auto_kernel(int dt, int prec, const GemmArgs& g) {
   const bool split_prec = (prec == BF16X3 || prec == FP16X2) && (dt == F32 || dt == C64);
   if (split_prec && split_worth(dt, g)) {
-    // One or two 128-row tiles of A: A split by a pass, B read in its own
-    // precision and split while staging (measured on gfx950: up to 1.6x faster
-    // than splitting B by a pass on short-k steps, where that pass's extra read
-    // and write of B dominate). With more row tiles every tile re-reads and
-    // re-splits B, and on operands read through gather tables (short runs of
-    // consecutive columns) one splitting pass over B is faster.
+    // One or two row tiles of A (32 rows for m <= 32, else 128): A split by a
+    // pass, B read in its own precision and split while staging (measured on
+    // gfx950: up to 1.6x faster than splitting B by a pass on short-k steps,
+    // where that pass's extra read and write of B dominate). With more row tiles
+    // every tile re-reads and re-splits B, and on operands read through gather
+    // tables (short runs of consecutive columns) one splitting pass over B is
+    // faster.
     return (g.m + 127) / 128 <= 2 ? GEMM_PRESPLIT_A : GEMM_PRESPLIT;
   }
   if (mfma_worth(dt, g)) return native_mfma_kernel(dt, g.m);
@@ -1504,9 +1523,11 @@ static int launch_gemm_kernels(int dt, int prec, int kernel, const GemmArgs& g0,
     const bool pre = kernel != GEMM_MFMA && g.scratch &&
                      g.scratch_bytes >= 256 + (raw_b ? 2 * psz.a_half : psz.total);
     int64_t N = dt == C64 ? 2 * g.n : g.n;
-    uint64_t tiles = (uint64_t)((g.m + 127) / 128) * (uint64_t)((N + 127) / 128);
+    // 32-row tiles of A for short A (pre-split kernels only)
+    const int bm = pre && g.m <= 32 ? 32 : 128;
+    uint64_t tiles = (uint64_t)((g.m + bm - 1) / bm) * (uint64_t)((N + 127) / 128);
     dim3 grid((unsigned)std::min<uint64_t>(tiles, 1u << 20), zgrid(g));
-    if (g.cmax) grid.x = std::min<unsigned>(grid.x, std::max<unsigned>(1, TRACK_WORKGROUPS / grid.y));
+    if (g.cmax && g.m <= bm) grid.x = std::min<unsigned>(grid.x, std::max<unsigned>(1, TRACK_WORKGROUPS / grid.y));
     if (prec == FP16X2) {
       if (!g.scratch) throw Error("FP16X2 GEMM needs 16 bytes of device scratch");
       // operand max magnitudes -> per-GEMM power-of-two scales (measured here
@@ -1531,19 +1552,24 @@ static int launch_gemm_kernels(int dt, int prec, int kernel, const GemmArgs& g0,
       const int64_t na = n_mats(g.sao, g.sai, g.nbo, g.nbi) * g.m * psz.kp / 4;
       const int64_t tiles_b = n_mats(g.sbo, g.sbi, g.nbo, g.nbi) * ((psz.kp / (cx ? 2 : 1) + 31) / 32) * ((g.n + 63) / 64);
       const dim3 ga((unsigned)std::min<int64_t>((na + 255) / 256, 65536)), gb((unsigned)std::min<int64_t>(tiles_b, 65536));
-#define PRESPLIT_LAUNCH(C_, F_, R_)                                                                        \
+#define PRESPLIT_KERNEL(C_, F_, R_, M_)                                                                    \
   do {                                                                                                     \
     static bool attr_set = false;                                                                          \
     if (!attr_set) {                                                                                       \
       ATN_HIP(hipFuncSetAttribute(                                                                         \
-          reinterpret_cast<const void*>(&gemm_presplit_kernel<C_, F_, PRESPLIT_BK, R_>),                   \
-          hipFuncAttributeMaxDynamicSharedMemorySize, (int)PRESPLIT_LDS));                                 \
+          reinterpret_cast<const void*>(&gemm_presplit_kernel<C_, F_, PRESPLIT_BK, R_, M_>),               \
+          hipFuncAttributeMaxDynamicSharedMemorySize, (int)presplit_lds(M_)));                             \
       attr_set = true;                                                                                     \
     }                                                                                                      \
+    LAUNCH((gemm_presplit_kernel<C_, F_, PRESPLIT_BK, R_, M_>), grid, dim3(256), presplit_lds(M_), s, g, ps); \
+  } while (0)
+#define PRESPLIT_LAUNCH(C_, F_, R_)                                                                        \
+  do {                                                                                                     \
     LAUNCH((split_a_kernel<C_, F_>), ga, dim3(256), 0, s, g, ps);                                         \
     if (R_ == 0) LAUNCH((split_bt_kernel<C_, F_>), gb, dim3(256), 0, s, g, ps);                           \
     g_last_kind = F_ ? K_GEMM_MFMA_FP16X2 : K_GEMM_MFMA_BF16X3;                                           \
-    LAUNCH((gemm_presplit_kernel<C_, F_, PRESPLIT_BK, R_>), grid, dim3(256), PRESPLIT_LDS, s, g, ps);     \
+    if (bm == 32) PRESPLIT_KERNEL(C_, F_, R_, 32);                                                         \
+    else PRESPLIT_KERNEL(C_, F_, R_, 128);                                                                 \
   } while (0)
 #define PRESPLIT_MODES(R_)                                 \
   do {                                                     \
@@ -1562,6 +1588,7 @@ static int launch_gemm_kernels(int dt, int prec, int kernel, const GemmArgs& g0,
       PRESPLIT_MODES(0);
 #undef PRESPLIT_MODES
 #undef PRESPLIT_LAUNCH
+#undef PRESPLIT_KERNEL
       return GEMM_PRESPLIT;
     }
     int64_t cf = dt == C64 ? 2 : 1;

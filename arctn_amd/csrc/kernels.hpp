@@ -138,7 +138,8 @@ struct GemmArgs {
 };
 // GEMM_PRESPLIT: split-precision MFMA GEMM whose operands are split into
 // 16-bit (hi, lo) arrays by a separate pass (GEMM_MFMA splits them while staging
-// each tile); for native precision it is GEMM_MFMA.
+// each tile); for native precision it is GEMM_MFMA. Its row tile is 32 rows of
+// A for m <= 32, else 128.
 // GEMM_MFMA_R32 / R64 / R128: the FP32 / FP64 MFMA kernel with that row tile
 // whatever the precision mode (GEMM_MFMA picks the row tile from m, or splits
 // the operands while staging under split precision; FP64 tiles have at most
