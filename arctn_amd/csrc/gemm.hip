@@ -544,6 +544,18 @@ __global__ __launch_bounds__(256) void gemm_mfma_kernel(GemmArgs g) {
       const int64_t bcn = CPLX ? n0 / 2 + tid % (BN / 2) : n0 + tid % BN;
       const bool bcn_ok = bcn < (CPLX ? g.n : N);
       const int64_t bco = bcn_ok ? b_col(g, bcn) : 0;
+      // gather offsets of the B rows of a k tile, read one tile ahead of its B
+      // loads (vector loads: a scalar load in flight would hold up the LDS waits)
+      int32_t rro[B_PER];
+      auto fetch_rows = [&](int64_t k0) {
+        if (!g.bcol) return;
+#pragma unroll
+        for (int u = 0; u < B_PER; ++u) {
+          const int e = tid + 256 * u;
+          const int64_t gr = CPLX ? k0 / 2 + e / (BN / 2) : k0 + e / BN;
+          rro[u] = gr < (CPLX ? kend / 2 : kend) ? g.brow[gr] : 0;
+        }
+      };
       auto load = [&](int64_t k0) {
 #pragma unroll
         for (int u = 0; u < A_PER; ++u) {
@@ -557,7 +569,7 @@ __global__ __launch_bounds__(256) void gemm_mfma_kernel(GemmArgs g) {
             int e = tid + 256 * u, p = e / (BN / 2);
             int64_t gp = k0 / 2 + p;
             bool ok = gp < kend / 2 && bcn_ok;
-            const int64_t bi = ok ? (b_row(g, gp) + bco) * 2 : 0;
+            const int64_t bi = ok ? ((g.bcol ? (int64_t)rro[u] : gp * g.ldb) + bco) * 2 : 0;
             rb[2 * u] = ok ? B[bi] : (F)0;
             rb[2 * u + 1] = ok ? B[bi + 1] : (F)0;
           }
@@ -566,7 +578,7 @@ __global__ __launch_bounds__(256) void gemm_mfma_kernel(GemmArgs g) {
           for (int u = 0; u < B_PER; ++u) {
             int e = tid + 256 * u, row = e / BN;
             int64_t gr = k0 + row;
-            rb[u] = (gr < kend && bcn_ok) ? B[b_row(g, gr) + bco] : (F)0;
+            rb[u] = (gr < kend && bcn_ok) ? B[(g.bcol ? (int64_t)rro[u] : gr * g.ldb) + bco] : (F)0;
           }
         }
       };
@@ -596,11 +608,16 @@ __global__ __launch_bounds__(256) void gemm_mfma_kernel(GemmArgs g) {
       };
 
       const int64_t nk = (kend - kbeg + BK - 1) / BK;
+      fetch_rows(kbeg);
       load(kbeg);
+      if (nk > 1) fetch_rows(kbeg + BK);
       store();
       __syncthreads();
       for (int64_t kt = 0; kt < nk; ++kt) {
-        if (kt + 1 < nk) load(kbeg + (kt + 1) * BK);
+        if (kt + 1 < nk) {
+          load(kbeg + (kt + 1) * BK);
+          if (kt + 2 < nk) fetch_rows(kbeg + (kt + 2) * BK);
+        }
 #pragma unroll
         for (int kk = 0; kk < BK; kk += 4) {
           F af[TM], bf[TN];
@@ -994,7 +1011,7 @@ __device__ inline int swz(int row, int ch) {
 // CU) for m <= 32, where B streams from HBM once and the MFMA work per B byte
 // is a quarter of the 128-row tile's.
 template <bool CPLX, bool F16, int BK, int BMODE, int BM>
-__global__ __launch_bounds__(256, BM == 32 ? 4 : 2) void gemm_presplit_kernel(GemmArgs g, PreSplit ps) {
+__global__ __launch_bounds__(256, BM == 32 ? 3 : 2) void gemm_presplit_kernel(GemmArgs g, PreSplit ps) {
   constexpr bool BRAW = BMODE != 0;
   using H = typename Half<F16>::H;
   using V = typename Half<F16>::V;
@@ -1065,6 +1082,30 @@ __global__ __launch_bounds__(256, BM == 32 ? 4 : 2) void gemm_presplit_kernel(Ge
           kco[u] = kok[u] ? b_col(g, j) : 0;
         }
       }
+      // Gather offsets of the B rows of a k tile, read one tile ahead of its B
+      // loads. Vector loads, also where the row is wave-uniform: a scalar load in
+      // flight would make every LDS wait of the MFMA loop wait for it as well
+      // (scalar and LDS reads share one counter, and scalar ones return out of
+      // order), which cost a third of the kernel's rate on gathered operands.
+      constexpr int NR = BMODE == 2 ? B_Q : BMODE == 1 ? B_Q * BE : 1;
+      int32_t rro[NR];
+      auto fetch_rows = [&](int64_t k0) {
+        if (!g.bcol) return;
+        if constexpr (BMODE == 2) {
+#pragma unroll
+          for (int u = 0; u < B_Q; ++u) {
+            const int64_t r0 = (k0 + ((tid + 256 * u) % CH) * 8) / (CPLX ? 2 : 1);
+            rro[u] = kok[u] && r0 < g.k ? g.brow[r0] : 0;
+          }
+        } else if constexpr (BMODE == 1) {
+#pragma unroll
+          for (int u = 0; u < B_Q; ++u) {
+            const int64_t r0 = (k0 + (int64_t)((tid + 256 * u) / BC) * 8) / (CPLX ? 2 : 1);
+#pragma unroll
+            for (int e = 0; e < BE; ++e) rro[u * BE + e] = r0 + e < g.k ? g.brow[r0 + e] : 0;
+          }
+        }
+      };
       auto load = [&](int64_t k0) {
 #pragma unroll
         for (int u = 0; u < A_Q; ++u) {
@@ -1080,7 +1121,7 @@ __global__ __launch_bounds__(256, BM == 32 ? 4 : 2) void gemm_presplit_kernel(Ge
             // BE consecutive k of one column: within one run of consecutive row
             // offsets (brow_run is a multiple of BE, chunks start at multiples of BE)
             const int64_t r0 = (k0 + ((tid + 256 * u) % CH) * 8) / (CPLX ? 2 : 1);
-            const BT* bp = Braw + kco[u] + (kok[u] && r0 < g.k ? b_row(g, r0) : 0);
+            const BT* bp = Braw + kco[u] + (g.bcol ? (int64_t)rro[u] : kok[u] && r0 < g.k ? r0 * g.ldb : 0);
 #pragma unroll
             for (int e = 0; e < BE; ++e) rbr[u * BE + e] = kok[u] && r0 + e < g.k ? bp[e] : BT{};
           }
@@ -1089,13 +1130,13 @@ __global__ __launch_bounds__(256, BM == 32 ? 4 : 2) void gemm_presplit_kernel(Ge
 #pragma unroll
           for (int u = 0; u < B_Q; ++u) {
             // first B row of the chunk (complex rows for complex B; k0 is even). BC is
-            // a multiple of the wave size, so the row is the same on every lane of a
-            // wave and its gather offset is one scalar load.
+            // a multiple of the wave size, so the row is the same on every lane of a wave.
             const int kc = __builtin_amdgcn_readfirstlane((tid + 256 * u) / BC);
             const int64_t r0 = (k0 + kc * 8) / (CPLX ? 2 : 1);
 #pragma unroll
             for (int e = 0; e < BE; ++e)
-              rbr[u * BE + e] = bcn_ok && r0 + e < g.k ? bp[b_row(g, r0 + e)] : BT{};
+              rbr[u * BE + e] =
+                  bcn_ok && r0 + e < g.k ? bp[g.bcol ? (int64_t)rro[u * BE + e] : (r0 + e) * g.ldb] : BT{};
           }
         } else {
 #pragma unroll
@@ -1151,11 +1192,16 @@ __global__ __launch_bounds__(256, BM == 32 ? 4 : 2) void gemm_presplit_kernel(Ge
       };
       const int64_t nk = (kp + BK - 1) / BK;
       const int cl = lane & 15;
+      fetch_rows(0);
       load(0);
+      if (nk > 1) fetch_rows(BK);
       store();
       __syncthreads();
       for (int64_t kt = 0; kt < nk; ++kt) {
-        if (kt + 1 < nk) load((kt + 1) * BK);
+        if (kt + 1 < nk) {
+          load((kt + 1) * BK);
+          if (kt + 2 < nk) fetch_rows((kt + 2) * BK);
+        }
 #pragma unroll
         for (int ks = 0; ks < BK; ks += 32) {
           const int kg = ks + (lane >> 4) * 8;
