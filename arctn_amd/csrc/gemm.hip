@@ -1122,8 +1122,15 @@ __global__ __launch_bounds__(256, BM == 32 ? 3 : 2) void gemm_presplit_kernel(Ge
             // offsets (brow_run is a multiple of BE, chunks start at multiples of BE)
             const int64_t r0 = (k0 + ((tid + 256 * u) % CH) * 8) / (CPLX ? 2 : 1);
             const BT* bp = Braw + kco[u] + (g.bcol ? (int64_t)rro[u] : kok[u] && r0 < g.k ? r0 * g.ldb : 0);
+            static_assert(BE * sizeof(BT) == 32, "a chunk is 32 bytes of B");
+            if (kok[u] && r0 + BE <= g.k && ((uintptr_t)bp & 15) == 0) {  // two 16-byte loads
+              const uint4* bw = reinterpret_cast<const uint4*>(bp);
+              uint4* dst = reinterpret_cast<uint4*>(&rbr[u * BE]);
+              dst[0] = bw[0], dst[1] = bw[1];
+            } else {
 #pragma unroll
-            for (int e = 0; e < BE; ++e) rbr[u * BE + e] = kok[u] && r0 + e < g.k ? bp[e] : BT{};
+              for (int e = 0; e < BE; ++e) rbr[u * BE + e] = kok[u] && r0 + e < g.k ? bp[e] : BT{};
+            }
           }
         } else if constexpr (BMODE == 1) {
           const BT* bp = Braw + bco;
@@ -1238,20 +1245,58 @@ __global__ __launch_bounds__(256, BM == 32 ? 3 : 2) void gemm_presplit_kernel(Ge
           __syncthreads();
         }
       }
+      // Epilogue through LDS (free after the loop's last barrier): the tile goes
+      // to a row-major [BM][BN] float array, then every thread writes 16-byte row
+      // pieces, so whole cache lines reach HBM (per-lane 4-byte stores at an 8-byte
+      // stride wrote complex64 results at about half the device's write rate).
+      // The column index is XOR-swizzled by row so that the four 16-lane groups
+      // of one store, 4 rows apart, hit distinct banks.
+      float* const cs = reinterpret_cast<float*>(smem_split);
+      auto cswz = [](int tr) {
+        return CPLX ? ((tr >> 2) & 1) | (((tr >> 3) & 1) << 5) : ((tr >> 2) & 3) << 4;
+      };
+      {
+        // rows wm0 + 16 i + 4 (lane / 16) + r: their swizzle depends on the lane only
+        const int g4 = lane >> 4;
+        const int xr = CPLX ? (g4 & 1) | ((g4 >> 1) << 5) : g4 << 4;
+        float* const crow = cs + (wm0 + 4 * g4) * BN;
 #pragma unroll
-      for (int i = 0; i < TM; ++i)
+        for (int j = 0; j < TN; ++j) {
+          const int lrow = wn0 + j * 16 + cl;  // LDS row of B' = tile column
+          const int tc = (CPLX ? 2 * (lrow & 63) + (lrow >> 6) : lrow) ^ xr;
 #pragma unroll
-        for (int j = 0; j < TN; ++j)
+          for (int i = 0; i < TM; ++i)
 #pragma unroll
-          for (int r = 0; r < 4; ++r) {
-            const int lrow = wn0 + j * 16 + cl;  // LDS row of B' = tile column
-            const int64_t row = m0 + wm0 + i * 16 + 4 * (lane >> 4) + r;
-            const int64_t col = CPLX ? n0 + 2 * (lrow & 63) + (lrow >> 6) : n0 + lrow;
-            if (row < M && col < N) {
-              C[row * ldc + col] = acc[i][j][r] * unscale;
-              if (g.cmax) fold_max(mx, acc[i][j][r] * unscale);
-            }
+            for (int r = 0; r < 4; ++r) crow[(i * 16 + r) * BN + tc] = acc[i][j][r];
+        }
+      }
+      __syncthreads();
+      const bool c16 = ((uintptr_t)C & 15) == 0 && (ldc & 3) == 0;
+      constexpr int C_Q = BM * BN / 4 / 256;  // 16-byte pieces per thread
+#pragma unroll 2
+      for (int u = 0; u < C_Q; ++u) {
+        const int q = tid + 256 * u, tr = q / (BN / 4), c4 = (q % (BN / 4)) * 4;
+        const int sw = cswz(tr);
+        float4 v = *reinterpret_cast<const float4*>(&cs[tr * BN + (c4 ^ (sw & ~3))]);
+        if (sw & 1) v = make_float4(v.y, v.x, v.w, v.z);
+        const int64_t row = m0 + tr, col = n0 + c4;
+        if (row < M) {
+          v.x *= unscale, v.y *= unscale, v.z *= unscale, v.w *= unscale;
+          if (c16 && col + 3 < N) {
+            *reinterpret_cast<float4*>(C + row * ldc + col) = v;
+            if (g.cmax) fold_max(mx, v.x), fold_max(mx, v.y), fold_max(mx, v.z), fold_max(mx, v.w);
+          } else {
+            const float e4[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+            for (int e = 0; e < 4; ++e)
+              if (col + e < N) {
+                C[row * ldc + col + e] = e4[e];
+                if (g.cmax) fold_max(mx, e4[e]);
+              }
           }
+        }
+      }
+      __syncthreads();  // the next tile stages into the same LDS
     }
   }
   if (g.cmax) publish_max((double)mx, g.cmax);
