@@ -1046,6 +1046,40 @@ __global__ __launch_bounds__(256, BM == 32 ? 3 : 2) void gemm_presplit_kernel(Ge
   const int wm0 = BM == 128 ? (wave >> 1) * WM : 0, wn0 = BM == 128 ? (wave & 1) * WN : wave * WN;
   const uint64_t nz = (uint64_t)g.nbo * g.nbi, ntile = (uint64_t)tiles_m * tiles_n;
   const int64_t ainner = g.sai ? g.nbi : 1, binner = g.sbi ? g.nbi : 1;
+  // Gather offsets of the B rows a thread loads in a k tile (the same in every
+  // tile and batch entry): read one k tile ahead of their B loads, and those of
+  // the first two k tiles once per workgroup where registers allow. Vector
+  // loads, also where the row is wave-uniform: a scalar load in flight would
+  // make every LDS wait of the MFMA loop wait for it as well (scalar and LDS
+  // reads share one counter, and scalar ones return out of order), which cost a
+  // third of the kernel's rate on gathered operands.
+  constexpr int NR = BMODE == 2 ? B_Q : BMODE == 1 ? B_Q * BE : 1;
+  struct RowOffs {
+    int32_t v[NR];
+  };
+  auto rows_at = [&](int64_t k0) {
+    RowOffs r{};
+    if constexpr (BMODE == 2) {
+      if (g.bcol)
+#pragma unroll
+        for (int u = 0; u < B_Q; ++u) {
+          const int64_t r0 = (k0 + ((tid + 256 * u) % CH) * 8) / (CPLX ? 2 : 1);
+          r.v[u] = r0 < g.k ? g.brow[r0] : 0;
+        }
+    } else if constexpr (BMODE == 1) {
+      if (g.bcol)
+#pragma unroll
+        for (int u = 0; u < B_Q; ++u) {
+          const int64_t r0 = (k0 + (int64_t)((tid + 256 * u) / BC) * 8) / (CPLX ? 2 : 1);
+#pragma unroll
+          for (int e = 0; e < BE; ++e) r.v[u * BE + e] = r0 + e < g.k ? g.brow[r0 + e] : 0;
+        }
+    }
+    return r;
+  };
+  constexpr bool HOIST = BRAW && NR <= 8 && (BM == 32 || (BMODE == 2 && CPLX));
+  RowOffs rr0{}, rr1{};
+  if constexpr (HOIST) rr0 = rows_at(0), rr1 = rows_at(BK);
 
   for (uint64_t z = blockIdx.y; z < nz; z += gridDim.y) {
     const int64_t zo = (int64_t)(z / g.nbi), zi = (int64_t)(z - zo * g.nbi);
@@ -1082,31 +1116,7 @@ __global__ __launch_bounds__(256, BM == 32 ? 3 : 2) void gemm_presplit_kernel(Ge
           kco[u] = kok[u] ? b_col(g, j) : 0;
         }
       }
-      // Gather offsets of the B rows of a k tile, read one tile ahead of its B
-      // loads. Vector loads, also where the row is wave-uniform: a scalar load in
-      // flight would make every LDS wait of the MFMA loop wait for it as well
-      // (scalar and LDS reads share one counter, and scalar ones return out of
-      // order), which cost a third of the kernel's rate on gathered operands.
-      constexpr int NR = BMODE == 2 ? B_Q : BMODE == 1 ? B_Q * BE : 1;
-      int32_t rro[NR];
-      auto fetch_rows = [&](int64_t k0) {
-        if (!g.bcol) return;
-        if constexpr (BMODE == 2) {
-#pragma unroll
-          for (int u = 0; u < B_Q; ++u) {
-            const int64_t r0 = (k0 + ((tid + 256 * u) % CH) * 8) / (CPLX ? 2 : 1);
-            rro[u] = kok[u] && r0 < g.k ? g.brow[r0] : 0;
-          }
-        } else if constexpr (BMODE == 1) {
-#pragma unroll
-          for (int u = 0; u < B_Q; ++u) {
-            const int64_t r0 = (k0 + (int64_t)((tid + 256 * u) / BC) * 8) / (CPLX ? 2 : 1);
-#pragma unroll
-            for (int e = 0; e < BE; ++e) rro[u * BE + e] = r0 + e < g.k ? g.brow[r0 + e] : 0;
-          }
-        }
-      };
-      auto load = [&](int64_t k0) {
+      auto load = [&](int64_t k0, const RowOffs& ro) {
 #pragma unroll
         for (int u = 0; u < A_Q; ++u) {
           const int q = tid + 256 * u, row = q / CH, ch = (q % CH) * 8;
@@ -1121,7 +1131,7 @@ __global__ __launch_bounds__(256, BM == 32 ? 3 : 2) void gemm_presplit_kernel(Ge
             // BE consecutive k of one column: within one run of consecutive row
             // offsets (brow_run is a multiple of BE, chunks start at multiples of BE)
             const int64_t r0 = (k0 + ((tid + 256 * u) % CH) * 8) / (CPLX ? 2 : 1);
-            const BT* bp = Braw + kco[u] + (g.bcol ? (int64_t)rro[u] : kok[u] && r0 < g.k ? r0 * g.ldb : 0);
+            const BT* bp = Braw + kco[u] + (g.bcol ? (int64_t)ro.v[u] : kok[u] && r0 < g.k ? r0 * g.ldb : 0);
             static_assert(BE * sizeof(BT) == 32, "a chunk is 32 bytes of B");
             if (kok[u] && r0 + BE <= g.k && ((uintptr_t)bp & 15) == 0) {  // two 16-byte loads
               const uint4* bw = reinterpret_cast<const uint4*>(bp);
@@ -1143,7 +1153,7 @@ __global__ __launch_bounds__(256, BM == 32 ? 3 : 2) void gemm_presplit_kernel(Ge
 #pragma unroll
             for (int e = 0; e < BE; ++e)
               rbr[u * BE + e] =
-                  bcn_ok && r0 + e < g.k ? bp[g.bcol ? (int64_t)rro[u * BE + e] : (r0 + e) * g.ldb] : BT{};
+                  bcn_ok && r0 + e < g.k ? bp[g.bcol ? (int64_t)ro.v[u * BE + e] : (r0 + e) * g.ldb] : BT{};
           }
         } else {
 #pragma unroll
@@ -1199,15 +1209,20 @@ __global__ __launch_bounds__(256, BM == 32 ? 3 : 2) void gemm_presplit_kernel(Ge
       };
       const int64_t nk = (kp + BK - 1) / BK;
       const int cl = lane & 15;
-      fetch_rows(0);
-      load(0);
-      if (nk > 1) fetch_rows(BK);
+      RowOffs rn{};  // row offsets of k tile kt + 1
+      if constexpr (HOIST) {
+        load(0, rr0);
+        rn = rr1;
+      } else {
+        load(0, rows_at(0));
+        if (nk > 1) rn = rows_at(BK);
+      }
       store();
       __syncthreads();
       for (int64_t kt = 0; kt < nk; ++kt) {
         if (kt + 1 < nk) {
-          load((kt + 1) * BK);
-          if (kt + 2 < nk) fetch_rows((kt + 2) * BK);
+          load((kt + 1) * BK, rn);
+          if (kt + 2 < nk) rn = rows_at((kt + 2) * BK);
         }
 #pragma unroll
         for (int ks = 0; ks < BK; ks += 32) {
@@ -1532,14 +1547,14 @@ static PreSplitSizes presplit_sizes(int dt, const GemmArgs& g) {
 // Split-precision products pay off on MFMA-bound shapes only; memory-bound
 // skinny steps run the FP32 kernels (same time, full FP32 accuracy).
 // Split products pay off with 128-row tiles from m = 64 and with 32-row tiles
-// for 8 <= m <= 32 (measured on gfx950, complex64 x256 with contiguous B, BF16x3
-// against the 32-row FP32 MFMA kernel: m=32 k=128 n=131072 9.0 vs 13.0 ms, m=8
-// k=256 n=32768 3.6 vs 5.3 ms; at k=16 the two tie, both bound by the result
-// write).
+// for 8 <= m <= 32 down to 32 real k (measured on gfx950, FP16x2 on complex64
+// x256 read through gather tables, against the 32-row FP32 MFMA kernel: m=32
+// k=128 n=131072 8.7 vs 13.0 ms, m=8 k=256 n=32768 3.7 vs 6.0 ms, m=16 k=16
+// n=262144 5.3 vs 5.9 ms; at m=32 k=8 the FP32 kernel is the faster).
 static bool split_worth(int dt, const GemmArgs& g) {
   const bool cplx = dtype_complex(dt);
   const int64_t K = cplx ? 2 * g.k : g.k, N = cplx ? 2 * g.n : g.n;
-  return (g.m >= 64 || (g.m >= 8 && g.m <= 32)) && N >= 128 && K >= 64;
+  return N >= 128 && ((g.m >= 64 && K >= 64) || (g.m >= 8 && g.m <= 32 && K >= 32));
 }
 
 static int native_mfma_kernel(int dt, int64_t m) {
