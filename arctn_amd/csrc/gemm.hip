@@ -520,6 +520,25 @@ __global__ __launch_bounds__(256) void gemm_mfma_kernel(GemmArgs g) {
   // k range of this blockIdx.z (real units; kc is a multiple of BK)
   const int64_t kbeg = g.kc ? (int64_t)blockIdx.z * g.kc * (CPLX ? 2 : 1) : 0;
   const int64_t kend = g.kc && kbeg + g.kc * (CPLX ? 2 : 1) < K ? kbeg + g.kc * (CPLX ? 2 : 1) : K;
+  // Gather offsets of the B rows a thread loads in a k tile (the same in every
+  // tile): those of the first two k tiles read once, the others one k tile
+  // ahead of their B loads. Vector loads: a scalar load in flight would hold up
+  // the LDS waits of the MFMA loop.
+  struct RowOffs {
+    int32_t v[B_PER];
+  };
+  auto rows_at = [&](int64_t k0) {
+    RowOffs r{};
+    if (g.bcol)
+#pragma unroll
+      for (int u = 0; u < B_PER; ++u) {
+        const int e = tid + 256 * u;
+        const int64_t gr = CPLX ? k0 / 2 + e / (BN / 2) : k0 + e / BN;
+        r.v[u] = gr < (CPLX ? kend / 2 : kend) ? g.brow[gr] : 0;
+      }
+    return r;
+  };
+  const RowOffs rr0 = rows_at(kbeg), rr1 = rows_at(kbeg + BK);
 
   for (uint64_t z = blockIdx.y; z < nz; z += gridDim.y) {
     uint64_t zo = z / g.nbi, zi = z - zo * g.nbi;
@@ -544,19 +563,7 @@ __global__ __launch_bounds__(256) void gemm_mfma_kernel(GemmArgs g) {
       const int64_t bcn = CPLX ? n0 / 2 + tid % (BN / 2) : n0 + tid % BN;
       const bool bcn_ok = bcn < (CPLX ? g.n : N);
       const int64_t bco = bcn_ok ? b_col(g, bcn) : 0;
-      // gather offsets of the B rows of a k tile, read one tile ahead of its B
-      // loads (vector loads: a scalar load in flight would hold up the LDS waits)
-      int32_t rro[B_PER];
-      auto fetch_rows = [&](int64_t k0) {
-        if (!g.bcol) return;
-#pragma unroll
-        for (int u = 0; u < B_PER; ++u) {
-          const int e = tid + 256 * u;
-          const int64_t gr = CPLX ? k0 / 2 + e / (BN / 2) : k0 + e / BN;
-          rro[u] = gr < (CPLX ? kend / 2 : kend) ? g.brow[gr] : 0;
-        }
-      };
-      auto load = [&](int64_t k0) {
+      auto load = [&](int64_t k0, const RowOffs& ro) {
 #pragma unroll
         for (int u = 0; u < A_PER; ++u) {
           int e = tid + 256 * u, col = e % BK, row = e / BK;
@@ -569,7 +576,7 @@ __global__ __launch_bounds__(256) void gemm_mfma_kernel(GemmArgs g) {
             int e = tid + 256 * u, p = e / (BN / 2);
             int64_t gp = k0 / 2 + p;
             bool ok = gp < kend / 2 && bcn_ok;
-            const int64_t bi = ok ? ((g.bcol ? (int64_t)rro[u] : gp * g.ldb) + bco) * 2 : 0;
+            const int64_t bi = ok ? ((g.bcol ? (int64_t)ro.v[u] : gp * g.ldb) + bco) * 2 : 0;
             rb[2 * u] = ok ? B[bi] : (F)0;
             rb[2 * u + 1] = ok ? B[bi + 1] : (F)0;
           }
@@ -578,7 +585,7 @@ __global__ __launch_bounds__(256) void gemm_mfma_kernel(GemmArgs g) {
           for (int u = 0; u < B_PER; ++u) {
             int e = tid + 256 * u, row = e / BN;
             int64_t gr = k0 + row;
-            rb[u] = (gr < kend && bcn_ok) ? B[(g.bcol ? (int64_t)rro[u] : gr * g.ldb) + bco] : (F)0;
+            rb[u] = (gr < kend && bcn_ok) ? B[(g.bcol ? (int64_t)ro.v[u] : gr * g.ldb) + bco] : (F)0;
           }
         }
       };
@@ -608,15 +615,14 @@ __global__ __launch_bounds__(256) void gemm_mfma_kernel(GemmArgs g) {
       };
 
       const int64_t nk = (kend - kbeg + BK - 1) / BK;
-      fetch_rows(kbeg);
-      load(kbeg);
-      if (nk > 1) fetch_rows(kbeg + BK);
+      load(kbeg, rr0);
+      RowOffs rn = rr1;  // row offsets of k tile kt + 1
       store();
       __syncthreads();
       for (int64_t kt = 0; kt < nk; ++kt) {
         if (kt + 1 < nk) {
-          load(kbeg + (kt + 1) * BK);
-          if (kt + 2 < nk) fetch_rows(kbeg + (kt + 2) * BK);
+          load(kbeg + (kt + 1) * BK, rn);
+          if (kt + 2 < nk) rn = rows_at(kbeg + (kt + 2) * BK);
         }
 #pragma unroll
         for (int kk = 0; kk < BK; kk += 4) {
