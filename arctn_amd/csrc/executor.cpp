@@ -776,8 +776,10 @@ void Executor::autotune(const Ctx& c, hipStream_t s) {
     const bool split_prec = o_.precision != NATIVE;
     const double macs = (double)q.nbo * (double)q.nbi * (double)q.m * (double)q.n * (double)q.k;
     const double outs = (double)q.nbo * (double)q.nbi * (double)q.m * (double)q.n;
-    for (int cand : {GEMM_SIMPLE, GEMM_SKINNY, GEMM_SPLITK, GEMM_PRESPLIT, GEMM_PRESPLIT_A, GEMM_MFMA, GEMM_MFMA_R32,
-                     GEMM_MFMA_R64, GEMM_MFMA_R128}) {
+    // the heuristic's own choice first: another kernel replaces it only when at
+    // least 3% faster, so near ties do not flip from run to run
+    for (int cand : {GEMM_AUTO, GEMM_SIMPLE, GEMM_SKINNY, GEMM_SPLITK, GEMM_PRESPLIT, GEMM_PRESPLIT_A, GEMM_MFMA,
+                     GEMM_MFMA_R32, GEMM_MFMA_R64, GEMM_MFMA_R128}) {
       // candidates that cannot win are not timed (the one-thread-per-output kernel
       // on large products, split-K with many outputs, row tiles far from m)
       if (cand == GEMM_SIMPLE && macs > (double)(1 << 26)) continue;
@@ -800,6 +802,7 @@ void Executor::autotune(const Ctx& c, hipStream_t s) {
       ATN_HIP(hipEventSynchronize(e1));
       float ms = 0;
       ATN_HIP(hipEventElapsedTime(&ms, e0, e1));
+      if (cand == GEMM_AUTO) ms *= 0.97f;
       if (ms < best_ms) best_ms = ms, choice = cand;
     }
     gemm_choice_[i] = best[key] = choice;
