@@ -57,13 +57,13 @@ template <> struct CM<double2> {
 // Result max-magnitude tracking for the strip (GemmArgs::cmax): each thread
 // folds max |re|, |im| of the values it stores, one atomicMax per wave at the
 // end. Non-negative doubles order like their bit patterns; NaN never wins.
-// Grid bound of kernels that track the result max where A is one row tile
-// (persistent loops over column tiles). With several row tiles the full grid
-// runs: the workgroups in flight then hold all row tiles of a range of column
-// tiles of one batch entry, which share each B panel in L2, where a capped grid
-// spreads them over many batch entries (complex64 m=1024 k=512 n=8192 x256:
-// 24.2 ms full, 26.9 capped; on one row tile the cap is the faster, 6.6 against
-// 7.5 ms on m=16 k=16 n=262144 x256).
+// Grid bound of kernels that track the result max (persistent loops over
+// tiles). The split-precision GEMMs use it only where A is one row tile: with
+// several row tiles their full grid holds all row tiles of a range of column
+// tiles of one batch entry in flight, which share each B panel in L2, where a
+// capped grid spreads them over many batch entries (complex64 m=1024 k=512
+// n=8192 x256: 24.2 ms full, 26.9 capped; on one row tile the cap is the faster,
+// 6.6 against 7.5 ms on m=16 k=16 n=262144 x256).
 constexpr unsigned TRACK_WORKGROUPS = 8192;
 
 // The running max is kept in the value's own precision (float compares for
@@ -492,8 +492,10 @@ template <> struct Mfma<double> {
 //   D[4*(l/16) + r][l%16] for the f32 instruction and D[4*r + l/16][l%16] for
 //   the f64 instruction.
 // TRACK: fold the result max into g.cmax (a separate instantiation, so the
-// plain kernel carries no extra live register or epilogue work).
-template <class F, bool CPLX, int BM, int BN, int BK, bool TRACK>
+// plain kernel carries no extra live register or epilogue work). GATHER: B is
+// read through the gather tables (a separate instantiation: the row-offset
+// loads in the k loop cost the row-major kernel a fifth of its rate).
+template <class F, bool CPLX, int BM, int BN, int BK, bool TRACK, bool GATHER>
 __global__ __launch_bounds__(256) void gemm_mfma_kernel(GemmArgs g) {
   constexpr int WM = BM / 2, WN = BN / 2;
   constexpr int TM = WM / 16, TN = WN / 16;
@@ -529,7 +531,7 @@ __global__ __launch_bounds__(256) void gemm_mfma_kernel(GemmArgs g) {
   };
   auto rows_at = [&](int64_t k0) {
     RowOffs r{};
-    if (g.bcol)
+    if constexpr (GATHER)
 #pragma unroll
       for (int u = 0; u < B_PER; ++u) {
         const int e = tid + 256 * u;
@@ -538,7 +540,8 @@ __global__ __launch_bounds__(256) void gemm_mfma_kernel(GemmArgs g) {
       }
     return r;
   };
-  const RowOffs rr0 = rows_at(kbeg), rr1 = rows_at(kbeg + BK);
+  RowOffs rr0{}, rr1{};
+  if constexpr (GATHER) rr0 = rows_at(kbeg), rr1 = rows_at(kbeg + BK);
 
   for (uint64_t z = blockIdx.y; z < nz; z += gridDim.y) {
     uint64_t zo = z / g.nbi, zi = z - zo * g.nbi;
@@ -562,7 +565,7 @@ __global__ __launch_bounds__(256) void gemm_mfma_kernel(GemmArgs g) {
       // every B element a thread loads lies in one column (256 is a multiple of BN)
       const int64_t bcn = CPLX ? n0 / 2 + tid % (BN / 2) : n0 + tid % BN;
       const bool bcn_ok = bcn < (CPLX ? g.n : N);
-      const int64_t bco = bcn_ok ? b_col(g, bcn) : 0;
+      const int64_t bco = bcn_ok ? (GATHER ? (int64_t)g.bcol[bcn] : bcn) : 0;
       auto load = [&](int64_t k0, const RowOffs& ro) {
 #pragma unroll
         for (int u = 0; u < A_PER; ++u) {
@@ -576,7 +579,7 @@ __global__ __launch_bounds__(256) void gemm_mfma_kernel(GemmArgs g) {
             int e = tid + 256 * u, p = e / (BN / 2);
             int64_t gp = k0 / 2 + p;
             bool ok = gp < kend / 2 && bcn_ok;
-            const int64_t bi = ok ? ((g.bcol ? (int64_t)ro.v[u] : gp * g.ldb) + bco) * 2 : 0;
+            const int64_t bi = ok ? ((GATHER ? (int64_t)ro.v[u] : gp * g.ldb) + bco) * 2 : 0;
             rb[2 * u] = ok ? B[bi] : (F)0;
             rb[2 * u + 1] = ok ? B[bi + 1] : (F)0;
           }
@@ -585,7 +588,7 @@ __global__ __launch_bounds__(256) void gemm_mfma_kernel(GemmArgs g) {
           for (int u = 0; u < B_PER; ++u) {
             int e = tid + 256 * u, row = e / BN;
             int64_t gr = k0 + row;
-            rb[u] = (gr < kend && bcn_ok) ? B[(g.bcol ? (int64_t)ro.v[u] : gr * g.ldb) + bco] : (F)0;
+            rb[u] = (gr < kend && bcn_ok) ? B[(GATHER ? (int64_t)ro.v[u] : gr * g.ldb) + bco] : (F)0;
           }
         }
       };
@@ -622,7 +625,8 @@ __global__ __launch_bounds__(256) void gemm_mfma_kernel(GemmArgs g) {
       for (int64_t kt = 0; kt < nk; ++kt) {
         if (kt + 1 < nk) {
           load(kbeg + (kt + 1) * BK, rn);
-          if (kt + 2 < nk) rn = rows_at(kbeg + (kt + 2) * BK);
+          if constexpr (GATHER)
+            if (kt + 2 < nk) rn = rows_at(kbeg + (kt + 2) * BK);
         }
 #pragma unroll
         for (int kk = 0; kk < BK; kk += 4) {
@@ -1013,16 +1017,19 @@ __device__ inline int swz(int row, int ch) {
 // lanes along k (chunk q = column q / CH, k chunk q % CH, as for pre-split B: for
 // operands whose contracted legs are innermost in memory, GemmArgs::brow_run).
 // BM: rows of A per tile. 128: four 64x64 wave tiles (two workgroups per CU);
-// 32: four 32x32 wave tiles side by side (40 KiB of LDS, four workgroups per
+// 32: four 32x32 wave tiles side by side (40 KiB of LDS, three workgroups per
 // CU) for m <= 32, where B streams from HBM once and the MFMA work per B byte
-// is a quarter of the 128-row tile's.
-template <bool CPLX, bool F16, int BK, int BMODE, int BM>
+// is a quarter of the 128-row tile's. GATHER: raw B read through the gather
+// tables (always with lanes along k; a separate instantiation with lanes along
+// columns, so that row-major B carries no row-offset loads).
+template <bool CPLX, bool F16, int BK, int BMODE, int BM, bool GATHER>
 __global__ __launch_bounds__(256, BM == 32 ? 3 : 2) void gemm_presplit_kernel(GemmArgs g, PreSplit ps) {
   constexpr bool BRAW = BMODE != 0;
   using H = typename Half<F16>::H;
   using V = typename Half<F16>::V;
   static_assert(BK == 32 || BK == 64, "k tile");
   static_assert(BM == 128 || (BM == 32 && BK == 64), "row tile");
+  static_assert(GATHER == (BMODE == 2) || BMODE == 1, "lanes along k read gathered B; pre-split B no tables");
   constexpr int BN = 128, CH = BK / 8;
   constexpr int WM = BM == 128 ? 64 : 32, WN = WM, TM = WM / 16, TN = WN / 16;
   constexpr int A_Q = BM * CH / 256;                    // 16-byte A chunks per thread
@@ -1066,14 +1073,12 @@ __global__ __launch_bounds__(256, BM == 32 ? 3 : 2) void gemm_presplit_kernel(Ge
   auto rows_at = [&](int64_t k0) {
     RowOffs r{};
     if constexpr (BMODE == 2) {
-      if (g.bcol)
 #pragma unroll
         for (int u = 0; u < B_Q; ++u) {
           const int64_t r0 = (k0 + ((tid + 256 * u) % CH) * 8) / (CPLX ? 2 : 1);
           r.v[u] = r0 < g.k ? g.brow[r0] : 0;
         }
-    } else if constexpr (BMODE == 1) {
-      if (g.bcol)
+    } else if constexpr (BMODE == 1 && GATHER) {
 #pragma unroll
         for (int u = 0; u < B_Q; ++u) {
           const int64_t r0 = (k0 + (int64_t)((tid + 256 * u) / BC) * 8) / (CPLX ? 2 : 1);
@@ -1083,7 +1088,7 @@ __global__ __launch_bounds__(256, BM == 32 ? 3 : 2) void gemm_presplit_kernel(Ge
     }
     return r;
   };
-  constexpr bool HOIST = BRAW && NR <= 8 && (BM == 32 || (BMODE == 2 && CPLX));
+  constexpr bool HOIST = GATHER && NR <= 8 && (BM == 32 || (BMODE == 2 && CPLX));
   RowOffs rr0{}, rr1{};
   if constexpr (HOIST) rr0 = rows_at(0), rr1 = rows_at(BK);
 
@@ -1111,7 +1116,7 @@ __global__ __launch_bounds__(256, BM == 32 ? 3 : 2) void gemm_presplit_kernel(Ge
       // offset hoisted out of the k loop
       const int64_t bcn = (CPLX ? n0 / 2 : n0) + tid % BC;
       const bool bcn_ok = BMODE == 1 && bcn < g.n;
-      const int64_t bco = bcn_ok ? b_col(g, bcn) : 0;
+      const int64_t bco = bcn_ok ? (GATHER ? (int64_t)g.bcol[bcn] : bcn) : 0;
       int64_t kco[BMODE == 2 ? B_Q : 1];
       bool kok[BMODE == 2 ? B_Q : 1];
       if constexpr (BMODE == 2) {
@@ -1119,7 +1124,7 @@ __global__ __launch_bounds__(256, BM == 32 ? 3 : 2) void gemm_presplit_kernel(Ge
         for (int u = 0; u < B_Q; ++u) {
           const int64_t j = (CPLX ? n0 / 2 : n0) + (tid + 256 * u) / CH;
           kok[u] = j < g.n;
-          kco[u] = kok[u] ? b_col(g, j) : 0;
+          kco[u] = kok[u] ? (int64_t)g.bcol[j] : 0;
         }
       }
       auto load = [&](int64_t k0, const RowOffs& ro) {
@@ -1137,7 +1142,7 @@ __global__ __launch_bounds__(256, BM == 32 ? 3 : 2) void gemm_presplit_kernel(Ge
             // BE consecutive k of one column: within one run of consecutive row
             // offsets (brow_run is a multiple of BE, chunks start at multiples of BE)
             const int64_t r0 = (k0 + ((tid + 256 * u) % CH) * 8) / (CPLX ? 2 : 1);
-            const BT* bp = Braw + kco[u] + (g.bcol ? (int64_t)ro.v[u] : kok[u] && r0 < g.k ? r0 * g.ldb : 0);
+            const BT* bp = Braw + kco[u] + ro.v[u];
             static_assert(BE * sizeof(BT) == 32, "a chunk is 32 bytes of B");
             if (kok[u] && r0 + BE <= g.k && ((uintptr_t)bp & 15) == 0) {  // two 16-byte loads
               const uint4* bw = reinterpret_cast<const uint4*>(bp);
@@ -1159,7 +1164,7 @@ __global__ __launch_bounds__(256, BM == 32 ? 3 : 2) void gemm_presplit_kernel(Ge
 #pragma unroll
             for (int e = 0; e < BE; ++e)
               rbr[u * BE + e] =
-                  bcn_ok && r0 + e < g.k ? bp[g.bcol ? (int64_t)ro.v[u * BE + e] : (r0 + e) * g.ldb] : BT{};
+                  bcn_ok && r0 + e < g.k ? bp[GATHER ? (int64_t)ro.v[u * BE + e] : (r0 + e) * g.ldb] : BT{};
           }
         } else {
 #pragma unroll
@@ -1221,14 +1226,16 @@ __global__ __launch_bounds__(256, BM == 32 ? 3 : 2) void gemm_presplit_kernel(Ge
         rn = rr1;
       } else {
         load(0, rows_at(0));
-        if (nk > 1) rn = rows_at(BK);
+        if constexpr (GATHER)
+          if (nk > 1) rn = rows_at(BK);
       }
       store();
       __syncthreads();
       for (int64_t kt = 0; kt < nk; ++kt) {
         if (kt + 1 < nk) {
           load((kt + 1) * BK, rn);
-          if (kt + 2 < nk) rn = rows_at((kt + 2) * BK);
+          if constexpr (GATHER)
+            if (kt + 2 < nk) rn = rows_at((kt + 2) * BK);
         }
 #pragma unroll
         for (int ks = 0; ks < BK; ks += 32) {
@@ -1363,13 +1370,20 @@ static void run_mfma(int dt, const GemmArgs& g0, hipStream_t s) {
   int64_t N = CPLX ? 2 * g.n : g.n;
   uint64_t tiles = (uint64_t)((g.m + BM - 1) / BM) * (uint64_t)((N + BN - 1) / BN);
   dim3 grid((unsigned)std::min<uint64_t>(tiles, 1u << 20), zgrid(g), (unsigned)ks.n_split);
-  // tracking the result max over one row tile: a persistent grid
-  if (g.cmax && g.m <= BM)
-    grid.x = std::min<unsigned>(grid.x, std::max<unsigned>(1, TRACK_WORKGROUPS / (grid.y * grid.z)));
+  // tracking the result max: a persistent grid (with several row tiles too: on
+  // the uneven 3-regular workload, whose every GEMM folds its max, the full grid
+  // ran 6% slower)
+  if (g.cmax) grid.x = std::min<unsigned>(grid.x, std::max<unsigned>(1, TRACK_WORKGROUPS / (grid.y * grid.z)));
   g_last_kind = sizeof(F) == 4 ? K_GEMM_MFMA_F32 : K_GEMM_MFMA_F64;
   // with split k the reduce kernel tracks the result max
-  if (g.cmax && !g.part) LAUNCH((gemm_mfma_kernel<F, CPLX, BM, BN, BK, true>), grid, dim3(256), 0, s, g);
-  else LAUNCH((gemm_mfma_kernel<F, CPLX, BM, BN, BK, false>), grid, dim3(256), 0, s, g);
+  const bool track = g.cmax && !g.part;
+  if (g.bcol) {
+    if (track) LAUNCH((gemm_mfma_kernel<F, CPLX, BM, BN, BK, true, true>), grid, dim3(256), 0, s, g);
+    else LAUNCH((gemm_mfma_kernel<F, CPLX, BM, BN, BK, false, true>), grid, dim3(256), 0, s, g);
+  } else {
+    if (track) LAUNCH((gemm_mfma_kernel<F, CPLX, BM, BN, BK, true, false>), grid, dim3(256), 0, s, g);
+    else LAUNCH((gemm_mfma_kernel<F, CPLX, BM, BN, BK, false, false>), grid, dim3(256), 0, s, g);
+  }
   if (ks.n_split > 1) {
     SplitK sk{};
     sk.n_split = ks.n_split;
@@ -1606,6 +1620,28 @@ int launch_gemm(int dt, int prec, int kernel, const GemmArgs& g, hipStream_t s) 
   return used < 0 ? GEMM_SKINNY : used;
 }
 
+// One pre-split GEMM launch: B pre-split (R = 0), raw B with lanes along columns
+// (R = 1, gathered or row-major) or along k (R = 2, gathered).
+template <bool C, bool F, int R, int M, bool G>
+static void presplit_kernel_launch(const GemmArgs& g, const PreSplit& ps, dim3 grid, hipStream_t s) {
+  static bool attr_set = false;
+  if (!attr_set) {
+    ATN_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&gemm_presplit_kernel<C, F, PRESPLIT_BK, R, M, G>),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)presplit_lds(M)));
+    attr_set = true;
+  }
+  LAUNCH((gemm_presplit_kernel<C, F, PRESPLIT_BK, R, M, G>), grid, dim3(256), presplit_lds(M), s, g, ps);
+}
+template <bool C, bool F, int R, int M>
+static void presplit_gemm(const GemmArgs& g, const PreSplit& ps, dim3 grid, hipStream_t s) {
+  if constexpr (R == 1) {
+    if (g.bcol) presplit_kernel_launch<C, F, 1, M, true>(g, ps, grid, s);
+    else presplit_kernel_launch<C, F, 1, M, false>(g, ps, grid, s);
+  } else {
+    presplit_kernel_launch<C, F, R, M, R == 2>(g, ps, grid, s);
+  }
+}
+
 static int launch_gemm_kernels(int dt, int prec, int kernel, const GemmArgs& g0, hipStream_t s) {
   GemmArgs g = g0;
   if (kernel == GEMM_AUTO) kernel = auto_kernel(dt, prec, g);
@@ -1664,24 +1700,13 @@ static int launch_gemm_kernels(int dt, int prec, int kernel, const GemmArgs& g0,
       const int64_t na = n_mats(g.sao, g.sai, g.nbo, g.nbi) * g.m * psz.kp / 4;
       const int64_t tiles_b = n_mats(g.sbo, g.sbi, g.nbo, g.nbi) * ((psz.kp / (cx ? 2 : 1) + 31) / 32) * ((g.n + 63) / 64);
       const dim3 ga((unsigned)std::min<int64_t>((na + 255) / 256, 65536)), gb((unsigned)std::min<int64_t>(tiles_b, 65536));
-#define PRESPLIT_KERNEL(C_, F_, R_, M_)                                                                    \
-  do {                                                                                                     \
-    static bool attr_set = false;                                                                          \
-    if (!attr_set) {                                                                                       \
-      ATN_HIP(hipFuncSetAttribute(                                                                         \
-          reinterpret_cast<const void*>(&gemm_presplit_kernel<C_, F_, PRESPLIT_BK, R_, M_>),               \
-          hipFuncAttributeMaxDynamicSharedMemorySize, (int)presplit_lds(M_)));                             \
-      attr_set = true;                                                                                     \
-    }                                                                                                      \
-    LAUNCH((gemm_presplit_kernel<C_, F_, PRESPLIT_BK, R_, M_>), grid, dim3(256), presplit_lds(M_), s, g, ps); \
-  } while (0)
 #define PRESPLIT_LAUNCH(C_, F_, R_)                                                                        \
   do {                                                                                                     \
     LAUNCH((split_a_kernel<C_, F_>), ga, dim3(256), 0, s, g, ps);                                         \
     if (R_ == 0) LAUNCH((split_bt_kernel<C_, F_>), gb, dim3(256), 0, s, g, ps);                           \
     g_last_kind = F_ ? K_GEMM_MFMA_FP16X2 : K_GEMM_MFMA_BF16X3;                                           \
-    if (bm == 32) PRESPLIT_KERNEL(C_, F_, R_, 32);                                                         \
-    else PRESPLIT_KERNEL(C_, F_, R_, 128);                                                                 \
+    if (bm == 32) presplit_gemm<C_, F_, R_, 32>(g, ps, grid, s);                                           \
+    else presplit_gemm<C_, F_, R_, 128>(g, ps, grid, s);                                                   \
   } while (0)
 #define PRESPLIT_MODES(R_)                                 \
   do {                                                     \
@@ -1700,7 +1725,6 @@ static int launch_gemm_kernels(int dt, int prec, int kernel, const GemmArgs& g0,
       PRESPLIT_MODES(0);
 #undef PRESPLIT_MODES
 #undef PRESPLIT_LAUNCH
-#undef PRESPLIT_KERNEL
       return GEMM_PRESPLIT;
     }
     int64_t cf = dt == C64 ? 2 : 1;
