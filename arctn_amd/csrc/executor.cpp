@@ -191,7 +191,7 @@ bool Executor::add_gather(const std::vector<size_t>& shape, const std::vector<si
 
 void Executor::gen_gemm(const PairStep& st, size_t nbo, bool va, bool vb, const Ref& a, const Ref& b, const Ref& c,
                         const Ref* cmax, int64_t brow, int64_t bcol, int32_t bcol_run, const Ref* amax,
-                        const Ref* bmax, int32_t brow_run) {
+                        const Ref* bmax, int32_t brow_run, int64_t arow, int64_t acol) {
   GemmArgs g{};
   g.m = (int64_t)st.m, g.k = (int64_t)st.k, g.n = (int64_t)st.n;
   g.lda = g.k, g.ldb = g.n, g.ldc = g.n;
@@ -214,7 +214,7 @@ void Executor::gen_gemm(const PairStep& st, size_t nbo, bool va, bool vb, const 
   const Ref am = amax ? *amax : Ref{}, bm = bmax ? *bmax : Ref{};
   if (amax) use(am);
   if (bmax) use(bm);
-  gemm_sites_.push_back(GemmSite{g, a, b, c, brow, bcol, bcol_run, am, bm});
+  gemm_sites_.push_back(GemmSite{g, a, b, c, brow, bcol, bcol_run, am, bm, arow, acol});
   int dt = o_.dtype, prec = o_.precision;
   use(a);
   use(b);
@@ -241,6 +241,7 @@ void Executor::gen_gemm(const PairStep& st, size_t nbo, bool va, bool vb, const 
         if (am.kind != Ref::NONE) q.amax = (const unsigned long long*)cx.ptr(am);
         if (bm.kind != Ref::NONE) q.bmax = (const unsigned long long*)cx.ptr(bm);
         if (bcol >= 0) q.brow = consts_ + brow, q.bcol = consts_ + bcol;
+        if (acol >= 0) q.arow = consts_ + arow, q.acol = consts_ + acol;
         launch_gemm(dt, prec, gemm_choice_[idx], q, cx.s);
       },
       flops, bytes);
@@ -358,7 +359,26 @@ void Executor::gen_steps(bool body, size_t nb, std::vector<Ref>& slots, std::vec
                                  st.result_legs.begin());
       }
     }
-    Ref a = chain(slots[st.li], shapes[st.li], st.sum_a, a_perm, nbA);
+    // A read through gather tables too (no permute launch) on unbatched steps that
+    // run on the simple kernel: the many tiny steps of a plan's slice-invariant part,
+    // and every step of the plain form. Not in the batched body, whose tail batch
+    // replays the body's buffers and must make the same choices at another batch size.
+    int64_t arow = -1, acol = -1;
+    Ref a;
+    if (gatherable && (!o_.batched || !body) && nbA == 1 && nbC == 1 && st.sum_a.empty() && !o_.strip &&
+        plan_permute(shapes[st.li], a_perm, 1, es).kind != 0) {
+      GemmArgs t{};
+      t.m = (int64_t)st.m, t.k = (int64_t)st.k, t.n = (int64_t)st.n, t.nbo = t.nbi = 1;
+      int32_t cr = 1, rr = 1;
+      if (gemm_runs_simple(dt, o_.precision, o_.gemm_kernel, t) &&
+          add_gather(shapes[st.li], a_perm, a_perm.size() - n_con, arow, acol, cr, rr)) {
+        a = slots[st.li];
+        use(a);
+      } else {
+        arow = acol = -1;
+      }
+    }
+    if (acol < 0) a = chain(slots[st.li], shapes[st.li], st.sum_a, a_perm, nbA);
     int64_t brow = -1, bcol = -1;
     int32_t bcol_run = 1, brow_run = 1;
     Ref b;
@@ -380,7 +400,7 @@ void Executor::gen_steps(bool body, size_t nb, std::vector<Ref>& slots, std::vec
       Ref mb{Ref::WS, maxbits_buf_};
       use(mb);
       emit([=](const Ctx& cx) { launch_fill_zero(cx.ptr(mb), 8, cx.s); });
-      gen_gemm(st, nbC, va, vb, a, b, c, &mb, brow, bcol, bcol_run, am, bm, brow_run);
+      gen_gemm(st, nbC, va, vb, a, b, c, &mb, brow, bcol, bcol_run, am, bm, brow_run, arow, acol);
       size_t numel = nbC * st.bt * st.m * st.n;
       Ref ex{Ref::WS, expo_buf_};
       const int keep = si + 1 == P.cp.steps.size() ? 0 : STRIP_KEEP;
@@ -392,10 +412,10 @@ void Executor::gen_steps(bool body, size_t nb, std::vector<Ref>& slots, std::vec
       Ref mc{Ref::WS, newbuf(8)};
       use(mc);
       emit([=](const Ctx& cx) { launch_fill_zero(cx.ptr(mc), 8, cx.s); });
-      gen_gemm(st, nbC, va, vb, a, b, c, &mc, brow, bcol, bcol_run, am, bm, brow_run);
+      gen_gemm(st, nbC, va, vb, a, b, c, &mc, brow, bcol, bcol_run, am, bm, brow_run, arow, acol);
       slot_max_[st.out] = mc;
     } else {
-      gen_gemm(st, nbC, va, vb, a, b, c, nullptr, brow, bcol, bcol_run, am, bm, brow_run);
+      gen_gemm(st, nbC, va, vb, a, b, c, nullptr, brow, bcol, bcol_run, am, bm, brow_run, arow, acol);
     }
     slots[st.out] = c;
     shapes[st.out].resize(oc.size());
@@ -756,6 +776,7 @@ void Executor::autotune(const Ctx& c, hipStream_t s) {
   ATN_HIP(hipEventCreate(&e1));
   for (size_t i = 0; i < gemm_sites_.size(); ++i) {
     auto& site = gemm_sites_[i];
+    if (site.acol >= 0) continue;  // A read through gather tables: the simple kernel
     // shared (stride-0) operands change the memory traffic: part of the key
     const int shared = (site.g.sao == 0) | (site.g.sai == 0) << 1 | (site.g.sbo == 0) << 2 | (site.g.sbi == 0) << 3;
     auto key = std::make_tuple(site.g.m, site.g.k, site.g.n, site.g.nbo, site.g.nbi, shared);

@@ -98,9 +98,12 @@ class Executor {
   // the FP16X2 scale of a later product); brow / bcol: gather tables of B in
   // consts_ (-1: B is a row-major [k, n] operand); amax / bmax: slots holding the
   // operands' max magnitudes (FP16X2), or null
+  // arow / acol: gather tables of A in consts_ (-1: A is a row-major [m, k] operand;
+  // set only where the GEMM runs on the simple kernel)
   void gen_gemm(const PairStep& st, size_t nb_outer, bool va, bool vb, const Ref& a, const Ref& b, const Ref& c,
                 const Ref* cmax = nullptr, int64_t brow = -1, int64_t bcol = -1, int32_t bcol_run = 1,
-                const Ref* amax = nullptr, const Ref* bmax = nullptr, int32_t brow_run = 1);
+                const Ref* amax = nullptr, const Ref* bmax = nullptr, int32_t brow_run = 1, int64_t arow = -1,
+                int64_t acol = -1);
   void run_ops(const std::vector<Op>& ops, const Ctx& c);
   void enqueue(const Ctx& c, const std::function<void(const Op&, const Ctx&)>& visit = nullptr);
   void autotune(const Ctx& c, hipStream_t s);
@@ -111,6 +114,7 @@ class Executor {
     int64_t brow = -1, bcol = -1;  // gather tables of B in consts_ (element offsets), or -1
     int32_t bcol_run = 1;
     Ref amax, bmax;  // operand max magnitudes (FP16X2), or NONE
+    int64_t arow = -1, acol = -1;  // gather tables of A (simple kernel), or -1
   };
   // FP16X2: step results whose max magnitude a later split product reads as its
   // operand scale (folded into the producing GEMM instead of a separate pass),

@@ -121,7 +121,8 @@ __global__ __launch_bounds__(256) void gemm_simple_kernel(GemmArgs g) {
       uint64_t i = e / g.n, j = e - i * g.n;
       T acc = CM<T>::zero();
       for (int64_t p = 0; p < g.k; ++p) {
-        T a = OPA ? CM<T>::conj(A[p * g.lda + i]) : A[i * g.lda + p];
+        T a = OPA ? CM<T>::conj(A[p * g.lda + i])
+                  : A[g.acol ? (int64_t)g.arow[i] + (int64_t)g.acol[p] : (int64_t)i * g.lda + p];
         T b = OPB ? CM<T>::conj(B[j * g.ldb + p]) : B[b_index(g, p, j)];
         acc = CM<T>::mac(acc, a, b);
       }
@@ -1646,6 +1647,7 @@ static int launch_gemm_kernels(int dt, int prec, int kernel, const GemmArgs& g0,
   GemmArgs g = g0;
   if (kernel == GEMM_AUTO) kernel = auto_kernel(dt, prec, g);
   if (g.k == 0) kernel = GEMM_SIMPLE;  // empty sums: every kernel but this one assumes k tiles
+  if (g.acol) kernel = GEMM_SIMPLE;    // A through gather tables: only this kernel reads them
   if (kernel == GEMM_SPLITK) {
     run_splitk(dt, g, s);
     return GEMM_SPLITK;
@@ -1786,6 +1788,11 @@ bool split_products_apply(int dt, int prec, int64_t m, int64_t k, int64_t n) {
   GemmArgs g{};
   g.m = m, g.k = k, g.n = n;
   return (prec == BF16X3 || prec == FP16X2) && (dt == F32 || dt == C64) && split_worth(dt, g);
+}
+
+bool gemm_runs_simple(int dt, int prec, int kernel, const GemmArgs& g) {
+  if (kernel == GEMM_AUTO) kernel = auto_kernel(dt, prec, g);
+  return g.k == 0 || kernel == GEMM_SIMPLE;
 }
 
 size_t gemm_scratch_bytes(int dt, int prec, int kernel, const GemmArgs& g) {

@@ -135,6 +135,11 @@ struct GemmArgs {
   // a pass before the GEMM. An upper bound is enough: it only sets the scale.
   const unsigned long long* amax = nullptr;
   const unsigned long long* bmax = nullptr;
+  // Optional gather tables of A, read by the simple kernel only (launch_gemm
+  // runs that kernel when they are set): A element (i, p) of a batch entry sits
+  // at arow[i] + acol[p] instead of i * lda + p.
+  const int32_t* arow = nullptr;
+  const int32_t* acol = nullptr;
 };
 // GEMM_PRESPLIT: split-precision MFMA GEMM whose operands are split into
 // 16-bit (hi, lo) arrays by a separate pass (GEMM_MFMA splits them while staging
@@ -154,6 +159,8 @@ enum GemmKernel {
 int launch_gemm(int dt, int prec, int kernel, const GemmArgs& g, hipStream_t s);
 // Device scratch launch_gemm can use for this GEMM (FP16X2 scales, split-K partials).
 size_t gemm_scratch_bytes(int dt, int prec, int kernel, const GemmArgs& g);
+// Whether this GEMM runs on the simple kernel under kernel choice `kernel`.
+bool gemm_runs_simple(int dt, int prec, int kernel, const GemmArgs& g);
 // Whether the heuristic choice for an m x k x n product (element units) runs
 // split-precision products under precision prec.
 bool split_products_apply(int dt, int prec, int64_t m, int64_t k, int64_t n);
