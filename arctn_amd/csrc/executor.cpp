@@ -250,10 +250,53 @@ void Executor::gen_gemm(const PairStep& st, size_t nbo, bool va, bool vb, const 
 void Executor::gen_steps(bool body, size_t nb, std::vector<Ref>& slots, std::vector<std::vector<size_t>>& shapes) {
   const Plan& P = *P_;
   const int dt = o_.dtype, es = es_;
+  // The slice-invariant part of the batched form runs level by level of the
+  // contraction tree (a step's level: one more than its operands'), and the steps
+  // of one level that run on the simple kernel share one grouped launch.
+  const bool grouping = o_.batched && !body && !o_.strip;
+  std::vector<size_t> order(P.cp.steps.size());
+  std::vector<size_t> level(P.cp.n_inputs + P.cp.steps.size(), 0);
   for (size_t si = 0; si < P.cp.steps.size(); ++si) {
+    const PairStep& st = P.cp.steps[si];
+    level[st.out] = std::max(level[st.li], level[st.ri]) + 1;
+    order[si] = si;
+  }
+  if (grouping)
+    std::stable_sort(order.begin(), order.end(),
+                     [&](size_t x, size_t y) { return level[P.cp.steps[x].out] < level[P.cp.steps[y].out]; });
+  std::vector<GroupGemm> group;
+  auto flush = [&]() {
+    if (group.empty()) return;
+    const size_t first = ggemm_.size();
+    uint64_t blocks = 0;
+    double flops = 0, bytes = 0;
+    const double cf = dtype_complex(dt) ? 8.0 : 2.0;
+    for (GroupGemm& q : group) {
+      use(q.a);
+      use(q.b);
+      use(q.c);
+      q.block0 = blocks;
+      blocks += ((uint64_t)(q.m * q.n) + 255) / 256;
+      flops += cf * (double)q.m * (double)q.k * (double)q.n;
+      bytes += (double)es * (double)(q.m * q.k + q.k * q.n + q.m * q.n);
+      ggemm_.push_back(q);
+    }
+    const int cnt = (int)group.size();
+    emit([=](const Ctx& c) {
+      launch_gemm_simple_group(dt, ggemm_dev_ + first, cnt, blocks, c.ws, inputs_dev_, consts_, c.s);
+    }, flops, bytes);
+    group.clear();
+  };
+  size_t cur_level = 0;
+  for (size_t oi = 0; oi < order.size(); ++oi) {
+    const size_t si = order[oi];
     const PairStep& st = P.cp.steps[si];
     bool v = P.varying[st.out];
     if (o_.batched && v != body) continue;
+    if (grouping && level[st.out] != cur_level) {
+      flush();
+      cur_level = level[st.out];
+    }
     bool va = o_.batched && P.varying[st.li], vb = o_.batched && P.varying[st.ri];
     size_t nbA = va ? nb : 1, nbB = vb ? nb : 1, nbC = (va || vb) ? nb : 1;
     std::vector<Ref> temps;
@@ -415,12 +458,18 @@ void Executor::gen_steps(bool body, size_t nb, std::vector<Ref>& slots, std::vec
       gen_gemm(st, nbC, va, vb, a, b, c, &mc, brow, bcol, bcol_run, am, bm, brow_run, arow, acol);
       slot_max_[st.out] = mc;
     } else {
-      gen_gemm(st, nbC, va, vb, a, b, c, nullptr, brow, bcol, bcol_run, am, bm, brow_run, arow, acol);
+      GemmArgs t{};
+      t.m = (int64_t)st.m, t.k = (int64_t)st.k, t.n = (int64_t)st.n, t.nbo = t.nbi = 1;
+      if (grouping && st.bt == 1 && nbC == 1 && gemm_runs_simple(dt, o_.precision, o_.gemm_kernel, t))
+        group.push_back(GroupGemm{a, b, c, arow, acol, brow, bcol, t.m, t.k, t.n, 0});
+      else
+        gen_gemm(st, nbC, va, vb, a, b, c, nullptr, brow, bcol, bcol_run, am, bm, brow_run, arow, acol);
     }
     slots[st.out] = c;
     shapes[st.out].resize(oc.size());
     for (size_t d = 0; d < oc.size(); ++d) shapes[st.out][d] = st.result_shape[oc[d]];
   }
+  flush();
 }
 
 Ref Executor::gen_finalize(size_t nb, const Ref& src0, bool to_output) {
@@ -472,6 +521,7 @@ void Executor::build(size_t B) {
   pre_.clear(), body_.clear(), tail_.clear(), post_.clear();
   gemm_choice_.clear();
   gemm_sites_.clear();
+  ggemm_.clear();
   const_host_.clear();
   body_ids_.clear();
   replay_ = record_ = false;
@@ -669,6 +719,22 @@ Executor::Executor(std::shared_ptr<const Plan> plan, const ExecOptions& o) : P_(
     ATN_HIP(hipMalloc((void**)&consts_, const_host_.size() * sizeof(int32_t)));
     ATN_HIP(hipMemcpy(consts_, const_host_.data(), const_host_.size() * sizeof(int32_t), hipMemcpyHostToDevice));
   }
+  if (!ggemm_.empty()) {
+    auto where = [&](const Ref& r) -> int64_t {
+      if (r.kind == Ref::WS) return (int64_t)offs_[r.id];
+      if (r.kind == Ref::INPUT) return -(int64_t)r.id - 1;
+      throw Error("internal: grouped GEMM operand outside the workspace and inputs", -3);
+    };
+    std::vector<SimpleGemmDesc> d(ggemm_.size());
+    for (size_t i = 0; i < ggemm_.size(); ++i) {
+      const GroupGemm& q = ggemm_[i];
+      if (q.c.kind != Ref::WS) throw Error("internal: grouped GEMM result outside the workspace", -3);
+      d[i] = SimpleGemmDesc{where(q.a), where(q.b), where(q.c), q.arow, q.acol, q.brow, q.bcol, q.m, q.k, q.n, q.block0};
+    }
+    ATN_HIP(hipMalloc((void**)&ggemm_dev_, d.size() * sizeof(SimpleGemmDesc)));
+    ATN_HIP(hipMemcpy(ggemm_dev_, d.data(), d.size() * sizeof(SimpleGemmDesc), hipMemcpyHostToDevice));
+    ATN_HIP(hipMalloc((void**)&inputs_dev_, std::max<size_t>(P.net.inputs.size(), 1) * sizeof(void*)));
+  }
   if (o_.use_graph) ATN_HIP(hipStreamCreateWithFlags(&cap_stream_, hipStreamNonBlocking));
 }
 
@@ -678,6 +744,19 @@ Executor::~Executor() {
   if (cap_stream_) (void)hipStreamDestroy(cap_stream_);
   if (ws_) (void)hipFree(ws_);
   if (consts_) (void)hipFree(consts_);
+  if (ggemm_dev_) (void)hipFree(ggemm_dev_);
+  if (inputs_dev_) (void)hipFree(inputs_dev_);
+}
+
+void Executor::sync_inputs(const void* const* inputs) {
+  if (!inputs_dev_) return;
+  const size_t n = P_->net.inputs.size();
+  std::vector<const void*> now(inputs, inputs + n);
+  if (now == inputs_seen_) return;
+  // earlier runs, on any stream, may still read the table
+  ATN_HIP(hipDeviceSynchronize());
+  ATN_HIP(hipMemcpy(inputs_dev_, now.data(), n * sizeof(void*), hipMemcpyHostToDevice));
+  inputs_seen_ = now;
 }
 
 void Executor::run_ops(const std::vector<Op>& ops, const Ctx& c) {
@@ -721,6 +800,7 @@ void Executor::profile(const void* const* inputs, void* out, hipStream_t s, std:
   c.inputs = inputs;
   c.out = out;
   c.s = s;
+  sync_inputs(inputs);
   if (o_.autotune && o_.gemm_kernel == GEMM_AUTO && !tuned_) autotune(c, s);
   struct Rec {
     hipEvent_t a, b;
@@ -841,6 +921,7 @@ void Executor::run(const void* const* inputs, void* out, hipStream_t s, Report* 
   c.inputs = inputs;
   c.out = out;
   c.s = s;
+  sync_inputs(inputs);
   if (o_.autotune && o_.gemm_kernel == GEMM_AUTO && !tuned_) autotune(c, s);
   hipEvent_t e0 = nullptr, e1 = nullptr;
   if (rep) {

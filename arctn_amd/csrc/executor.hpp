@@ -165,6 +165,20 @@ class Executor {
   std::vector<int> body_ids_;
   size_t replay_pos_ = 0;
   bool replay_ = false, record_ = false;
+  // Small GEMMs of the slice-invariant part that run on the simple kernel, grouped
+  // per level of the contraction tree into one launch each (their descriptors on
+  // the device once the workspace is placed), and the device table of input
+  // pointers those launches read (rewritten when a run brings other inputs).
+  struct GroupGemm {
+    Ref a, b, c;
+    int64_t arow, acol, brow, bcol, m, k, n;
+    uint64_t block0;
+  };
+  std::vector<GroupGemm> ggemm_;
+  SimpleGemmDesc* ggemm_dev_ = nullptr;
+  const void** inputs_dev_ = nullptr;
+  std::vector<const void*> inputs_seen_;
+  void sync_inputs(const void* const* inputs);
 };
 
 // Stage execution with stream-ordered scratch (src/compiled.rs:60-63, 143-153, 206-209).

@@ -131,6 +131,41 @@ __global__ __launch_bounds__(256) void gemm_simple_kernel(GemmArgs g) {
   }
 }
 
+// A group of small independent GEMMs in one launch (a level of a plan's tiny
+// slice-invariant steps): each 256-thread block finds its GEMM by binary search
+// over the blocks' starting indices, then computes one C element per thread as
+// the simple kernel does.
+template <class T>
+__global__ __launch_bounds__(256) void gemm_simple_group_kernel(const SimpleGemmDesc* __restrict__ d, int n,
+                                                                uint64_t blocks, char* ws,
+                                                                const void* const* __restrict__ inputs,
+                                                                const int32_t* __restrict__ tab) {
+  for (uint64_t blk = blockIdx.x; blk < blocks; blk += gridDim.x) {
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) / 2;
+      if (d[mid].block0 <= blk) lo = mid;
+      else hi = mid - 1;
+    }
+    const SimpleGemmDesc& q = d[lo];
+    const uint64_t e = (blk - q.block0) * 256 + threadIdx.x;
+    if (e >= (uint64_t)(q.m * q.n)) continue;
+    const T* A = q.a >= 0 ? (const T*)(ws + q.a) : (const T*)inputs[-q.a - 1];
+    const T* B = q.b >= 0 ? (const T*)(ws + q.b) : (const T*)inputs[-q.b - 1];
+    T* C = (T*)(ws + q.c);
+    const int64_t i = (int64_t)(e / (uint64_t)q.n), j = (int64_t)(e - (uint64_t)i * q.n);
+    const int64_t ar = q.acol >= 0 ? (int64_t)tab[q.arow + i] : i * q.k;
+    const int64_t bc = q.bcol >= 0 ? (int64_t)tab[q.bcol + j] : j;
+    T acc = CM<T>::zero();
+    for (int64_t p = 0; p < q.k; ++p) {
+      const T a = A[ar + (q.acol >= 0 ? (int64_t)tab[q.acol + p] : p)];
+      const T b = B[(q.bcol >= 0 ? (int64_t)tab[q.brow + p] : p * q.n) + bc];
+      acc = CM<T>::mac(acc, a, b);
+    }
+    C[e] = acc;
+  }
+}
+
 // ----------------------------------------------------------- skinny GEMMs --
 // Small m (m <= MM): one thread per output column j keeps all m accumulators,
 // so B and C are streamed exactly once with coalesced accesses; A (m x k) is
@@ -1810,5 +1845,16 @@ size_t gemm_scratch_bytes(int dt, int prec, int kernel, const GemmArgs& g) {
 }
 
 void launch_gemm_adjoint(int dt, int opa, int opb, const GemmArgs& g, hipStream_t s) { run_simple(dt, g, s, opa, opb); }
+
+void launch_gemm_simple_group(int dt, const SimpleGemmDesc* d, int n, uint64_t blocks, char* ws,
+                              const void* const* inputs, const int32_t* tables, hipStream_t s) {
+  if (n <= 0 || blocks == 0) return;
+  g_last_kind = K_GEMM_SIMPLE;
+  const dim3 grid((unsigned)std::min<uint64_t>(blocks, 1u << 16));
+  dispatch_dt(dt, [&](auto tag) {
+    using T = decltype(tag);
+    LAUNCH((gemm_simple_group_kernel<T>), grid, dim3(256), 0, s, d, n, blocks, ws, inputs, tables);
+  });
+}
 
 }  // namespace atn

@@ -164,6 +164,19 @@ bool gemm_runs_simple(int dt, int prec, int kernel, const GemmArgs& g);
 // Whether the heuristic choice for an m x k x n product (element units) runs
 // split-precision products under precision prec.
 bool split_products_apply(int dt, int prec, int64_t m, int64_t k, int64_t n);
+// Many small independent GEMMs (C = A B, one batch entry each, row-major C) in one
+// launch of the simple kernel. a, b: workspace byte offsets (>= 0) or -(input
+// index + 1); c: a workspace byte offset. A(i, p) = A[arow[i] + acol[p]] or
+// A[i * k + p]; B(p, j) = B[brow[p] + bcol[j]] or B[p * n + j] (offsets into the
+// gather tables, or -1). block0: first 256-thread block of the GEMM in the launch.
+struct SimpleGemmDesc {
+  int64_t a, b, c;
+  int64_t arow, acol, brow, bcol;
+  int64_t m, k, n;
+  uint64_t block0;
+};
+void launch_gemm_simple_group(int dt, const SimpleGemmDesc* d, int n, uint64_t blocks, char* ws,
+                              const void* const* inputs, const int32_t* tables, hipStream_t s);
 // Conjugating variants for the adjoint: opA/opB: 0 = N, 1 = conj-transpose (as stored).
 void launch_gemm_adjoint(int dt, int opa, int opb, const GemmArgs& g, hipStream_t s);
 
