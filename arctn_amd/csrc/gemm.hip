@@ -966,12 +966,15 @@ __global__ __launch_bounds__(256) void split_a_kernel(GemmArgs g, PreSplit ps) {
   }
 }
 
-// B [k][n] -> transposed [zb][n][Kp] hi/lo through a 32 (k) x 64 (n) LDS tile.
+// B [k][n] -> transposed [zb][n][Kp] hi/lo through a TK_SPLIT_BT (k) x 64 (n) LDS
+// tile (measured on gfx950: a 64-deep tile ran the complex64 k=512 n=8192 x256
+// pass in 6.3 ms against 4.3 for 32).
+constexpr int TK_SPLIT_BT = 32;
 template <bool CPLX, bool F16>
 __global__ __launch_bounds__(256) void split_bt_kernel(GemmArgs g, PreSplit ps) {
   using H = typename Half<F16>::H;
   using E = typename std::conditional<CPLX, float2, float>::type;
-  constexpr int TK = 32, TN = 64, EH = CPLX ? 2 : 1;  // halves per element
+  constexpr int TK = TK_SPLIT_BT, TN = 64, EH = CPLX ? 2 : 1;  // halves per element
   __shared__ E tile[TK][TN + 1];
   const int64_t kp = ps.kp, inner = g.sbi ? g.nbi : 1, nm = n_mats(g.sbo, g.sbi, g.nbo, g.nbi);
   const int64_t tk = (kp / EH + TK - 1) / TK, tn = (g.n + TN - 1) / TN, total = nm * tk * tn;
@@ -993,8 +996,9 @@ __global__ __launch_bounds__(256) void split_bt_kernel(GemmArgs g, PreSplit ps) 
       tile[kk][nn] = v;
     }
     __syncthreads();
-    {  // thread -> (row n, 8 consecutive k elements)
-      const int nn = t / 4, kc = (t % 4) * 8;
+#pragma unroll
+    for (int w = t; w < TN * (TK / 8); w += 256) {  // item -> (row n, 8 consecutive k elements)
+      const int nn = w / (TK / 8), kc = (w % (TK / 8)) * 8;
       const int64_t gn = n0 + nn;
       if (gn < g.n) {
         H h[8 * EH], l[8 * EH];
@@ -1603,14 +1607,14 @@ static PreSplitSizes presplit_sizes(int dt, const GemmArgs& g) {
 // Split-precision products pay off on MFMA-bound shapes only; memory-bound
 // skinny steps run the FP32 kernels (same time, full FP32 accuracy).
 // Split products pay off with 128-row tiles from m = 64 and with 32-row tiles
-// for 8 <= m <= 32 down to 32 real k (measured on gfx950, FP16x2 on complex64
-// x256 read through gather tables, against the 32-row FP32 MFMA kernel: m=32
-// k=128 n=131072 8.7 vs 13.0 ms, m=8 k=256 n=32768 3.7 vs 6.0 ms, m=16 k=16
-// n=262144 5.3 vs 5.9 ms; at m=32 k=8 the FP32 kernel is the faster).
+// for 8 <= m <= 32 (measured on gfx950 in the 30-qubit circuit's contraction,
+// FP16x2 on complex64 x256 read through gather tables, against the 32-row FP32
+// MFMA kernel: m=32 k=128 n=131072 9.1 vs 15.9 ms, m=8 k=256 n=32768 3.9 vs
+// 7.5 ms; at m=16 k=16 n=262144 the FP32 kernel is the faster, 4.7 vs 5.5 ms).
 static bool split_worth(int dt, const GemmArgs& g) {
   const bool cplx = dtype_complex(dt);
   const int64_t K = cplx ? 2 * g.k : g.k, N = cplx ? 2 * g.n : g.n;
-  return N >= 128 && ((g.m >= 64 && K >= 64) || (g.m >= 8 && g.m <= 32 && K >= 32));
+  return N >= 128 && K >= 64 && (g.m >= 64 || (g.m >= 8 && g.m <= 32));
 }
 
 static int native_mfma_kernel(int dt, int64_t m) {
@@ -1621,14 +1625,13 @@ static int native_mfma_kernel(int dt, int64_t m) {
 static int auto_kernel(int dt, int prec, const GemmArgs& g) {
   const bool split_prec = (prec == BF16X3 || prec == FP16X2) && (dt == F32 || dt == C64);
   if (split_prec && split_worth(dt, g)) {
-    // One or two row tiles of A (32 rows for m <= 32, else 128): A split by a
+    // Up to four row tiles of A (32 rows for m <= 32, else 128): A split by a
     // pass, B read in its own precision and split while staging (measured on
-    // gfx950: up to 1.6x faster than splitting B by a pass on short-k steps,
-    // where that pass's extra read and write of B dominate). With more row tiles
-    // every tile re-reads and re-splits B, and on operands read through gather
-    // tables (short runs of consecutive columns) one splitting pass over B is
-    // faster.
-    return (g.m + 127) / 128 <= 2 ? GEMM_PRESPLIT_A : GEMM_PRESPLIT;
+    // gfx950 in the circuit's contraction: m=512 k=128 n=32768 x256 20.5 ms
+    // against 22.4 for a splitting pass over B and the pre-split GEMM, m=512
+    // k=512 n=8192 15.6 against 17.1). With more row tiles every tile re-reads
+    // and re-splits B, and one splitting pass over B is faster.
+    return (g.m + 127) / 128 <= 4 ? GEMM_PRESPLIT_A : GEMM_PRESPLIT;
   }
   if (mfma_worth(dt, g)) return native_mfma_kernel(dt, g.m);
   if (splitk_worth(g)) return GEMM_SPLITK;
@@ -1735,7 +1738,7 @@ static int launch_gemm_kernels(int dt, int prec, int kernel, const GemmArgs& g0,
       ps.bhi = raw_b ? nullptr : base + 2 * psz.a_half, ps.blo = raw_b ? nullptr : base + 2 * psz.a_half + psz.b_half;
       ps.kp = psz.kp;
       const int64_t na = n_mats(g.sao, g.sai, g.nbo, g.nbi) * g.m * psz.kp / 4;
-      const int64_t tiles_b = n_mats(g.sbo, g.sbi, g.nbo, g.nbi) * ((psz.kp / (cx ? 2 : 1) + 31) / 32) * ((g.n + 63) / 64);
+      const int64_t tiles_b = n_mats(g.sbo, g.sbi, g.nbo, g.nbi) * ((psz.kp / (cx ? 2 : 1) + TK_SPLIT_BT - 1) / TK_SPLIT_BT) * ((g.n + 63) / 64);
       const dim3 ga((unsigned)std::min<int64_t>((na + 255) / 256, 65536)), gb((unsigned)std::min<int64_t>(tiles_b, 65536));
 #define PRESPLIT_LAUNCH(C_, F_, R_)                                                                        \
   do {                                                                                                     \
