@@ -1058,6 +1058,9 @@ __device__ inline int swz(int row, int ch) {
 // one column, wave-uniform rows: columns contiguous in memory), 2 raw B with
 // lanes along k (chunk q = column q / CH, k chunk q % CH, as for pre-split B: for
 // operands whose contracted legs are innermost in memory, GemmArgs::brow_run).
+// (Loading the next tile's first k tile during the last k tile's MFMAs, with up
+// to four tiles per workgroup, measured the same on the circuit's steps: the
+// two workgroups per CU already overlap one's prologue with the other's MFMAs.)
 // BM: rows of A per tile. 128: four 64x64 wave tiles (two workgroups per CU);
 // 32: four 32x32 wave tiles side by side (40 KiB of LDS, three workgroups per
 // CU) for m <= 32, where B streams from HBM once and the MFMA work per B byte
