@@ -30,7 +30,7 @@ class NetworkDesc(C.Structure):
 class PlanInfo(C.Structure):
     _fields_ = [("n_inputs", C.c_size_t), ("n_steps", C.c_size_t), ("n_slices", C.c_size_t),
                 ("n_blocks", C.c_size_t), ("n_varying_steps", C.c_size_t), ("log10_flops", C.c_double),
-                ("log2_peak_size", C.c_double), ("flops_per_slice", C.c_double), ("mults_per_slice", C.c_double)]
+                ("log2_peak_size", C.c_double), ("macs_per_slice", C.c_double), ("mults_per_slice", C.c_double)]
 
 
 class ExecOptions(C.Structure):

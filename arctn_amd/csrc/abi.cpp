@@ -121,7 +121,7 @@ int atn_plan_stats(const atn_plan* plan, atn_plan_info* out) {
     out->n_varying_steps = p.n_varying_steps;
     out->log10_flops = p.log10_flops;
     out->log2_peak_size = p.log2_peak_size;
-    out->flops_per_slice = p.flops_per_slice;
+    out->macs_per_slice = p.macs_per_slice;
     out->mults_per_slice = p.mults_per_slice;
   });
 }

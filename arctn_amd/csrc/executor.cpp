@@ -967,7 +967,7 @@ void Executor::run(const void* const* inputs, void* out, hipStream_t s, Report* 
     (void)hipEventDestroy(e1);
     rep->elapsed_ms = ms;
     double c8 = dtype_complex(o_.dtype) ? 8.0 : 2.0;
-    rep->flops = c8 * P.flops_per_slice * (double)n_run_;
+    rep->flops = c8 * P.macs_per_slice * (double)n_run_;
     rep->tflops = ms > 0 ? rep->flops / (ms * 1e-3) / 1e12 : 0;
     rep->slices_per_s = ms > 0 ? (double)n_run_ / (ms * 1e-3) : 0;
     rep->n_slices = n_run_;

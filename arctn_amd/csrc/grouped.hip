@@ -713,7 +713,7 @@ void GroupedExecutor::run(const std::vector<const void* const*>& inputs, const s
       float ms = 0;
       ATN_HIP(hipEventElapsedTime(&ms, e0, e1));
       double flops = 0;
-      for (auto& p : plans_) flops += (dtype_complex(dt) ? 8.0 : 2.0) * p->flops_per_slice;
+      for (auto& p : plans_) flops += (dtype_complex(dt) ? 8.0 : 2.0) * p->macs_per_slice;
       rep->elapsed_ms = ms;
       rep->flops = flops;
       rep->tflops = ms > 0 ? flops / (ms * 1e-3) / 1e12 : 0;

@@ -372,7 +372,7 @@ Plan make_plan(const Net& net, const Path& path, const std::vector<Leg>& sliced)
     bool v = p.varying[s.li] || p.varying[s.ri];
     p.varying.push_back(v);
     p.n_varying_steps += v;
-    p.flops_per_slice += (double)s.bt * (double)s.m * (double)s.k * (double)s.n;
+    p.macs_per_slice += (double)s.bt * (double)s.m * (double)s.k * (double)s.n;
     std::set<Leg> uni(s.left_legs.begin(), s.left_legs.end());
     uni.insert(s.right_legs.begin(), s.right_legs.end());
     double mm = 1;

@@ -92,7 +92,7 @@ struct Plan {
   std::vector<size_t> full_out_shape;
   size_t sub_out_numel = 1, full_out_numel = 1;
   double log10_flops = 0, log2_peak_size = 0;  // stats of the whole (unsliced) network
-  double flops_per_slice = 0, mults_per_slice = 0;
+  double macs_per_slice = 0, mults_per_slice = 0;
   std::vector<bool> varying;  // per slot of cp: depends on the slice index
   size_t n_varying_steps = 0;
   size_t n_chunks = 1;        // reference chunking of internal slices (src/slice.rs:893-905)

@@ -341,16 +341,18 @@ def _slice_leg_allowed(net, leg, allow_output_slicing):
     return not any(legs.count(leg) > 1 for legs in net["inputs"])
 
 
-def find_slices_to_size(net, path, target_size, allow_output_slicing=False):
+def find_slices_to_size(net, path, target_size, allow_output_slicing=False, presliced=()):
     """Greedy slice-leg selection (src/slice.rs:254-288, 557-589).
 
+    ``presliced`` legs are sliced before the search starts (output-index
+    slicing of a fixed set of open legs); the search adds legs from there.
     Returns the sorted list of sliced legs, or None when no executable leg
     makes further progress.
     """
     if target_size <= 0:
         return None
     sd0 = net["size_dict"]
-    sliced = []
+    sliced = sorted(set(presliced))
     while True:
         cur_sd = dict(sd0)
         for l in sliced:
@@ -474,11 +476,11 @@ def random_greedy(net, ntrials, seed):
     return best[0], best[1]
 
 
-def random_greedy_sliced(net, ntrials, seed, target_size, allow_output_slicing=False):
+def random_greedy_sliced(net, ntrials, seed, target_size, allow_output_slicing=False, presliced=()):
     """Random-greedy trials (greedy.rs:325-412, nbranch 8), each sliced to
-    ``target_size`` with ``find_slices_to_size``; the trial with the lowest
-    total sliced FLOPs wins (ties: lower trial index). Returns
-    ``(path, sliced, log10_total_flops)``.
+    ``target_size`` with ``find_slices_to_size`` (starting from the
+    ``presliced`` legs); the trial with the lowest total sliced FLOPs wins
+    (ties: lower trial index). Returns ``(path, sliced, log10_total_flops)``.
     """
     import numpy as np
 
@@ -487,7 +489,7 @@ def random_greedy_sliced(net, ntrials, seed, target_size, allow_output_slicing=F
         rng = np.random.default_rng([seed, trial])
         cm, t, fn = _rgreedy_params(trial, rng)
         path = greedy(net, costmod=cm, temperature=t, nbranch=8 if trial > 2 else 1, cost_fn=fn, rng=rng)
-        sliced = find_slices_to_size(net, path, target_size, allow_output_slicing)
+        sliced = find_slices_to_size(net, path, target_size, allow_output_slicing, presliced)
         if sliced is None:
             continue
         cost = sliced_cost_log10(net, path, sliced)

@@ -20,8 +20,11 @@ PLANS = {
     "circuit_4x4_c8": dict(rows=4, cols=4, cycles=8, seed=0, open=0, target=None, trials=0),
     # 30-qubit 5x6 grid, 14 cycles, one amplitude, target 2^24 elements.
     "circuit_5x6_c14_t24": dict(rows=5, cols=6, cycles=14, seed=0, open=0, target=2**24, trials=40),
-    # 36-qubit 6x6 grid, 12 cycles, 6 open qubits (64 amplitudes), target 2^26.
-    "circuit_6x6_c12_o6_t26": dict(rows=6, cols=6, cycles=12, seed=0, open=6, target=2**26, trials=24),
+    # 36-qubit 6x6 grid, 12 cycles, 6 open qubits (64 amplitudes), target 2^26. All six open
+    # legs are output-sliced (64 blocks, src/slice.rs:1070-1173); the path is searched on the
+    # block network (open legs fixed) and the slice finder adds internal legs from there.
+    "circuit_6x6_c12_o6_t26": dict(rows=6, cols=6, cycles=12, seed=0, open=6, target=2**26, trials=24,
+                                   slice_open=True),
 }
 # 512 random 3-regular networks, n uniform over even [40, 120], dims {2, 3, 4},
 # float32 N(0, 1) entries, scalar output; per-instance random-greedy path, seed = instance id.
@@ -66,6 +69,12 @@ def build(name):
     t0 = time.time()
     if p["target"] is None:
         path, sliced = planner.greedy(net), []
+        cost = planner.sliced_cost_log10(net, path, sliced)
+    elif p.get("slice_open"):
+        block = {"inputs": net["inputs"], "output": [],
+                 "size_dict": {l: (1 if l in net["output"] else d) for l, d in net["size_dict"].items()}}
+        path, internal, _ = planner.random_greedy_sliced(block, p["trials"], p["seed"], p["target"])
+        sliced = sorted(set(internal) | set(net["output"]))
         cost = planner.sliced_cost_log10(net, path, sliced)
     else:
         path, sliced, cost = planner.random_greedy_sliced(

@@ -109,7 +109,7 @@ def main():
     x128 = [t.to(torch.complex128) for t in x64]
     stream = torch.cuda.current_stream(dev)
     form = dict(batched=True, use_graph=not args.no_graph, precision=args.precision, autotune=not args.no_autotune)
-    flops = 8.0 * info["flops_per_slice"] * info["n_slices"]
+    flops = 8.0 * info["macs_per_slice"] * info["n_slices"]
 
     log(f"[bench] plan: {info['n_steps']} steps, {info['n_slices']} slices, "
         f"{info['n_varying_steps']} slice-dependent steps, {flops:.3e} FLOP per contraction")
@@ -242,7 +242,7 @@ def main():
         dt = time.perf_counter() - t0
         cpu = {"value": n / dt, "unit": "slices/s", "cores": cores, "kind": "port",
                "sample": f"slices 0..{n - 1} of {info['n_slices']} (complex64), oracle CPU restatement, "
-                         f"{dt:.1f} s", "tflops": 8.0 * info["flops_per_slice"] * n / dt / 1e12}
+                         f"{dt:.1f} s", "tflops": 8.0 * info["macs_per_slice"] * n / dt / 1e12}
         log(f"[bench] CPU: {n / dt:.3f} slices/s")
 
     line = {
@@ -357,7 +357,7 @@ def bench_grouped(args):
     dev = torch.device("cuda", 0)
     plans = [Plan(n["inputs"], n["output"], n["size_dict"], p) for n, _, p in insts]
     arrs = [[torch.from_numpy(x).to(dev) for x in a] for _, a, _ in insts]
-    flops = sum(2.0 * p.info()["flops_per_slice"] for p in plans)
+    flops = sum(2.0 * p.info()["macs_per_slice"] for p in plans)
     # float64 oracle (and the same plans in float32 on the CPU) on the instances the CPU finishes quickly
     cheap = [i for i, c in enumerate(doc["log10_mults"]) if c <= 8.0]
     t0 = time.perf_counter()

@@ -75,7 +75,9 @@ typedef struct {
   size_t n_varying_steps;  /* steps whose operands depend on the slice index */
   double log10_flops;      /* reference convention: sum over steps of prod(union dims), whole network */
   double log2_peak_size;   /* live-set peak of the whole network (src/path.rs:396-399) */
-  double flops_per_slice;  /* real FLOPs of one slice: sum c*bt*m*k*n, c = 8 complex / 2 real */
+  double macs_per_slice;   /* multiply-adds of the GEMMs of one slice: sum over steps of bt*m*k*n in the
+                              plan's scalar type (a plan has no dtype); real FLOPs are 8x this for
+                              complex and 2x for real execution */
   double mults_per_slice;  /* scalar multiplications of one slice (reference convention) */
 } atn_plan_info;
 

@@ -78,6 +78,12 @@ def test_known_geometry_and_stats():
     fx = FX["chain_costs"]
     info = Plan(fx["inputs"], fx["output"], F.net_of(fx)["size_dict"], fx["ssa_path"]).info()
     assert abs(info["log10_flops"] - np.log10(fx["flops"])) < 1e-12
+    # A(BC): B.C is (m, k, n) = (4, 8, 3), A.(BC) is (2, 4, 3): 96 + 24 multiply-adds, which
+    # is also the reference's scalar-multiplication count (no pre-summed legs here)
+    assert info["macs_per_slice"] == 120 and info["mults_per_slice"] == fx["flops"]
+    fx = FX["two_node_geometry"]
+    info = Plan(fx["inputs"], fx["output"], F.net_of(fx)["size_dict"], fx["ssa_path"]).info()
+    assert info["macs_per_slice"] == 1 * 3 * 4 * 5
 
 
 def test_slice_bookkeeping_matches_reference_rules():

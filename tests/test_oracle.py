@@ -172,6 +172,57 @@ def test_sliced_matches_unsliced_every_subset():
             assert np.abs(got - want).max() < tol, sl
 
 
+def _einsum_eq(net, legs_fixed=()):
+    sub = {l: chr(97 + i) for i, l in enumerate(sorted(net["size_dict"]))}
+    terms = ["".join(sub[l] for l in t if l not in legs_fixed) for t in net["inputs"]]
+    return ",".join(terms) + "->" + "".join(sub[l] for l in net["output"])
+
+
+def _fix_legs(net, arrays, legs, values):
+    """Inputs with every sliced leg fixed by np.take (independent of the oracle's select_axis)."""
+    out = []
+    for t, a in zip(net["inputs"], arrays):
+        for ax in range(len(t) - 1, -1, -1):
+            if t[ax] in legs:
+                a = np.take(a, values[legs.index(t[ax])], axis=ax)
+        out.append(a)
+    return out
+
+
+def test_slice_range_pinned():
+    """contract_slice_range (the sum over internal slices [lo, hi), src/slice.rs:1018-1064):
+    the full range equals the sliced contraction bit for bit and the naive result; consecutive
+    ranges add up to the whole; a one-slice range equals np.einsum of the network with the
+    sliced legs fixed by np.take at np.unravel_index(s, dims) (last sliced leg fastest)."""
+    checked = 0
+    for seed in range(40):
+        net, path = _rand_case(seed)
+        internal = [l for l in sorted(net["size_dict"]) if l not in net["output"]
+                    and not any(t.count(l) > 1 for t in net["inputs"])][:3]
+        if len(internal) < 2:
+            continue
+        dims = [net["size_dict"][l] for l in internal]
+        n = int(np.prod(dims))
+        for dtype, tol in ((np.float64, 1e-10), (np.complex128, 1e-10)):
+            arrays = W.random_arrays(net, dtype, seed)
+            whole = O.contract_slice_range(net, arrays, path, internal, 0, n)
+            assert whole.tobytes() == O.contract(net, arrays, path, "sliced", internal).tobytes()
+            naive = O.contract(net, arrays, path, "naive")
+            scale = max(1.0, np.abs(naive).max())
+            assert np.abs(whole - naive).max() <= tol * scale
+            a, b = n // 3, (2 * n) // 3 + 1
+            parts = sum(O.contract_slice_range(net, arrays, path, internal, lo, hi)
+                        for lo, hi in ((0, a), (a, b), (b, n)) if lo < hi)
+            assert np.abs(parts - whole).max() <= tol * scale
+            for s in (0, n // 2, n - 1):
+                vals = [int(v) for v in np.unravel_index(s, dims)]
+                want = np.einsum(_einsum_eq(net, internal), *_fix_legs(net, arrays, internal, vals), optimize=False)
+                got = O.contract_slice_range(net, arrays, path, internal, s, s + 1)
+                assert np.abs(got - want).max() <= tol * max(1.0, np.abs(want).max()), (seed, s)
+        checked += 1
+    assert checked >= 10
+
+
 def test_output_slicing_cases_vs_einsum():
     for case in FX["output_slicing_cases"]["cases"]:
         name, eq, net, sliced, path = F.einsum_case(case)
