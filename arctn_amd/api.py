@@ -18,6 +18,7 @@ every entry point raises.
 
 from __future__ import annotations
 
+import itertools
 import math
 import operator
 import time
@@ -308,35 +309,30 @@ def contract_sliced(inputs, output, size_dict, arrays, sliced_legs, path=None, s
     return _finish(out, numpy_in)
 
 
+def _positive_dim(label, raw):
+    if isinstance(raw, bool):
+        raise ValueError(f"腿 {label!r} 的维度必须是正整数，不能是 bool")
+    dim = operator.index(raw)
+    if dim <= 0:
+        raise ValueError(f"腿 {label!r} 的维度必须为正整数，得到 {dim}")
+    return dim
+
+
 def _to_int_labels(inputs, output, size_dict):
-    """pybind/python/arctn/__init__.py:186-235: hashable labels -> dense u32 ids."""
-    label_of = {}
-
-    def idx(lab):
-        i = label_of.get(lab)
-        if i is None:
-            i = len(label_of)
-            label_of[lab] = i
-        return i
-
-    inputs_i = [[idx(l) for l in term] for term in inputs]
-    output_i = [idx(l) for l in output]
-    missing = [lab for lab in label_of if lab not in size_dict]
-    if missing:
-        raise ValueError(f"size_dict 缺少已使用腿的维度: {missing!r}")
-    unused = [lab for lab in size_dict if lab not in label_of]
-    if unused:
-        raise ValueError(f"size_dict 含未在 inputs/output 使用的腿: {unused!r}")
-    size_i = {}
-    for lab, i in label_of.items():
-        raw = size_dict[lab]
-        if isinstance(raw, bool):
-            raise ValueError(f"腿 {lab!r} 的维度必须是正整数，不能是 bool")
-        dim = operator.index(raw)
-        if dim <= 0:
-            raise ValueError(f"腿 {lab!r} 的维度必须为正整数，得到 {dim}")
-        size_i[i] = dim
-    return inputs_i, output_i, size_i, label_of
+    """Hashable labels -> dense u32 leg ids, numbered by first occurrence over the
+    inputs in order and then the output (the numbering and error messages of
+    pybind/python/arctn/__init__.py:186-235). size_dict must size exactly the
+    labels in use, each with a positive integer."""
+    order = list(dict.fromkeys(itertools.chain(itertools.chain.from_iterable(inputs), output)))
+    ids = {lab: i for i, lab in enumerate(order)}
+    unsized = [lab for lab in order if lab not in size_dict]
+    if unsized:
+        raise ValueError(f"size_dict 缺少已使用腿的维度: {unsized!r}")
+    stray = [lab for lab in size_dict if lab not in ids]
+    if stray:
+        raise ValueError(f"size_dict 含未在 inputs/output 使用的腿: {stray!r}")
+    dims = {ids[lab]: _positive_dim(lab, size_dict[lab]) for lab in order}
+    return [[ids[l] for l in t] for t in inputs], [ids[l] for l in output], dims, ids
 
 
 def arctn_contract(inputs, output, size_dict, arrays, *, preset="greedy", seed=0, target_size=None,

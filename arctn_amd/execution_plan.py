@@ -15,13 +15,15 @@ reference's messages.
 
 from __future__ import annotations
 
+import contextlib
 import json
 import math
 import operator
 import os
-import tempfile
+import secrets
 from pathlib import Path
 from types import MappingProxyType
+from typing import NamedTuple, Optional
 
 from . import planner as _planner
 
@@ -291,66 +293,75 @@ def parse_v2(artifact):
     return net, labels, path, sliced, target
 
 
-def _freeze(v):
-    if isinstance(v, dict):
-        return MappingProxyType({k: _freeze(x) for k, x in v.items()})
-    if isinstance(v, (list, tuple)):
-        return tuple(_freeze(x) for x in v)
-    return v
+class _Parsed(NamedTuple):
+    """The validated content of one artifact; `doc` is its canonical JSON text
+    (immutable by construction: readers get fresh objects decoded from it)."""
+    net: dict
+    ssa_path: tuple
+    sliced: tuple
+    target: Optional[int]
+    labels: tuple
+    doc: str
 
 
-def _thaw(v):
-    if isinstance(v, MappingProxyType):
-        return {k: _thaw(x) for k, x in v.items()}
-    if isinstance(v, tuple):
-        return [_thaw(x) for x in v]
-    return v
+# Flat top-level fields that a v2 artifact may carry outside its metadata
+# objects, and the object each belongs to when that object is absent.
+_METADATA_GROUPS = (("metrics", _METRIC_KEYS), ("planning", _PLANNING_KEYS),
+                    ("provenance", ("arctn_version", "source")))
+_BUILD = object()
+
+
+def _group_metadata(artifact):
+    for group, keys in _METADATA_GROUPS:
+        if group not in artifact:
+            artifact[group] = {k: artifact[k] for k in keys if k in artifact}
+        elif not isinstance(artifact[group], dict):
+            raise PlanValidationError(f"execution plan {group} must be an object")
+    return artifact
+
+
+def _same_dtype(dtypes):
+    return len({str(d) for d in dtypes}) <= 1
 
 
 class ExecutionPlan:
-    """An immutable, validated version 2 plan (pybind/python/arctn/_execution_plan.py:117-590)."""
+    """An immutable, validated version 2 plan; the surface of ArcTNExecutionPlan
+    (pybind/python/arctn/_execution_plan.py:117-590): from_dict, load, to_dict,
+    save, validate, to_linear_path, compile, execute."""
 
-    __slots__ = ("_net", "_ssa_path", "_sliced", "_target", "_labels", "_artifact")
+    __slots__ = ("_p",)
     __hash__ = None
 
-    def __init__(self, *args, **kwargs):
-        raise TypeError("use ExecutionPlan.from_dict() or ExecutionPlan.load()")
+    def __init__(self, token=None, parsed=None):
+        if token is not _BUILD:
+            raise TypeError("use ExecutionPlan.from_dict() or ExecutionPlan.load()")
+        ExecutionPlan._p.__set__(self, parsed)
+
+    def __setattr__(self, name, value):
+        raise AttributeError("ExecutionPlan is immutable")
 
     @classmethod
     def from_dict(cls, artifact):
         """Validate and load one self-contained v2 artifact mapping."""
         if not isinstance(artifact, dict):
             raise PlanValidationError("execution plan root must be an object")
-        try:
-            artifact = json.loads(json.dumps(artifact, ensure_ascii=False, allow_nan=False))
+        try:  # a private JSON-clean copy: the caller's mapping is never kept or changed
+            text = json.dumps(artifact, ensure_ascii=False, allow_nan=False)
         except (TypeError, ValueError) as e:
             raise PlanValidationError(f"invalid execution plan: {e}") from None
+        artifact = json.loads(text)
         net, labels, path, sliced, target = parse_v2(artifact)
-        # _execution_plan.py:165-186: grouped metadata objects
-        if "metrics" not in artifact:
-            artifact["metrics"] = {k: artifact[k] for k in _METRIC_KEYS if k in artifact}
-        if "planning" not in artifact:
-            artifact["planning"] = {k: artifact[k] for k in _PLANNING_KEYS if k in artifact}
-        if "provenance" not in artifact:
-            artifact["provenance"] = {k: artifact[k] for k in ("arctn_version", "source") if k in artifact}
-        for name in ("metrics", "planning", "provenance"):
-            if not isinstance(artifact[name], dict):
-                raise PlanValidationError(f"execution plan {name} must be an object")
-        v = object.__new__(cls)
-        object.__setattr__(v, "_net", net)
-        object.__setattr__(v, "_ssa_path", tuple(path))
-        object.__setattr__(v, "_sliced", tuple(sliced))
-        object.__setattr__(v, "_target", target)
-        object.__setattr__(v, "_labels", tuple(labels) if labels is not None else tuple(sorted(net["size_dict"])))
-        object.__setattr__(v, "_artifact", _freeze(artifact))
-        return v
+        artifact = _group_metadata(artifact)
+        labels = tuple(labels) if labels is not None else tuple(sorted(net["size_dict"]))
+        doc = json.dumps(artifact, ensure_ascii=False, allow_nan=False, sort_keys=True)
+        return cls(_BUILD, _Parsed(net, tuple(path), tuple(sliced), target, labels, doc))
 
     @classmethod
     def load(cls, path):
         """Load and validate one self-contained v2 JSON plan file."""
         try:
-            with open(os.fspath(path), "r", encoding="utf-8") as f:
-                artifact = json.load(f)
+            text = Path(os.fspath(path)).read_text(encoding="utf-8")
+            artifact = json.loads(text)
         except (OSError, UnicodeError, json.JSONDecodeError) as e:
             raise PlanValidationError(f"cannot load execution plan: {e}") from None
         return cls.from_dict(artifact)
@@ -383,89 +394,98 @@ class ExecutionPlan:
                           "sliced_log2_max_size": per.log2_max_size}
         return cls.from_dict(art)
 
-    def __setattr__(self, name, value):
-        raise AttributeError("ExecutionPlan is immutable")
+    def _part(self, key):
+        return MappingProxyType(json.loads(self._p.doc)[key])
 
     schema = property(lambda self: SCHEMA)
     schema_version = property(lambda self: SCHEMA_VERSION)
     path_format = property(lambda self: PATH_FORMAT)
-    inputs = property(lambda self: tuple(tuple(t) for t in self._net["inputs"]))
-    output = property(lambda self: tuple(self._net["output"]))
-    size_dict = property(lambda self: MappingProxyType(dict(self._net["size_dict"])))
-    ssa_path = property(lambda self: self._ssa_path)
-    sliced_legs = property(lambda self: self._sliced)
-    target_size = property(lambda self: self._target)
-    metrics = property(lambda self: self._artifact["metrics"])
-    planning = property(lambda self: self._artifact["planning"])
-    provenance = property(lambda self: self._artifact["provenance"])
-    leg_labels = property(lambda self: self._labels)
-    network_canon = property(lambda self: self._artifact["network_canon"])
-    is_sliced = property(lambda self: bool(self._sliced))
+    inputs = property(lambda self: tuple(tuple(t) for t in self._p.net["inputs"]))
+    output = property(lambda self: tuple(self._p.net["output"]))
+    size_dict = property(lambda self: MappingProxyType(dict(self._p.net["size_dict"])))
+    ssa_path = property(lambda self: self._p.ssa_path)
+    sliced_legs = property(lambda self: self._p.sliced)
+    target_size = property(lambda self: self._p.target)
+    metrics = property(lambda self: self._part("metrics"))
+    planning = property(lambda self: self._part("planning"))
+    provenance = property(lambda self: self._part("provenance"))
+    leg_labels = property(lambda self: self._p.labels)
+    network_canon = property(lambda self: json.loads(self._p.doc)["network_canon"])
+    is_sliced = property(lambda self: bool(self._p.sliced))
 
     def to_dict(self):
-        """The portable JSON-compatible v2 artifact mapping."""
-        return _thaw(self._artifact)
+        """The portable JSON-compatible v2 artifact mapping (a fresh copy)."""
+        return json.loads(self._p.doc)
 
     def save(self, path):
-        """Atomically write the v2 JSON representation (_execution_plan.py:368-400)."""
-        dest = Path(path)
-        tmp = None
+        """Write the v2 JSON representation to `path`, replacing it atomically: the
+        text goes to a new sibling file that is flushed to disk and then renamed
+        over the destination, so readers see the old file or the whole new one."""
+        dest = Path(os.fspath(path))
+        body = json.dumps(self.to_dict(), ensure_ascii=False, allow_nan=False, indent=2, sort_keys=True) + "\n"
+        partial = dest.with_name(f".{dest.name}.{os.getpid()}.{secrets.token_hex(4)}.partial")
         try:
-            with tempfile.NamedTemporaryFile(mode="w", encoding="utf-8", dir=dest.parent, prefix=f".{dest.name}.",
-                                             suffix=".tmp", delete=False) as f:
-                tmp = Path(f.name)
-                json.dump(self.to_dict(), f, ensure_ascii=False, allow_nan=False, indent=2, sort_keys=True)
-                f.write("\n")
-            os.replace(tmp, dest)
-        except (OSError, TypeError, ValueError) as e:
-            if tmp is not None and tmp.exists():
-                tmp.unlink()
+            with open(partial, "x", encoding="utf-8") as f:
+                f.write(body)
+                f.flush()
+                os.fsync(f.fileno())
+            os.replace(partial, dest)
+        except OSError as e:
+            with contextlib.suppress(OSError):
+                partial.unlink()
             raise PlanValidationError(f"cannot save execution plan: {e}") from None
         return dest
 
+    def _check_caller_network(self, inputs, output, size_dict):
+        mine = self._p.net
+        same = ([list(t) for t in inputs] == mine["inputs"] and list(output) == mine["output"]
+                and sorted(dict(size_dict).items()) == sorted(mine["size_dict"].items()))
+        if not same:
+            raise PlanValidationError("caller network does not match the embedded plan network")
+
+    def _check_arrays(self, arrays):
+        try:
+            values = tuple(arrays)
+        except TypeError:
+            raise PlanValidationError("arrays must be a sequence") from None
+        legs = self._p.net["inputs"]
+        if len(values) != len(legs):
+            raise PlanValidationError(f"arrays length {len(values)} does not match {len(legs)} plan inputs")
+        for i, a in enumerate(values):
+            if not (hasattr(a, "shape") and hasattr(a, "dtype")):
+                raise PlanValidationError(f"arrays[{i}] must provide shape and dtype")
+        for i, (a, t) in enumerate(zip(values, legs)):
+            want = tuple(self._p.net["size_dict"][l] for l in t)
+            got = tuple(operator.index(d) for d in a.shape)
+            if got != want:
+                raise PlanValidationError(f"arrays[{i}] shape {got} does not match {want}")
+        if not _same_dtype(a.dtype for a in values):
+            raise PlanValidationError("all arrays must use the same dtype")
+
     def validate(self, inputs=None, output=None, size_dict=None, *, arrays=None):
-        """Check a caller network (dense labels) and array shapes / dtypes against
-        the plan (_execution_plan.py:402-453)."""
-        given = (inputs is not None, output is not None, size_dict is not None)
-        if any(given) and not all(given):
-            raise TypeError("inputs, output, and size_dict must be provided together")
-        if all(given):
-            cand = ([list(t) for t in inputs], list(output), sorted(dict(size_dict).items()))
-            if cand != (self._net["inputs"], self._net["output"], sorted(self._net["size_dict"].items())):
-                raise PlanValidationError("caller network does not match the embedded plan network")
+        """Check a caller network (dense labels, given all together) and the
+        arrays' count, shapes and common dtype against the plan; raises
+        PlanValidationError on the first mismatch."""
+        given = [x is not None for x in (inputs, output, size_dict)]
+        if any(given):
+            if not all(given):
+                raise TypeError("inputs, output, and size_dict must be provided together")
+            self._check_caller_network(inputs, output, size_dict)
         if arrays is not None:
-            try:
-                values = tuple(arrays)
-            except TypeError:
-                raise PlanValidationError("arrays must be a sequence") from None
-            if len(values) != len(self._net["inputs"]):
-                raise PlanValidationError(f"arrays length {len(values)} does not match "
-                                          f"{len(self._net['inputs'])} plan inputs")
-            dt = None
-            for i, (a, t) in enumerate(zip(values, self._net["inputs"])):
-                if not hasattr(a, "shape") or not hasattr(a, "dtype"):
-                    raise PlanValidationError(f"arrays[{i}] must provide shape and dtype")
-                shape = tuple(operator.index(d) for d in a.shape)
-                want = tuple(self._net["size_dict"][l] for l in t)
-                if shape != want:
-                    raise PlanValidationError(f"arrays[{i}] shape {shape} does not match {want}")
-                if dt is None:
-                    dt = str(a.dtype)
-                elif str(a.dtype) != dt:
-                    raise PlanValidationError("all arrays must use the same dtype")
+            self._check_arrays(arrays)
 
     def to_linear_path(self):
         """The opt_einsum-style linear path."""
-        return tuple(_planner.ssa_to_linear(self._ssa_path, len(self._net["inputs"])))
+        return tuple(_planner.ssa_to_linear(self._p.ssa_path, len(self._p.net["inputs"])))
 
     def compile(self):
         """A `CompiledContraction` of an unsliced plan for repeated execution."""
-        if self._sliced:
+        if self._p.sliced:
             raise PlanValidationError("sliced plans cannot be compiled by CompiledContraction; "
                                       "call plan.execute(...)")
         from .api import CompiledContraction
-        return CompiledContraction.compile(self._net["inputs"], self._net["output"], self._net["size_dict"],
-                                           ssa_path=self._ssa_path)
+        net = self._p.net
+        return CompiledContraction.compile(net["inputs"], net["output"], net["size_dict"], ssa_path=self._p.ssa_path)
 
     def execute(self, arrays, *, return_info=False, **opts):
         """Run the plan on the GPU: internal slices summed, output blocks assembled,
@@ -477,24 +497,26 @@ class ExecutionPlan:
         from .api import contract_sliced
         arrays = tuple(arrays)
         self.validate(arrays=arrays)
+        net, sliced = self._p.net, self._p.sliced
         t0 = time.perf_counter()
-        if not self._sliced:
-            result = self.compile().execute(arrays, **opts)
+        if sliced:
+            result = contract_sliced(net["inputs"], net["output"], net["size_dict"], arrays, list(sliced),
+                                     ssa_path=self._p.ssa_path, **opts)
         else:
-            result = contract_sliced(self._net["inputs"], self._net["output"], self._net["size_dict"], arrays,
-                                     list(self._sliced), ssa_path=self._ssa_path, **opts)
+            result = self.compile().execute(arrays, **opts)
         if not return_info:
             return result
         return result, {"plan_schema": SCHEMA, "plan_schema_version": SCHEMA_VERSION, "execution_backend": "hip",
-                        "execution_implementation": "arctn_amd.contract_sliced" if self._sliced
+                        "execution_implementation": "arctn_amd.contract_sliced" if sliced
                         else "arctn_amd.CompiledContraction", "execution_wall_s": time.perf_counter() - t0,
-                        "sliced_legs": list(self._sliced)}
+                        "sliced_legs": list(sliced)}
 
     def __eq__(self, other):
         if not isinstance(other, ExecutionPlan):
             return NotImplemented
-        return self.to_dict() == other.to_dict()
+        return self._p.doc == other._p.doc
 
     def __repr__(self):
-        return (f"ExecutionPlan(n_inputs={len(self._net['inputs'])}, n_steps={len(self._ssa_path)}, "
-                f"n_sliced_legs={len(self._sliced)}, target_size={self._target!r})")
+        p = self._p
+        return (f"ExecutionPlan(n_inputs={len(p.net['inputs'])}, n_steps={len(p.ssa_path)}, "
+                f"n_sliced_legs={len(p.sliced)}, target_size={p.target!r})")
