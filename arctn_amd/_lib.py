@@ -81,6 +81,8 @@ _SIGS = {
                                 C.c_size_t, _VP, _VP, _VP]),
     "atn_gemm": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_size_t, C.c_size_t, C.c_size_t, C.c_size_t, _VP,
                            C.c_size_t, _VP, C.c_size_t, _VP, C.c_size_t, _VP]),
+    "atn_gemm_noalloc": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_size_t, C.c_size_t, C.c_size_t, C.c_size_t,
+                                   _VP, C.c_size_t, _VP, C.c_size_t, _VP, C.c_size_t, _VP]),
     "atn_grouped_execute": (C.c_int, [C.POINTER(_VP), C.c_size_t, C.c_int, C.c_int, C.c_int,
                                       C.POINTER(C.POINTER(_VP)), C.POINTER(_VP), C.POINTER(C.c_double), _VP,
                                       C.POINTER(Report)]),

@@ -2,6 +2,7 @@
 // failure becomes a negative status and a thread-local message.
 #include <cstdlib>
 #include <memory>
+#include <mutex>
 
 #include "../../include/arctn_hip.h"
 #include "executor.hpp"
@@ -295,22 +296,57 @@ int atn_tensor_op(int dtype, int op, size_t rank, const size_t* shape, const siz
   });
 }
 
+}  // extern "C"
+
+// Arguments of atn_gemm / atn_gemm_noalloc, validated before anything is launched.
+static GemmArgs gemm_args(int dtype, int precision, int kernel, size_t batch, size_t m, size_t k, size_t n,
+                          const void* d_a, size_t stride_a, const void* d_b, size_t stride_b, void* d_c,
+                          size_t stride_c) {
+  check_dtype(dtype);
+  if (!batch || !m || !k || !n) throw Error("GEMM dimensions must be positive");
+  if (kernel < -1 || kernel > 8) throw Error("unknown GEMM kernel");
+  if (precision < 0 || precision > 3) throw Error("unsupported precision code");
+  GemmArgs g{};
+  g.a = d_a, g.b = d_b, g.c = d_c;
+  g.m = (int64_t)m, g.k = (int64_t)k, g.n = (int64_t)n;
+  g.lda = g.k, g.ldb = g.n, g.ldc = g.n;
+  g.nbo = (int64_t)batch, g.nbi = 1;
+  g.sao = (int64_t)stride_a, g.sbo = (int64_t)stride_b, g.sco = (int64_t)stride_c;
+  g.scratch_bytes = gemm_scratch_bytes(dtype, precision, kernel, g);
+  return g;
+}
+
+// Scratch of atn_gemm_noalloc: a fixed device array, one copy per GPU, held
+// until the process ends. Host code never passes `scratch` itself to a kernel
+// (on the host the name is only a stand-in); it uses the device address that
+// hipGetSymbolAddress returns for the current GPU, looked up once per device.
+static __device__ __align__(256) unsigned char scratch[ATN_NOALLOC_SCRATCH_BYTES];
+
+static std::mutex g_scratch_mu;
+static void* g_scratch_addr[64];
+
+static void* noalloc_scratch() {
+  int dev = 0;
+  ATN_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= 64) throw Error(cat("device ordinal ", dev, " out of range"));
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_addr[dev]) {
+    void* p = nullptr;
+    ATN_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) throw Error("hipGetSymbolAddress returned no address for the scratch array", ATN_ERR_DEVICE);
+    g_scratch_addr[dev] = p;
+  }
+  return g_scratch_addr[dev];
+}
+
+extern "C" {
+
 int atn_gemm(int dtype, int precision, int kernel, size_t batch, size_t m, size_t k, size_t n, const void* d_a,
              size_t stride_a, const void* d_b, size_t stride_b, void* d_c, size_t stride_c, void* stream) {
   return guard([&] {
-    check_dtype(dtype);
-    if (!batch || !m || !k || !n) throw Error("GEMM dimensions must be positive");
-    GemmArgs g{};
-    g.a = d_a, g.b = d_b, g.c = d_c;
-    g.m = (int64_t)m, g.k = (int64_t)k, g.n = (int64_t)n;
-    g.lda = g.k, g.ldb = g.n, g.ldc = g.n;
-    g.nbo = (int64_t)batch, g.nbi = 1;
-    g.sao = (int64_t)stride_a, g.sbo = (int64_t)stride_b, g.sco = (int64_t)stride_c;
-    if (kernel < -1 || kernel > 8) throw Error("unknown GEMM kernel");
-    if (precision < 0 || precision > 3) throw Error("unsupported precision code");
+    GemmArgs g = gemm_args(dtype, precision, kernel, batch, m, k, n, d_a, stride_a, d_b, stride_b, d_c, stride_c);
     hipStream_t s = static_cast<hipStream_t>(stream);
     void* scr = nullptr;
-    g.scratch_bytes = gemm_scratch_bytes(dtype, precision, kernel, g);
     ATN_HIP(hipMallocAsync(&scr, g.scratch_bytes, s));
     g.scratch = scr;
     try {
@@ -320,6 +356,19 @@ int atn_gemm(int dtype, int precision, int kernel, size_t batch, size_t m, size_
       throw;
     }
     ATN_HIP(hipFreeAsync(scr, s));
+  });
+}
+
+int atn_gemm_noalloc(int dtype, int precision, int kernel, size_t batch, size_t m, size_t k, size_t n,
+                     const void* d_a, size_t stride_a, const void* d_b, size_t stride_b, void* d_c, size_t stride_c,
+                     void* stream) {
+  return guard([&] {
+    GemmArgs g = gemm_args(dtype, precision, kernel, batch, m, k, n, d_a, stride_a, d_b, stride_b, d_c, stride_c);
+    if (g.scratch_bytes > ATN_NOALLOC_SCRATCH_BYTES)
+      throw Error(cat("GEMM needs ", g.scratch_bytes, " bytes of scratch; atn_gemm_noalloc has ",
+                      (size_t)ATN_NOALLOC_SCRATCH_BYTES));
+    g.scratch = noalloc_scratch();
+    launch_gemm(dtype, precision, kernel, g, static_cast<hipStream_t>(stream));
   });
 }
 

@@ -197,6 +197,21 @@ int atn_tensor_op(int dtype, int op, size_t rank, const size_t* shape, const siz
 int atn_gemm(int dtype, int precision, int kernel, size_t batch, size_t m, size_t k, size_t n, const void* d_a,
              size_t stride_a, const void* d_b, size_t stride_b, void* d_c, size_t stride_c, void* stream);
 
+/* No counterpart in the reference. atn_gemm for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc): same arguments, same kernels, same results. The
+ * scratch a kernel choice needs (split-K partial sums, pre-split 16-bit operand
+ * halves, FP16X2 operand maxima) comes from a fixed device array of
+ * ATN_NOALLOC_SCRATCH_BYTES per GPU, held until the process ends, instead of
+ * hipMallocAsync; the call allocates nothing. A GEMM needing more scratch than
+ * that fails with ATN_ERR_INVALID before anything is launched. Every call on
+ * one GPU shares that GPU's array: calls that overlap in time on one GPU (other
+ * streams, other threads) race. */
+#define ATN_NOALLOC_SCRATCH_BYTES ((size_t)64 << 20)
+int atn_gemm_noalloc(int dtype, int precision, int kernel, size_t batch, size_t m, size_t k, size_t n,
+                     const void* d_a, size_t stride_a, const void* d_b, size_t stride_b, void* d_c, size_t stride_c,
+                     void* stream);
+
 /* Grouped execution of many independent unsliced plans (uneven instances):
  * every plan's steps run in lockstep rounds, each round one grouped launch per
  * kernel kind. inputs[p] is the d_inputs array of plan p; outputs[p] its output.
