@@ -252,11 +252,14 @@ class CompiledContraction:
         return _finish(out, numpy_in)
 
     def execute_stripped(self, arrays, **opts):
-        """(mantissa, log10_exponent): result = mantissa * 10**log10_exponent.
+        """(mantissa, log10_exponent): result = mantissa * 10**log10_exponent,
+        the pair of CompiledContraction::execute_stripped (compiled.rs:376-381, 412-430).
 
-        Every step result is renormalised by a power of two (exact), unlike
-        the reference's division by the exact max magnitude (compiled.rs:412-430);
-        mantissa * 10**exponent is the same value.
+        The last step's result is divided by its exact max magnitude (f64 |z|,
+        as the reference), so the mantissa before finalize has max magnitude 1,
+        and the exponent is the sum of log10 of every factor. Intermediate steps
+        are rescaled by powers of two only when their max leaves [2^-32, 2^32]
+        (exact, and the same value as the reference's per-step division).
         """
         dev, code, numpy_in = self._prepare(arrays)
         try:

@@ -306,6 +306,7 @@ static GemmArgs gemm_args(int dtype, int precision, int kernel, size_t batch, si
   if (!batch || !m || !k || !n) throw Error("GEMM dimensions must be positive");
   if (kernel < -1 || kernel > 8) throw Error("unknown GEMM kernel");
   if (precision < 0 || precision > 3) throw Error("unsupported precision code");
+  if (precision == C3M && dtype != C64) throw Error("precision c3m applies to complex64 only");
   GemmArgs g{};
   g.a = d_a, g.b = d_b, g.c = d_c;
   g.m = (int64_t)m, g.k = (int64_t)k, g.n = (int64_t)n;

@@ -446,11 +446,17 @@ void Executor::gen_steps(bool body, size_t nb, std::vector<Ref>& slots, std::vec
       gen_gemm(st, nbC, va, vb, a, b, c, &mb, brow, bcol, bcol_run, am, bm, brow_run, arow, acol);
       size_t numel = nbC * st.bt * st.m * st.n;
       Ref ex{Ref::WS, expo_buf_};
-      const int keep = si + 1 == P.cp.steps.size() ? 0 : STRIP_KEEP;
       use(c);
-      emit([=](const Ctx& cx) {
-        launch_scale_pow2(dt, cx.ptr(c), numel, (const uint64_t*)cx.ptr(mb), (int64_t*)cx.ptr(ex), cx.s, keep);
-      });
+      if (si + 1 == P.cp.steps.size()) {
+        // last step: the reference's exact division by the max magnitude
+        emit([=](const Ctx& cx) {
+          launch_strip_exact(dt, cx.ptr(c), numel, (uint64_t*)cx.ptr(mb), (double*)((char*)cx.ptr(ex) + 8), cx.s);
+        });
+      } else {
+        emit([=](const Ctx& cx) {
+          launch_scale_pow2(dt, cx.ptr(c), numel, (const uint64_t*)cx.ptr(mb), (int64_t*)cx.ptr(ex), cx.s, STRIP_KEEP);
+        });
+      }
     } else if (want_max_[st.out]) {
       Ref mc{Ref::WS, newbuf(8)};
       use(mc);
@@ -532,11 +538,13 @@ void Executor::build(size_t B) {
   cur_ = &pre_;
   scratch_buf_ = newbuf(256);  // FP16X2 scales, split-K partials (grown per GEMM site); lives for the whole run
   if (o_.strip) {
+    // expo: int64 power-of-two exponent of the intermediate rescales, then the
+    // f64 log10 of the last step's exact max
     maxbits_buf_ = newbuf(8);
-    expo_buf_ = newbuf(8);
+    expo_buf_ = newbuf(16);
     Ref ex{Ref::WS, expo_buf_};
     use(ex);
-    emit([=](const Ctx& c) { launch_fill_zero(c.ptr(ex), 8, c.s); });
+    emit([=](const Ctx& c) { launch_fill_zero(c.ptr(ex), 16, c.s); });
   }
   std::vector<Ref> slots(nslot);
   std::vector<std::vector<size_t>> shapes(nslot);
@@ -683,7 +691,7 @@ Executor::Executor(std::shared_ptr<const Plan> plan, const ExecOptions& o) : P_(
   const Plan& P = *P_;
   es_ = (int)dtype_size(o_.dtype);
   if (o_.precision != NATIVE) {
-    if (o_.precision == C3M) throw Error("precision c3m is not available in this build");
+    if (o_.precision == C3M && o_.dtype != C64) throw Error("precision c3m applies to complex64 only");
     if ((o_.precision == BF16X3 || o_.precision == FP16X2) && !(o_.dtype == C64 || o_.dtype == F32))
       throw Error("split precision applies to float32 / complex64 only");
   }
@@ -976,9 +984,12 @@ void Executor::run(const void* const* inputs, void* out, hipStream_t s, Report* 
     rep->workspace_bytes = ws_bytes_;
     rep->log10_exponent = 0;
     if (o_.strip) {
-      int64_t e = 0;
-      ATN_HIP(hipMemcpy(&e, ws_ + offs_[expo_buf_], 8, hipMemcpyDeviceToHost));
-      rep->log10_exponent = (double)e * std::log10(2.0);
+      struct {
+        int64_t pow2;
+        double log10_last;
+      } e{};
+      ATN_HIP(hipMemcpy(&e, ws_ + offs_[expo_buf_], 16, hipMemcpyDeviceToHost));
+      rep->log10_exponent = (double)e.pow2 * std::log10(2.0) + e.log10_last;
     }
   }
 }

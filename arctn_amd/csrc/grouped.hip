@@ -95,12 +95,14 @@ template <> struct GA<float> {
   static __device__ float zero() { return 0.f; }
   static __device__ float mac(float c, float a, float b) { return c + a * b; }
   static __device__ double mag(float a) { return fabs((double)a); }
+  static __device__ double abs(float a) { return fabs((double)a); }
   static __device__ float scale(float a, double s) { return a * (float)s; }
 };
 template <> struct GA<double> {
   static __device__ double zero() { return 0.0; }
   static __device__ double mac(double c, double a, double b) { return c + a * b; }
   static __device__ double mag(double a) { return fabs(a); }
+  static __device__ double abs(double a) { return fabs(a); }
   static __device__ double scale(double a, double s) { return a * s; }
 };
 template <> struct GA<float2> {
@@ -109,6 +111,7 @@ template <> struct GA<float2> {
     return make_float2(c.x + (a.x * b.x - a.y * b.y), c.y + (a.x * b.y + a.y * b.x));
   }
   static __device__ double mag(float2 a) { return fmax(fabs((double)a.x), fabs((double)a.y)); }
+  static __device__ double abs(float2 a) { return hypot((double)a.x, (double)a.y); }
   static __device__ float2 scale(float2 a, double s) { return make_float2(a.x * (float)s, a.y * (float)s); }
 };
 template <> struct GA<double2> {
@@ -117,6 +120,7 @@ template <> struct GA<double2> {
     return make_double2(c.x + (a.x * b.x - a.y * b.y), c.y + (a.x * b.y + a.y * b.x));
   }
   static __device__ double mag(double2 a) { return fmax(fabs(a.x), fabs(a.y)); }
+  static __device__ double abs(double2 a) { return hypot(a.x, a.y); }
   static __device__ double2 scale(double2 a, double s) { return make_double2(a.x * s, a.y * s); }
 };
 
@@ -172,7 +176,11 @@ __global__ __launch_bounds__(256) void grouped_maxabs_kernel(const GStripDesc* _
     uint64_t e = (vb - p.block0) * 256ull + threadIdx.x;
     __shared__ double wmax[4];
     double m = 0.0;
-    if (e < p.numel) m = GA<T>::mag(reinterpret_cast<const T*>(gres(p.c, bases))[e]);
+    // a plan's last step is divided by its exact max magnitude, the others by powers of two
+    if (e < p.numel) {
+      const T v = reinterpret_cast<const T*>(gres(p.c, bases))[e];
+      m = p.final ? GA<T>::abs(v) : GA<T>::mag(v);
+    }
     for (int o = 32; o > 0; o >>= 1) {
       double t = __shfl_down(m, o, 64);
       m = t > m ? t : m;  // NaN never wins; a NaN result stays unscaled
@@ -191,19 +199,23 @@ __global__ __launch_bounds__(256) void grouped_maxabs_kernel(const GStripDesc* _
 template <class T>
 __global__ __launch_bounds__(256) void grouped_scale_kernel(const GStripDesc* __restrict__ d, int n, uint64_t total,
                                                             char* const* bases, const unsigned long long* slots,
-                                                            long long* expo) {
+                                                            long long* expo, double* log10_last) {
   for (uint64_t vb = blockIdx.x; vb < total; vb += gridDim.x) {
     const GStripDesc& p = find_problem(d, n, vb);
     double m = __longlong_as_double((long long)slots[p.plan]);
     if (!(m > 0.0) || !isfinite(m)) continue;
+    uint64_t e = (vb - p.block0) * 256ull + threadIdx.x;
+    T* c = reinterpret_cast<T*>(gres(p.c, bases));
+    if (p.final) {  // src/compiled.rs:412-430: x * from_real(1 / m), exponent + log10(m)
+      if (vb == p.block0 && threadIdx.x == 0) log10_last[p.plan] += log10(m);
+      if (e < p.numel) c[e] = GA<T>::scale(c[e], 1.0 / m);
+      continue;
+    }
     int ex;
     frexp(m, &ex);
-    if (!p.final && ex >= -STRIP_KEEP && ex <= STRIP_KEEP) continue;
+    if (ex >= -STRIP_KEEP && ex <= STRIP_KEEP) continue;
     if (vb == p.block0 && threadIdx.x == 0) expo[p.plan] += ex;
-    uint64_t e = (vb - p.block0) * 256ull + threadIdx.x;
-    if (e >= p.numel) continue;
-    T* c = reinterpret_cast<T*>(gres(p.c, bases));
-    c[e] = GA<T>::scale(c[e], ldexp(1.0, -ex));
+    if (e < p.numel) c[e] = GA<T>::scale(c[e], ldexp(1.0, -ex));
   }
 }
 
@@ -475,7 +487,7 @@ struct GroupedExecutor::Group {
 
 GroupedExecutor::GroupedExecutor(std::vector<std::shared_ptr<const Plan>> plans, int dt, bool strip, int prec)
     : plans_(std::move(plans)), dt_(dt), strip_(strip), prec_(prec) {
-  if (prec == C3M) throw Error("precision c3m is not available in this build");
+  if (prec == C3M && dt != C64) throw Error("precision c3m applies to complex64 only");
   if (prec != NATIVE && !(dt == F32 || dt == C64)) throw Error("split precision applies to float32 / complex64 only");
   const size_t np = plans_.size();
   size_t max_steps = 0;
@@ -569,9 +581,10 @@ GroupedExecutor::GroupedExecutor(std::vector<std::shared_ptr<const Plan>> plans,
         if (L.n_strip) L.strip_off = append(strips.data(), strips.size() * sizeof(GStripDesc));
         for (auto& sg : L.mid) G.gemm_scratch = std::max(G.gemm_scratch, gemm_scratch_bytes(dt, prec, GEMM_AUTO, sg.gemm));
       }
-      G.slots_off = append(nullptr, 2 * ng * 8);
+      // per plan: max slot, power-of-two exponent, log10 of the last step's exact max
+      G.slots_off = append(nullptr, 3 * ng * 8);
       G.expo_off = G.slots_off + ng * 8;
-      std::memset(G.blob.data() + G.slots_off, 0, 2 * ng * 8);
+      std::memset(G.blob.data() + G.slots_off, 0, 3 * ng * 8);
       G.gemm_scratch_off = append(nullptr, G.gemm_scratch);
       G.bases_off = append(nullptr, G.nb * sizeof(char*));
       ATN_HIP(hipMalloc((void**)&G.dev, G.blob.size()));
@@ -602,6 +615,7 @@ void GroupedExecutor::run(const std::vector<const void* const*>& inputs, const s
   const int dt = dt_;
   hipEvent_t e0 = nullptr, e1 = nullptr;
   std::vector<long long> expo_host(np, 0);
+  std::vector<double> last_host(np, 0.0);
   const size_t l0 = g_launch_count;
   try {
     if (rep) {
@@ -630,9 +644,10 @@ void GroupedExecutor::run(const std::vector<const void* const*>& inputs, const s
       char* const* dbases = reinterpret_cast<char* const*>(dev + G.bases_off);
       auto* slots = reinterpret_cast<unsigned long long*>(dev + G.slots_off);
       auto* expo = reinterpret_cast<long long*>(dev + G.expo_off);
-      if (strip_) {
-        hipLaunchKernelGGL(grouped_zero_kernel, dim3((unsigned)((ng + 255) / 256)), dim3(256), 0, s,
-                           reinterpret_cast<unsigned long long*>(expo), (uint64_t)ng);
+      auto* log10_last = reinterpret_cast<double*>(dev + G.expo_off + ng * 8);
+      if (strip_) {  // exponents and log10 parts
+        hipLaunchKernelGGL(grouped_zero_kernel, dim3((unsigned)((2 * ng + 255) / 256)), dim3(256), 0, s,
+                           reinterpret_cast<unsigned long long*>(expo), (uint64_t)(2 * ng));
         ATN_HIP(hipGetLastError());
         ++g_launch_count;
       }
@@ -695,17 +710,24 @@ void GroupedExecutor::run(const std::vector<const void* const*>& inputs, const s
             if (L.strip_blocks) {
               hipLaunchKernelGGL(grouped_scale_kernel<T>, grid(L.strip_blocks), dim3(256), 0, s,
                                  reinterpret_cast<const GStripDesc*>(dev + L.strip_off), L.n_strip, L.strip_blocks, dbases,
-                                 (const unsigned long long*)slots, expo);
+                                 (const unsigned long long*)slots, expo, log10_last);
               ++g_launch_count;
             }
-            for (auto& sg : L.big_strips)
-              launch_scale_pow2(dt, at(sg.in), sg.A, (const uint64_t*)(slots + sg.plan), (int64_t*)(expo + sg.plan), s,
-                                sg.final ? 0 : STRIP_KEEP);
+            for (auto& sg : L.big_strips) {
+              if (sg.final)
+                launch_strip_exact(dt, at(sg.in), sg.A, (uint64_t*)(slots + sg.plan), log10_last + sg.plan, s);
+              else
+                launch_scale_pow2(dt, at(sg.in), sg.A, (const uint64_t*)(slots + sg.plan), (int64_t*)(expo + sg.plan), s,
+                                  STRIP_KEEP);
+            }
             ATN_HIP(hipGetLastError());
           }
         }
       });
-      if (strip_ && log10_exp) ATN_HIP(hipMemcpyAsync(expo_host.data() + G.g0, expo, ng * 8, hipMemcpyDeviceToHost, s));
+      if (strip_ && log10_exp) {
+        ATN_HIP(hipMemcpyAsync(expo_host.data() + G.g0, expo, ng * 8, hipMemcpyDeviceToHost, s));
+        ATN_HIP(hipMemcpyAsync(last_host.data() + G.g0, log10_last, ng * 8, hipMemcpyDeviceToHost, s));
+      }
     }
     if (rep) {
       ATN_HIP(hipEventRecord(e1, s));
@@ -732,7 +754,7 @@ void GroupedExecutor::run(const std::vector<const void* const*>& inputs, const s
   if (e0) (void)hipEventDestroy(e0);
   if (e1) (void)hipEventDestroy(e1);
   if (log10_exp)
-    for (size_t p = 0; p < np; ++p) log10_exp[p] = strip_ ? (double)expo_host[p] * std::log10(2.0) : 0.0;
+    for (size_t p = 0; p < np; ++p) log10_exp[p] = strip_ ? (double)expo_host[p] * std::log10(2.0) + last_host[p] : 0.0;
 }
 
 void grouped_execute(const std::vector<std::shared_ptr<const Plan>>& plans, int dt, bool strip, int prec,

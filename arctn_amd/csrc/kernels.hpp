@@ -93,6 +93,9 @@ void launch_maxabs(int dt, const void* x, size_t numel, uint64_t* maxbits, hipSt
 // [-keep, keep] — the power-of-two scale is only needed against over/underflow.
 void launch_scale_pow2(int dt, void* x, size_t numel, const uint64_t* maxbits, int64_t* expo, hipStream_t s,
                        int keep = 0);
+// The last step of a stripped run: divide by the exact max magnitude m (Scalar::abs,
+// f64) and add log10(m) to log10_exp (src/compiled.rs:412-430); maxbits is scratch.
+void launch_strip_exact(int dt, void* x, size_t numel, uint64_t* maxbits, double* log10_exp, hipStream_t s);
 // Exponent bound below which intermediate results of a stripped run are left unscaled.
 constexpr int STRIP_KEEP = 32;
 

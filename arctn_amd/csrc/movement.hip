@@ -25,25 +25,37 @@ template <> struct Bits<4> { using T = uint32_t; };
 template <> struct Bits<8> { using T = uint64_t; };
 template <> struct Bits<16> { using T = uint4; };
 
+// mag: max(|re|, |im|) (power-of-two strip scales); abs: the magnitude of
+// Scalar::abs (src/tensor.rs:125-127, 173-175, 245-247, 317-319): |x| widened to
+// f64, hypot of the widened parts for complex; inv: the reciprocal 1/m as the
+// element type's real part (Scalar::from_real(1.0 / factor), compiled.rs:423).
 template <class T> struct Arith;
 template <> struct Arith<float> {
+  using R = float;
   static __device__ float add(float a, float b) { return a + b; }
   static __device__ double mag(float a) { return fabs((double)a); }
+  static __device__ double abs(float a) { return fabs((double)a); }
   static __device__ float scale(float a, float s) { return a * s; }
 };
 template <> struct Arith<double> {
+  using R = double;
   static __device__ double add(double a, double b) { return a + b; }
   static __device__ double mag(double a) { return fabs(a); }
+  static __device__ double abs(double a) { return fabs(a); }
   static __device__ double scale(double a, double s) { return a * s; }
 };
 template <> struct Arith<float2> {
+  using R = float;
   static __device__ float2 add(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
   static __device__ double mag(float2 a) { return fmax(fabs((double)a.x), fabs((double)a.y)); }
+  static __device__ double abs(float2 a) { return hypot((double)a.x, (double)a.y); }
   static __device__ float2 scale(float2 a, float s) { return make_float2(a.x * s, a.y * s); }
 };
 template <> struct Arith<double2> {
+  using R = double;
   static __device__ double2 add(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
   static __device__ double mag(double2 a) { return fmax(fabs(a.x), fabs(a.y)); }
+  static __device__ double abs(double2 a) { return hypot(a.x, a.y); }
   static __device__ double2 scale(double2 a, double s) { return make_double2(a.x * s, a.y * s); }
 };
 
@@ -723,6 +735,45 @@ __global__ void scale_pow2_kernel<double>(double* __restrict__ x, uint64_t n, co
   if (blockIdx.x == 0 && threadIdx.x == 0) *expo += e;
   double s = ldexp(1.0, -e);
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) x[i] *= s;
+}
+
+// Exact normalisation of a stripped run's last step (src/compiled.rs:412-430):
+// m = max over elements of Scalar::abs (f64), then x *= T::from_real(1 / m) and
+// the log10 exponent gains log10(m); nothing changes when m is 0 or not finite.
+template <class T>
+__global__ void max_abs_exact_kernel(const T* __restrict__ x, uint64_t n, unsigned long long* maxbits) {
+  double m = 0.0;
+  for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (uint64_t)gridDim.x * blockDim.x) {
+    double v = Arith<T>::abs(x[e]);
+    m = v > m ? v : m;
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    double t = __shfl_down(m, o, 64);
+    m = t > m ? t : m;
+  }
+  if ((threadIdx.x & 63) == 0 && m > 0.0) atomicMax(maxbits, (unsigned long long)__double_as_longlong(m));
+}
+
+template <class T>
+__global__ void scale_exact_kernel(T* __restrict__ x, uint64_t n, const unsigned long long* maxbits, double* log10_exp) {
+  const double m = __longlong_as_double((long long)*maxbits);
+  if (!(m > 0.0) || !isfinite(m)) return;
+  if (blockIdx.x == 0 && threadIdx.x == 0) *log10_exp += log10(m);
+  const typename Arith<T>::R inv = (typename Arith<T>::R)(1.0 / m);
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+    x[i] = Arith<T>::scale(x[i], inv);
+}
+
+void launch_strip_exact(int dt, void* x, size_t numel, uint64_t* maxbits, double* log10_exp, hipStream_t s) {
+  dispatch_dt(dt, [&](auto tag) {
+    using T = decltype(tag);
+    g_last_kind = K_STRIP;
+    LAUNCH(fill_zero_kernel, dim3(1), dim3(64), 0, s, (uint32_t*)maxbits, (uint64_t)2);
+    LAUNCH(max_abs_exact_kernel<T>, grid_for(numel, 256, 2048), 256, 0, s, (const T*)x, (uint64_t)numel,
+           (unsigned long long*)maxbits);
+    LAUNCH(scale_exact_kernel<T>, grid_for(numel, 256), 256, 0, s, (T*)x, (uint64_t)numel,
+           (const unsigned long long*)maxbits, log10_exp);
+  });
 }
 
 void launch_maxabs(int dt, const void* x, size_t numel, uint64_t* maxbits, hipStream_t s) {

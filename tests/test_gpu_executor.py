@@ -169,6 +169,33 @@ def test_stripped_reconstructs_plain(cuda):
     assert e > 199 and abs(float(m) * 10.0 ** e - 1e200) / 1e200 < 1e-12
     m, e = cc.execute_stripped([np.array(1e200 - 1e200j), np.array(1.0 + 0j)])
     assert abs(complex(m) * 10.0 ** e - (1e200 - 1e200j)) / abs(1e200 - 1e200j) < 1e-12
+    # the reference's pair (compiled.rs:412-430): |mantissa| = 1 at the exact max, exponent = log10 |z|
+    assert abs(abs(complex(m)) - 1.0) < 1e-15 and abs(e - np.log10(abs(1e200 - 1e200j))) < 1e-12
+
+
+@pytest.mark.parametrize("dtype", DTYPES)
+def test_stripped_pair_matches_oracle(cuda, dtype):
+    """execute_stripped returns the reference's (mantissa, log10_exponent) pair: the last
+    step divided by its exact max magnitude, the exponent the sum of log10 of the factors
+    (compiled.rs:376-381, 412-430), compared with the oracle's stripped mode."""
+    double = np.dtype(dtype) in (np.dtype(np.float64), np.dtype(np.complex128))
+    tol, span = (1e-12, 100) if double else (1e-5, 15)
+    for seed in range(12):
+        rng = np.random.default_rng(seed)
+        n = int(rng.integers(3, 9))
+        net = W.random_connected(n, int(rng.integers(0, n)), [2, 3, 4], int(rng.integers(0, 3)), seed)
+        path = P.greedy(net)
+        arrays = [a * 10.0 ** int(rng.integers(-span, span)) for a in W.random_arrays(net, dtype, seed)]
+        arrays = [np.asarray(a, dtype=dtype) for a in arrays]
+        cc = CompiledContraction.compile(net["inputs"], net["output"], net["size_dict"], ssa_path=path)
+        m, e = cc.execute_stripped(arrays)
+        wide = np.complex128 if np.iscomplexobj(arrays[0]) else np.float64
+        m_ref, e_ref = O.contract(net, [a.astype(wide) for a in arrays], path, "stripped")
+        assert m.shape == m_ref.shape and m.dtype == np.dtype(dtype)
+        assert abs(e - e_ref) <= tol * max(1.0, abs(e_ref)), (seed, e, e_ref)
+        assert np.abs(m - m_ref).max() <= 4 * tol * max(1.0, np.abs(m_ref).max()), seed
+        if abs(np.abs(m_ref).max() - 1.0) < 1e-12:  # finalize summed no residual leg
+            assert abs(np.abs(m).max() - 1.0) <= tol, seed
 
 
 def test_sliced_circuit_plain_batched_agree(cuda):
@@ -244,7 +271,7 @@ def test_slice_ranges_partial_sums(cuda):
     np.testing.assert_allclose(parts, whole.cpu().numpy(), rtol=1e-12, atol=1e-12)
 
 
-@pytest.mark.parametrize("precision", ["native", "bf16x3", "fp16x2"])
+@pytest.mark.parametrize("precision", ["native", "c3m", "bf16x3", "fp16x2"])
 def test_autotuned_batched_graph_forms(cuda, precision):
     """The autotuner's per-step kernel / tile choices keep the result within 1e-5."""
     net, arrays, _ = W.random_circuit_amplitude(4, 4, 8, 0)
@@ -287,7 +314,7 @@ def test_gather_tables_match_permuted_operands(cuda, dtype):
     want = complex(O.contract(net, [a.astype(np.complex128) for a in arrays], path, "compiled"))
     plan = Plan(net["inputs"], net["output"], net["size_dict"], path, sliced)
     dev = [torch.from_numpy(a).cuda() for a in arrays]
-    for prec in ("native", "bf16x3", "fp16x2"):
+    for prec in ("native", "c3m", "bf16x3", "fp16x2"):
         for gather in (True, False):
             got, _ = plan.run(dev, 2, batched=True, use_graph=True, precision=prec, gather=gather)
             assert abs(complex(got.cpu()) - want) / abs(want) < 1e-5, (prec, gather)
@@ -316,8 +343,10 @@ def test_forced_gemm_kernels_with_gather_tables(cuda, dtype):
         dev = [torch.from_numpy(a).cuda() for a in arrays]
         # BF16x3 drops products of 2^-16 relative size: forced on all ~300 steps of
         # the circuit (the heuristic keeps it to large steps) its error adds up
-        for prec, kernels, tol in (("native", (0, 1, 2, 3, 5, 6, 7, 8), 1e-5),
-                                   ("bf16x3", (1, 4, 8), 3e-4), ("fp16x2", (1, 4, 8), 1e-5)):
+        forms = [("native", (0, 1, 2, 3, 5, 6, 7, 8), 1e-5), ("bf16x3", (1, 4, 8), 3e-4), ("fp16x2", (1, 4, 8), 1e-5)]
+        if dtype == np.complex64:
+            forms.append(("c3m", (1, 5), 1e-5))
+        for prec, kernels, tol in forms:
             for kern in kernels:
                 for gather in (True, False):
                     got, _ = plan.run(dev, code, batched=True, precision=prec, gather=gather, gemm_kernel=kern)

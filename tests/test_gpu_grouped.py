@@ -92,6 +92,8 @@ def test_grouped_matches_single_plan_stripped(cuda):
         cc = CompiledContraction.compile(net["inputs"], net["output"], net["size_dict"], ssa_path=path)
         m1, e1 = cc.execute_stripped(arrays)
         assert abs(_value(m, e) - _value(m1, e1)) <= 1e-12 * abs(_value(m1, e1))
+        # the pair itself: scalar mantissa of magnitude 1, exponent log10 of the value
+        assert abs(abs(float(m)) - 1.0) < 1e-12 and abs(e - e1) < 1e-12
 
 
 def test_grouped_rejects_sliced_plans_and_mixed_dtypes(cuda):
