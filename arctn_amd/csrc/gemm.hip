@@ -703,6 +703,141 @@ __global__ __launch_bounds__(256) void gemm_mfma_kernel(GemmArgs g) {
   if constexpr (TRACK) publish_max((double)mx, g.cmax);
 }
 
+// ------------------------------------------------------- complex 3M GEMM --
+// complex64 C = A.B as three real products on the FP32 MFMA (Gauss's 3M):
+//   T1 = Ar.Br, T2 = Ai.Bi, T3 = (Ar + Ai).(Br + Bi);  Cr = T1 - T2, Ci = T3 - T1 - T2
+// 3 real multiply-adds per complex one instead of the 4 of the 2x2 expansion
+// (gemm_mfma_kernel with CPLX). The sums Ar + Ai and Br + Bi are formed in FP32
+// while staging; Ci comes out of the cancellation T3 - T1 - T2, so its error
+// bound grows with |Ar + Ai||Br + Bi| rather than |Ar||Bi| + |Ai||Br|.
+// Tile: BM rows x BN complex columns x BK complex k; 4 waves in a 2x2
+// arrangement, three accumulator planes per wave tile. LDS holds the three A
+// planes [BK][BM+pad] (transposed) and the three B planes [BK][BN+pad].
+// Same batch, split-k (part[z][M][n] complex), gather-table and max-tracking
+// conventions as gemm_mfma_kernel.
+template <int BM, int BN, int BK, bool TRACK, bool GATHER>
+__global__ __launch_bounds__(256) void gemm_c3m_kernel(GemmArgs g) {
+  constexpr int WM = BM / 2, WN = BN / 2;
+  constexpr int TM = WM / 16, TN = WN / 16;
+  constexpr int PAD = 16;
+  constexpr int SA = BM + PAD, SB = BN + PAD;
+  constexpr int A_PER = BM * BK / 256, B_PER = BK * BN / 256;
+  static_assert(256 % BN == 0 && 256 % BK == 0 && A_PER >= 1 && B_PER >= 1, "tile shape");
+  using V = v4f;
+  __shared__ float As[3][BK * SA];
+  __shared__ float Bs[3][BK * SB];
+
+  const int64_t M = g.m, N = g.n, K = g.k;  // complex units
+  const int tiles_n = (int)((N + BN - 1) / BN), tiles_m = (int)((M + BM - 1) / BM);
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm0 = (wave >> 1) * WM, wn0 = (wave & 1) * WN;
+  const uint64_t nz = (uint64_t)g.nbo * g.nbi;
+  const uint64_t ntile = (uint64_t)tiles_m * tiles_n;
+  const int64_t kbeg = g.kc ? (int64_t)blockIdx.z * g.kc : 0;
+  const int64_t kend = g.kc && kbeg + g.kc < K ? kbeg + g.kc : K;
+  float mx = 0;
+
+  for (uint64_t z = blockIdx.y; z < nz; z += gridDim.y) {
+    uint64_t zo = z / g.nbi, zi = z - zo * g.nbi;
+    const float2* A = (const float2*)g.a + (zo * g.sao + zi * g.sai);
+    const float2* B = (const float2*)g.b + (zo * g.sbo + zi * g.sbi);
+    float2* C = g.part ? (float2*)g.part + ((uint64_t)blockIdx.z * nz + z) * (uint64_t)(M * N)
+                       : (float2*)g.c + (zo * g.sco + zi * g.sci);
+    const int64_t ldo = g.part ? N : g.ldc;
+    for (uint64_t t = blockIdx.x; t < ntile; t += gridDim.x) {
+      int64_t m0, n0;
+      tile_origin(t, ntile, tiles_m, tiles_n, BM, BN, m0, n0);
+      V acc[3][TM][TN];
+#pragma unroll
+      for (int q = 0; q < 3; ++q)
+#pragma unroll
+        for (int i = 0; i < TM; ++i)
+#pragma unroll
+          for (int j = 0; j < TN; ++j) acc[q][i][j] = V{0, 0, 0, 0};
+      float2 ra[A_PER], rb[B_PER];
+      // a thread's B loads share one column (256 is a multiple of BN)
+      const int64_t bcn = n0 + tid % BN;
+      const bool bcn_ok = bcn < N;
+      const int64_t bco = bcn_ok ? (GATHER ? (int64_t)g.bcol[bcn] : bcn) : 0;
+      auto load = [&](int64_t k0) {
+#pragma unroll
+        for (int u = 0; u < A_PER; ++u) {
+          const int e = tid + 256 * u, col = e % BK, row = e / BK;
+          const int64_t gr = m0 + row, gc = k0 + col;
+          ra[u] = (gr < M && gc < kend) ? A[gr * g.lda + gc] : make_float2(0.f, 0.f);
+        }
+#pragma unroll
+        for (int u = 0; u < B_PER; ++u) {
+          const int e = tid + 256 * u, p = e / BN;
+          const int64_t gp = k0 + p;
+          const bool ok = gp < kend && bcn_ok;
+          rb[u] = ok ? B[(GATHER ? (int64_t)g.brow[gp] : gp * g.ldb) + bco] : make_float2(0.f, 0.f);
+        }
+      };
+      auto store = [&]() {
+#pragma unroll
+        for (int u = 0; u < A_PER; ++u) {
+          const int e = tid + 256 * u, col = e % BK, row = e / BK;
+          As[0][col * SA + row] = ra[u].x;
+          As[1][col * SA + row] = ra[u].y;
+          As[2][col * SA + row] = ra[u].x + ra[u].y;
+        }
+#pragma unroll
+        for (int u = 0; u < B_PER; ++u) {
+          const int e = tid + 256 * u, p = e / BN, j = e % BN;
+          Bs[0][p * SB + j] = rb[u].x;
+          Bs[1][p * SB + j] = rb[u].y;
+          Bs[2][p * SB + j] = rb[u].x + rb[u].y;
+        }
+      };
+      const int64_t nk = (kend - kbeg + BK - 1) / BK;
+      load(kbeg);
+      store();
+      __syncthreads();
+      for (int64_t kt = 0; kt < nk; ++kt) {
+        if (kt + 1 < nk) load(kbeg + (kt + 1) * BK);
+#pragma unroll
+        for (int kk = 0; kk < BK; kk += 4) {
+          const int kr = kk + (lane >> 4), cl = lane & 15;
+#pragma unroll
+          for (int q = 0; q < 3; ++q) {
+            float af[TM], bf[TN];
+#pragma unroll
+            for (int i = 0; i < TM; ++i) af[i] = As[q][kr * SA + wm0 + i * 16 + cl];
+#pragma unroll
+            for (int j = 0; j < TN; ++j) bf[j] = Bs[q][kr * SB + wn0 + j * 16 + cl];
+#pragma unroll
+            for (int i = 0; i < TM; ++i)
+#pragma unroll
+              for (int j = 0; j < TN; ++j) acc[q][i][j] = Mfma<float>::op(af[i], bf[j], acc[q][i][j]);
+          }
+        }
+        __syncthreads();
+        if (kt + 1 < nk) {
+          store();
+          __syncthreads();
+        }
+      }
+      const int cl = lane & 15;
+#pragma unroll
+      for (int i = 0; i < TM; ++i)
+#pragma unroll
+        for (int j = 0; j < TN; ++j)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const int64_t row = m0 + wm0 + i * 16 + Mfma<float>::row(lane, r), col = n0 + wn0 + j * 16 + cl;
+            if (row < M && col < N) {
+              const float t1 = acc[0][i][j][r], t2 = acc[1][i][j][r], t3 = acc[2][i][j][r];
+              const float2 v = make_float2(t1 - t2, (t3 - t1) - t2);
+              C[row * ldo + col] = v;
+              if constexpr (TRACK) fold_max(mx, v);
+            }
+          }
+    }
+  }
+  if constexpr (TRACK) publish_max((double)mx, g.cmax);
+}
+
 // ------------------------------------------------------ split-precision GEMM --
 // FP32 (or complex64 via the 2x2 expansion of B) products on the 16-bit MFMA:
 // every operand value x is split into hi = round16(x * 2^s) and
@@ -1442,6 +1577,41 @@ static void run_mfma(int dt, const GemmArgs& g0, hipStream_t s) {
   }
 }
 
+template <int BM>
+static void run_c3m_t(const GemmArgs& g0, hipStream_t s) {
+  constexpr int BN = 64, BK = 16;
+  GemmArgs g = g0;
+  const KSplit ks = plan_mfma_split(C64, NATIVE, g, scratch_avail(g), BM);
+  g.kc = ks.n_split > 1 ? ks.k_chunk : 0;
+  g.part = ks.n_split > 1 ? (char*)g.scratch + 256 : nullptr;
+  uint64_t tiles = (uint64_t)((g.m + BM - 1) / BM) * (uint64_t)((g.n + BN - 1) / BN);
+  dim3 grid((unsigned)std::min<uint64_t>(tiles, 1u << 20), zgrid(g), (unsigned)ks.n_split);
+  if (g.cmax) grid.x = std::min<unsigned>(grid.x, std::max<unsigned>(1, TRACK_WORKGROUPS / (grid.y * grid.z)));
+  g_last_kind = K_GEMM_MFMA_C3M;
+  const bool track = g.cmax && !g.part;
+  if (g.bcol) {
+    if (track) LAUNCH((gemm_c3m_kernel<BM, BN, BK, true, true>), grid, dim3(256), 0, s, g);
+    else LAUNCH((gemm_c3m_kernel<BM, BN, BK, false, true>), grid, dim3(256), 0, s, g);
+  } else {
+    if (track) LAUNCH((gemm_c3m_kernel<BM, BN, BK, true, false>), grid, dim3(256), 0, s, g);
+    else LAUNCH((gemm_c3m_kernel<BM, BN, BK, false, false>), grid, dim3(256), 0, s, g);
+  }
+  if (ks.n_split > 1) {
+    SplitK sk{};
+    sk.n_split = ks.n_split;
+    sk.part = g.part;
+    const int64_t n_out = g.nbo * g.nbi * g.m * g.n;
+    dim3 rg((unsigned)std::min<int64_t>((n_out + 255) / 256, 65536));
+    LAUNCH((splitk_reduce_kernel<float2>), rg, dim3(256), 0, s, g, sk);
+  }
+}
+
+// Complex 3M on the FP32 MFMA: 64-row tiles up to m = 64, else 128.
+static void run_c3m(const GemmArgs& g, hipStream_t s) {
+  if (g.m <= 64) run_c3m_t<64>(g, s);
+  else run_c3m_t<128>(g, s);
+}
+
 // The MFMA kernels zero-pad partial tiles; with 32-row tiles they pay off from
 // m = 16 (k of 16 real units) or m = 32 (k of 8), measured against the
 // streaming kernel on gfx950.
@@ -1707,7 +1877,11 @@ static int launch_gemm_kernels(int dt, int prec, int kernel, const GemmArgs& g0,
     run_simple(dt, g, s);
     return GEMM_SIMPLE;
   }
-  if (prec == C3M) throw Error("precision c3m is not available in this build");
+  if (prec == C3M) {  // complex64 MFMA-shaped steps as three real products
+    if (dt != C64) throw Error("precision c3m applies to complex64 only");
+    run_c3m(g, s);
+    return GEMM_MFMA;
+  }
   const bool split_prec = (prec == BF16X3 || prec == FP16X2) && (dt == F32 || dt == C64);
   if ((kernel == GEMM_PRESPLIT || kernel == GEMM_PRESPLIT_A) && !split_prec) kernel = GEMM_MFMA;
   // split precision runs on GEMM_MFMA (split while staging) and GEMM_PRESPLIT;
@@ -1845,7 +2019,8 @@ size_t gemm_scratch_bytes(int dt, int prec, int kernel, const GemmArgs& g) {
   KSplit ks;
   if (kernel == GEMM_SPLITK) ks = plan_splitk(dt, g, MAX_PARTIAL_BYTES);
   else if (kernel == GEMM_MFMA || kernel >= GEMM_MFMA_R32)
-    ks = plan_mfma_split(dt, prec, g, MAX_PARTIAL_BYTES, mfma_rows_for(dt, kernel, g.m));
+    ks = plan_mfma_split(dt, prec == C3M ? NATIVE : prec, g, MAX_PARTIAL_BYTES,
+                         prec == C3M ? (g.m <= 64 ? 64 : 128) : mfma_rows_for(dt, kernel, g.m));
   size_t part = ks.n_split > 1 ? (size_t)ks.n_split * dtype_size(dt) * (size_t)(g.nbo * g.nbi * g.m * g.n) : 0;
   if ((prec == BF16X3 || prec == FP16X2) && (dt == F32 || dt == C64)) {
     if (kernel == GEMM_PRESPLIT) part = presplit_sizes(dt, g).total;

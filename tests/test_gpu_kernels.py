@@ -85,38 +85,51 @@ GEMM_SHAPES = [(1, 1, 1, 1), (1, 3, 4, 5), (2, 17, 33, 9), (1, 128, 64, 128), (3
                (1, 3, 5, 100), (1, 200, 7, 3), (1, 32, 128, 2000),
                (1, 1, 100000, 1), (3, 2, 5000, 3), (1, 5, 70000, 7), (2, 4, 300, 16), (1, 3, 40000, 64),
                (2, 100, 30000, 40)]
-# (dtype, precision code, per-product relative error bound)
-PREC_CASES = [(np.float32, 0, 1e-6), (np.float64, 0, 1e-14), (np.complex64, 0, 1e-6), (np.complex128, 0, 1e-14),
-              (np.float32, 2, 1e-5), (np.complex64, 2, 1e-5), (np.float32, 3, 1e-6), (np.complex64, 3, 1e-6)]
+# Unit roundoff of each arithmetic: FP32 / FP64 products; split products keep about
+# 16 (BF16 hi + lo) or 22 (FP16 hi + lo) significant bits of every operand.
+U32, U64, U_BF16X3, U_FP16X2 = 2.0 ** -24, 2.0 ** -53, 2.0 ** -16, 2.0 ** -21
+NATIVE_KERNELS = (0, 1, 2, 3, 5, 6, 7)
+# (dtype, precision code, kernel, unit roundoff): only the pairs that apply - split
+# precision runs on the MFMA kernels (1, 4 pre-split, 8 A pre-split), 3M on the MFMA
+# kernel for complex64, FP64 MFMA tiles have at most 64 rows (kernel 7 is 6 there)
+GEMM_CASES = ([(dt, 0, k, U32) for dt in (np.float32, np.complex64) for k in NATIVE_KERNELS]
+              + [(dt, 0, k, U64) for dt in (np.float64, np.complex128) for k in NATIVE_KERNELS if k != 7]
+              + [(dt, 2, k, U_BF16X3) for dt in (np.float32, np.complex64) for k in (1, 4, 8)]
+              + [(dt, 3, k, U_FP16X2) for dt in (np.float32, np.complex64) for k in (1, 4, 8)]
+              + [(np.complex64, 1, 1, U32)])
 
 
-@pytest.mark.parametrize("case", range(len(PREC_CASES)))
-@pytest.mark.parametrize("kernel", [0, 1, 2, 3, 4, 5, 6, 7, 8])
-def test_gemm_against_numpy(cuda, case, kernel):
-    """Simple (0), MFMA (1, native or split precision), skinny (2), split-K (3)
-    pre-split MFMA (4) and A-pre-split MFMA (8) GEMMs vs complex128 NumPy."""
+def _magnitude_bound(a, b):
+    """sum_p |a_ip| |b_pj| with |z| taken as |re| + |im|: bounds every real product
+    that any of the 4M, 3M or split forms accumulates (|ar + ai| <= |ar| + |ai|)."""
+    mag = lambda x: (np.abs(x.real) + np.abs(x.imag)).astype(np.float64)
+    return np.matmul(mag(a), mag(b))
+
+
+@pytest.mark.parametrize("case", range(len(GEMM_CASES)), ids=lambda c: "%s-p%d-k%d" % (
+    np.dtype(GEMM_CASES[c][0]).name, GEMM_CASES[c][1], GEMM_CASES[c][2]))
+def test_gemm_against_numpy(cuda, case):
+    """Every GEMM kernel and arithmetic against complex128 NumPy products, each output
+    element within 16 * u * sum_p |a_ip||b_pj| for the arithmetic's unit roundoff u
+    (a bound independent of k: dropping a k tile, a term or a sign fails it)."""
     import torch
     from arctn_amd import _lib
 
-    dtype, prec, eps = PREC_CASES[case]
-    if prec and kernel not in (1, 4, 8):
-        pytest.skip("split precision applies to the MFMA kernels")
-    if kernel == 7 and dtype in (np.float64, np.complex128):
-        pytest.skip("FP64 MFMA tiles have at most 64 rows (7 runs as 6)")
+    dtype, prec, kernel, u = GEMM_CASES[case]
     code = {np.float32: 0, np.float64: 1, np.complex64: 2, np.complex128: 3}[dtype]
     rng = np.random.default_rng(5)
     for bt, m, k, n in GEMM_SHAPES:
         a = _rand((bt, m, k), dtype, rng)
         b = _rand((bt, k, n), dtype, rng)
         want = np.matmul(a.astype(np.complex128), b.astype(np.complex128))
+        bound = 16 * u * _magnitude_bound(a, b)
         ta, tb = torch.from_numpy(a).cuda(), torch.from_numpy(b).cuda()
         tc = torch.empty((bt, m, n), dtype=ta.dtype, device="cuda")
         _lib.check(_lib.lib().atn_gemm(code, prec, kernel, bt, m, k, n, ta.data_ptr(), m * k, tb.data_ptr(), k * n,
                                        tc.data_ptr(), m * n, None))
         got = tc.cpu().numpy()
-        scale = np.sqrt(k) * max(1.0, np.abs(want).max())
-        err = np.abs(got - want).max() / scale
-        assert err < 20 * eps, (bt, m, k, n, err)
+        err = np.abs(got - want)
+        assert np.all(err <= bound), ((bt, m, k, n), float((err / bound).max()))
 
 
 @pytest.mark.parametrize("dtype", [np.float32, np.complex64])
