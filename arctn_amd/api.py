@@ -222,8 +222,13 @@ def _finish(out, numpy_in):
 class CompiledContraction:
     """A reusable exact contraction compiled from an SSA path (pybind/src/lib.rs:355-454).
 
-    ``execute`` runs the plain per-step path on the GPU; keyword options select
-    the precision form (``precision=``) and the batched/graph forms.
+    ``execute`` runs the unsliced plan with its tiny steps grouped: the steps of
+    one level of the contraction tree that run on the one-thread-per-element
+    kernel share one launch (the run of small absorptions that opens a circuit
+    plan, simplify.rs:241-310, becomes a few launches), every other step runs as
+    in the per-step path, with the same kernels and summation order.
+    ``batched=False`` selects the per-step path itself; keyword options select
+    the precision form (``precision=``) and the graph form.
     """
 
     def __init__(self, plan):
@@ -245,6 +250,7 @@ class CompiledContraction:
 
     def execute(self, arrays, **opts):
         dev, code, numpy_in = self._prepare(arrays)
+        opts.setdefault("batched", True)  # unsliced: one "batch", tiny steps grouped per tree level
         try:
             out, _ = self._plan.run(dev, code, **opts)
         except ValueError as e:

@@ -5,10 +5,15 @@ the EXEC line :729-775): the network comes from a `.net.json` file, the path and
 slice set from a saved version 2 execution plan (``--load-path``) or from the
 restated greedy planner (``--method greedy`` with ``--target-size``), the inputs
 from a little-endian binary file (``--data``) or seeded random normals. The
-contraction runs on the HIP executor; ``--check`` compares it with the same
-network contracted unsliced in double precision on the GPU (max |diff| ≤ 1e-6 ×
-max(1, max |reference|), src/bin/tnexec.rs:23, 505-545). One ``EXEC,...`` line
-on stdout; exit status 1 when the check fails, 2 on bad input.
+contraction runs on the HIP executor. ``--check`` compares it on the CPU, in
+double precision (f64 / complex128), with the reference's choice of check
+(src/bin/tnexec.rs:23, 505-545, 646-683): NumPy's naive einsum for networks of
+at most 8 tensors and 2^20 output elements ("naive"), else for sliced plans of
+at most 2^12 slices and 2^22 output elements the unsliced contraction along the
+plan's SSA path by NumPy pairwise einsums ("slice_oracle"), each only when its
+live set fits 512 MiB; the bar is max |diff| <= 1e-6 x max(1, max |reference|).
+One ``EXEC,...`` line on stdout; exit status 1 when the check fails, 2 on bad
+input. ``--save-output`` writes the result tensor (little-endian, row-major).
 """
 
 from __future__ import annotations
@@ -22,9 +27,71 @@ import numpy as np
 
 from . import planner as _planner
 from .execution_plan import ExecutionPlan, PlanValidationError
-from .netio import load_network_json, load_tensors_bin
+from .netio import load_network_json, load_tensors_bin, save_tensors_bin
 
 CHECK_RELATIVE_TOLERANCE = 1e-6
+FULL_CHECK_BYTE_BUDGET = 512 << 20
+
+
+def _einsum_pair(a, la, b, lb, keep):
+    """np.einsum of two tensors by leg lists (legs renumbered into einsum's 52 labels)."""
+    ids = {l: i for i, l in enumerate(dict.fromkeys(list(la) + list(lb) + list(keep)))}
+    if len(ids) > 52:
+        raise ValueError("step has more than 52 distinct legs")
+    return np.einsum(a, [ids[l] for l in la], b, [ids[l] for l in lb], [ids[l] for l in keep], optimize=False)
+
+
+def numpy_path_contract(net, arrays, ssa_path):
+    """The unsliced contraction along an SSA path on the CPU with NumPy: each step
+    keeps the legs still held by a live tensor or the output (hyperedges stay batch
+    legs), then the result is summed / permuted to the output order."""
+    slots = [(np.asarray(a), list(legs)) for a, legs in zip(arrays, net["inputs"])]
+    alive = [True] * len(slots)
+    out = list(net["output"])
+    for i, j in ssa_path:
+        (a, la), (b, lb) = slots[i], slots[j]
+        alive[i] = alive[j] = False
+        held = set(out)
+        for k, (_, lk) in enumerate(slots):
+            if alive[k]:
+                held.update(lk)
+        keep = [l for l in dict.fromkeys(la + lb) if l in held]
+        slots.append((_einsum_pair(a, la, b, lb, keep), keep))
+        alive.append(True)
+    (x, lx), = [s for s, live in zip(slots, alive) if live]
+    ids = {l: i for i, l in enumerate(dict.fromkeys(lx + out))}
+    return np.einsum(x, [ids[l] for l in lx], [ids[l] for l in out], optimize=False)
+
+
+def cpu_check(net, arrays, plan, result, is_complex):
+    """(check string, failed) following src/bin/tnexec.rs:505-545, 646-683."""
+    wide = np.complex128 if is_complex else np.float64
+    n_slices = math.prod(net["size_dict"][l] for l in plan.sliced_legs)
+    stats, _ = _planner.simulate_path_full(net, list(plan.ssa_path))
+    live = 2.0 ** stats.log2_peak_size + result.size + 2 * sum(a.size for a in arrays)
+    fits = live * np.dtype(wide).itemsize <= FULL_CHECK_BYTE_BUDGET
+    ref, label = None, None
+    if len(net["inputs"]) <= 8 and result.size <= (1 << 20) and fits:
+        label = "naive"
+        space = sum(math.log2(d) for d in net["size_dict"].values())
+        if space <= 28:
+            ids = {l: i for i, l in enumerate(sorted(net["size_dict"]))}
+            ops = []
+            for a, legs in zip(arrays, net["inputs"]):
+                ops += [a.astype(wide), [ids[l] for l in legs]]
+            ref = np.einsum(*ops, [ids[l] for l in net["output"]], optimize=False)
+    elif plan.sliced_legs and result.size <= (1 << 22) and n_slices <= (1 << 12) and fits:
+        label = "slice_oracle"
+        ref = numpy_path_contract(net, [a.astype(wide) for a in arrays], plan.ssa_path)
+    if ref is None:
+        return "too_large_for_oracle", False
+    ref = np.asarray(ref, dtype=wide)
+    diff = float(np.max(np.abs(result.astype(wide) - ref))) if ref.size else 0.0
+    scale = max(1.0, float(np.max(np.abs(ref))) if ref.size else 1.0)
+    tol = CHECK_RELATIVE_TOLERANCE * scale
+    failed = not (math.isfinite(diff) and math.isfinite(scale) and diff <= tol)
+    return (f"{'FAIL' if failed else 'PASS'}({label}_max_abs_diff={diff:.2e};scale={scale:.2e};"
+            f"tolerance={tol:.2e};oracle_dtype={'c64' if is_complex else 'f64'})"), failed
 
 
 def _args(argv):
@@ -43,7 +110,8 @@ def _args(argv):
     ap.add_argument("--plain", action="store_true", help="per-slice executor instead of the batched one")
     ap.add_argument("--graph", action="store_true", help="batched form captured in a hipGraph")
     ap.add_argument("--autotune", action="store_true")
-    ap.add_argument("--check", action="store_true", help="compare with a double-precision unsliced GPU run")
+    ap.add_argument("--check", action="store_true", help="compare with a double-precision CPU contraction (NumPy)")
+    ap.add_argument("--save-output", help="write the result tensor (little-endian, row-major)")
     return ap.parse_args(argv)
 
 
@@ -125,19 +193,9 @@ def main(argv=None):
 
     check, failed = "skipped", False
     if a.check:
-        if result.size <= (1 << 22):
-            ref_dt = np.complex128 if a.complex else np.float64
-            ref_plan = ExecutionPlan.from_network(net["inputs"], net["output"], net["size_dict"], plan.ssa_path, [])
-            ref = ref_plan.execute([t.to(torch.complex128 if a.complex else torch.float64) for t in dev])
-            ref = ref.cpu().numpy().astype(ref_dt)
-            diff = float(np.max(np.abs(result.astype(ref_dt) - ref))) if ref.size else 0.0
-            scale = max(1.0, float(np.max(np.abs(ref))) if ref.size else 1.0)
-            tol = CHECK_RELATIVE_TOLERANCE * scale
-            failed = not (math.isfinite(diff) and math.isfinite(scale) and diff <= tol)
-            check = (f"{'FAIL' if failed else 'PASS'}(gpu_unsliced_max_abs_diff={diff:.2e};scale={scale:.2e};"
-                     f"tolerance={tol:.2e};oracle_dtype={'c64' if a.complex else 'f64'})")
-        else:
-            check = "too_large_for_oracle"
+        check, failed = cpu_check(net, arrays, plan, result, a.complex)
+    if a.save_output:
+        save_tensors_bin(a.save_output, [result])
 
     target = plan.target_size
     print(f"EXEC,net={a.net},dtype={dname},method={method},backend=hip,precision={a.precision},"

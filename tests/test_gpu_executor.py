@@ -93,6 +93,54 @@ def test_output_slicing_cases_vs_einsum(cuda, case, dtype):
         np.testing.assert_allclose(got, ref, rtol=tol, atol=tol, err_msg=f"{name} {mode}")
 
 
+@pytest.mark.parametrize("case", range(len(FX["output_slicing_cases"]["cases"])))
+@pytest.mark.parametrize("dtype", DTYPES)
+def test_arctn_contract_string_labels_vs_einsum(cuda, case, dtype):
+    """arctn_contract (pybind/python/arctn/__init__.py:903-1072) with backend="hip" on the
+    einsum cases of test_output_slicing.py, labels as strings: planned on the CPU, sliced
+    to a small target (output slicing allowed) and executed on the GPU, against np.einsum."""
+    from arctn_amd import arctn_contract
+
+    name, eq, _, _, _ = F.einsum_case(FX["output_slicing_cases"]["cases"][case])
+    dims = FX["output_slicing_cases"]["cases"][case][2]
+    terms, out = eq.split("->")
+    inputs = [list(t) for t in terms.split(",")]
+    net = {"inputs": inputs, "output": list(out), "size_dict": dict(dims)}
+    rng = np.random.default_rng(case)
+    arrays = []
+    for t in inputs:
+        x = rng.standard_normal(tuple(dims[c] for c in t))
+        if np.issubdtype(np.dtype(dtype), np.complexfloating):
+            x = x + 1j * rng.standard_normal(x.shape)
+        arrays.append(x.astype(dtype))
+    ref = np.einsum(eq, *arrays, optimize=False)
+    single = np.dtype(dtype) in (np.dtype(np.float32), np.dtype(np.complex64))
+    tol = 2e-4 if single else 1e-11
+    for target, out_slicing, preset in ((None, False, "greedy"), (2, True, "greedy"), (2, True, "rgreedy")):
+        got, info = arctn_contract(net["inputs"], net["output"], net["size_dict"], arrays, target_size=target,
+                                   allow_output_slicing=out_slicing, preset=preset, ntrials=3, return_info=True)
+        assert got.shape == ref.shape and got.dtype == np.dtype(dtype)
+        np.testing.assert_allclose(got, ref, rtol=tol, atol=tol, err_msg=f"{name} target={target}")
+        assert info["execution_backend"] == "hip"
+
+
+def test_arctn_contract_circuit_vs_oracle(cuda):
+    """arctn_contract on the 4x4 random circuit amplitude (string labels), unsliced and
+    sliced, against the complex128 oracle."""
+    from arctn_amd import arctn_contract
+
+    net, arrays, _ = W.random_circuit_amplitude(4, 4, 8, 0)
+    want = complex(O.contract(net, [a.astype(np.complex128) for a in arrays], P.greedy(net), "compiled"))
+    name = lambda l: f"q{l}"
+    inputs = [[name(l) for l in t] for t in net["inputs"]]
+    sd = {name(l): d for l, d in net["size_dict"].items()}
+    for target, preset in ((None, "greedy"), (2 ** 8, "greedy"), (2 ** 8, "rgreedy")):
+        got, info = arctn_contract(inputs, [], sd, arrays, target_size=target, preset=preset, ntrials=4,
+                                   return_info=True)
+        assert abs(complex(got) - want) / abs(want) < 1e-5, (target, preset)
+        assert bool(info["sliced"]) == (target is not None)
+
+
 def _random_case(seed, dims=(2, 3, 4)):
     rng = np.random.default_rng(seed)
     n = int(rng.integers(3, 8))
@@ -153,6 +201,24 @@ def test_circuit_4x4_amplitude_vs_complex128_oracle(cuda):
     for mode in MODES[:2]:
         got = cc.execute(arrays, **mode)
         assert abs(complex(got) - complex(want)) / abs(complex(want)) < 1e-5
+
+
+@pytest.mark.parametrize("dtype", DTYPES)
+def test_grouped_small_steps_equal_per_step_path(cuda, dtype):
+    """CompiledContraction.execute groups the tiny steps of each tree level into one
+    launch; the result is bit-identical to the per-step path (same kernels, same
+    summation order): the 4x4 circuit and random networks."""
+    cases = []
+    for seed in range(8):
+        net, path = _random_case(seed)
+        cases.append((net, W.random_arrays(net, dtype, seed), path))
+    if np.issubdtype(np.dtype(dtype), np.complexfloating):
+        net, arrays, _ = W.random_circuit_amplitude(4, 4, 8, 0, dtype=dtype)
+        cases.append((net, arrays, P.greedy(net)))
+    for net, arrays, path in cases:
+        cc = CompiledContraction.compile(net["inputs"], net["output"], net["size_dict"], ssa_path=path)
+        grouped, per_step = cc.execute(arrays), cc.execute(arrays, batched=False)
+        assert grouped.tobytes() == per_step.tobytes(), len(net["inputs"])
 
 
 def test_stripped_reconstructs_plain(cuda):
