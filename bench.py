@@ -29,9 +29,54 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = "contraction TFLOP/s + slices/s vs plain FP32 GPU path at rel err ≤1e-5"
-PEAKS = {"gemm_mfma_f32": 157.3e12, "gemm_mfma_f64": 78.6e12, "gemm_mfma_c3m": 157.3e12,
-         "gemm_mfma_bf16x3": 2.5e15, "gemm_mfma_fp16x2": 2.5e15}
 HBM_PEAK = 8.0e12
+# MI355X dense peak (FLOP/s) of the arithmetic unit each GEMM kind runs on, and the
+# arithmetic-unit FLOPs per algorithmic FLOP (algorithmic = 8 per complex / 2 per
+# real multiply-add): 4M complex is one real GEMM over the 2x2 expansion of B (1),
+# 3M does 3 real products per complex one (6/8), split precision three 16-bit
+# products per FP32 product (3).
+ROOFS = {"gemm_mfma_f32": (157.3e12, 1.0), "gemm_mfma_f64": (78.6e12, 1.0), "gemm_mfma_c3m": (157.3e12, 0.75),
+         "gemm_mfma_bf16x3": (2.5e15, 3.0), "gemm_mfma_fp16x2": (2.5e15, 3.0)}
+VECTOR_FP32 = (157.3e12, 1.0)  # every other kernel kind (vector ALU)
+FIXTURES = os.path.join(ROOT, "tests", "golden", "circuit_fixtures.json")
+
+
+def roofline_of(name, k):
+    """One kernel kind on the roof that binds it: t_min = max(F * r / P, B / BW) over its
+    summed launches; bound = the larger term; frac = t_min / measured time."""
+    peak, ratio = ROOFS.get(name, VECTOR_FP32)
+    t = k["ms"] * 1e-3
+    t_flop, t_hbm = k["flops"] * ratio / peak, k["bytes"] / HBM_PEAK
+    if t_flop >= t_hbm:
+        r = {"bound": "mfma" if name in ROOFS else "valu", "achieved": k["flops"] * ratio / t / 1e12,
+             "peak": peak / 1e12, "unit": "TFLOP/s", "algorithmic_tflops": k["flops"] / t / 1e12}
+    else:
+        r = {"bound": "hbm", "achieved": k["bytes"] / t / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s"}
+    r.update({"frac": max(t_flop, t_hbm) / t, "t_mfma_ms": t_flop * 1e3, "t_hbm_ms": t_hbm * 1e3,
+              "t_measured_ms": k["ms"], "kernel": name, "launches": k["calls"], "kernel_launches": k["launches"],
+              "avg_launch_ms": k["ms"] / k["calls"], "algorithmic_bytes_per_launch": k["bytes"] / k["calls"],
+              "algorithmic_flops_per_launch": k["flops"] / k["calls"]})
+    return r
+
+
+def host_cpu():
+    """CPU threads this process may use (its affinity set, at most 16: the host share
+    of one GPU on the benchmark machines), the host's logical CPU count and model."""
+    avail = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            model = next((l.split(":", 1)[1].strip() for l in f if l.startswith("model name")), "")
+    except OSError:
+        pass
+    return min(16, avail), {"nproc": os.cpu_count(), "affinity_cpus": avail, "cpu_model": model}
+
+
+def fixture(name):
+    if not os.path.exists(FIXTURES):
+        return None
+    with open(FIXTURES) as f:
+        return json.load(f).get(name)
 
 
 def log(*a):
@@ -47,12 +92,14 @@ def main():
     ap.add_argument("--precision", default="auto",
                     help="native | bf16x3 | fp16x2 | auto (fastest variant with rel. error <= 1e-5)")
     ap.add_argument("--no-graph", action="store_true")
-    ap.add_argument("--no-autotune", action="store_true", help="skip the per-step GEMM kernel and tile search")
+    ap.add_argument("--autotune", action="store_true",
+                    help="also time autotuned variants (per-step GEMM kernel and tile search) as candidate forms")
     ap.add_argument("--skip-plain", action="store_true")
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-profile", action="store_true")
     ap.add_argument("--skip-ref", action="store_true",
-                    help="skip the complex128 reference contraction (rel_err is then null; for profiler runs)")
+                    help="without the committed CPU-oracle fixture, skip the complex128 GPU reference too "
+                         "(rel_err is then null; for profiler runs)")
     ap.add_argument("--cpu-slices", type=int, default=0, help="CPU sample size (0 = three per core)")
     ap.add_argument("--library-gemm", action="store_true",
                     help="only time the ROCm BLAS libraries' 16-bit batched GEMM (torch.bmm) on the real product "
@@ -106,39 +153,41 @@ def main():
     info = plan.info()
     dev = torch.device("cuda", 0)
     x64 = [torch.from_numpy(a).to(dev) for a in arrays]
-    x128 = [t.to(torch.complex128) for t in x64]
     stream = torch.cuda.current_stream(dev)
-    form = dict(batched=True, use_graph=not args.no_graph, precision=args.precision, autotune=not args.no_autotune)
+    form = dict(batched=True, use_graph=not args.no_graph, precision=args.precision, autotune=False)
     flops = 8.0 * info["macs_per_slice"] * info["n_slices"]
-
     log(f"[bench] plan: {info['n_steps']} steps, {info['n_slices']} slices, "
         f"{info['n_varying_steps']} slice-dependent steps, {flops:.3e} FLOP per contraction")
-    ref = None
-    if not args.skip_ref:
-        ref_out, _ = plan.run(x128, 3, batched=True)
-        ref = complex(ref_out.cpu())
+    # reference amplitude: the complex128 CPU oracle over all slices (committed fixture)
+    fx = fixture(args.workload)
+    ref, ref_kind = None, None
+    if fx is not None:
+        ref, ref_kind = complex(*fx["amplitude"]), "complex128 CPU oracle, all slices (tests/golden)"
+    elif not args.skip_ref:
+        out128, _ = plan.run([t.to(torch.complex128) for t in x64], 3, batched=True)
+        ref, ref_kind = complex(out128.cpu()), "complex128 GPU path (no CPU fixture present)"
         plan.release()
-        log(f"[bench] complex128 reference amplitude {ref:.9e}")
+    if ref is not None:
+        log(f"[bench] reference amplitude {ref:.9e} ({ref_kind})")
 
-    # Form: the fastest (precision, autotune) variant whose error meets the 1e-5 bar.
-    precs = ["native", "bf16x3", "fp16x2"] if args.precision == "auto" else [args.precision]
-    tunes = [False] if args.no_autotune else [False, True]
+    # Form: the fastest precision variant (optionally with autotune) whose error meets 1e-5.
+    precs = ["native", "c3m", "bf16x3", "fp16x2"] if args.precision == "auto" else [args.precision]
+    tunes = [False, True] if args.autotune else [False]
     cands = [(p, t) for p in precs for t in tunes]
     trial = {}
+    key = lambda c: c[0] + ("+autotune" if c[1] else "")
     for prec, tune in cands:
         f = dict(form, precision=prec, autotune=tune)
         for _ in range(2):
             o, rep = plan.run(x64, 2, report=True, **f)
         err = abs(complex(o.cpu()) - ref) / abs(ref) if ref is not None else None
-        name = prec + ("+autotune" if tune else "")
-        trial[name] = {"ms": rep["elapsed_ms"], "rel_err": err, "launches": rep["n_launches"]}
-        log(f"[bench] candidate {name}: {rep['elapsed_ms']:.2f} ms, {rep['n_launches']} launches, rel err "
-            + (f"{err:.2e}" if err is not None else "n/a"))
+        trial[key((prec, tune))] = {"ms": rep["elapsed_ms"], "rel_err": err, "launches": rep["n_launches"]}
+        log(f"[bench] candidate {key((prec, tune))}: {rep['elapsed_ms']:.2f} ms, {rep['n_launches']} launches, "
+            f"rel err " + (f"{err:.2e}" if err is not None else "n/a"))
         plan.release()
-    key = lambda c: c[0] + ("+autotune" if c[1] else "")
-    # half the 1e-5 bar as the selection margin: a form's error varies a little with
-    # the autotuner's per-run kernel choices, and the timed run must stay within 1e-5
     err = lambda c: trial[key(c)]["rel_err"]
+    # half the bar as the selection margin (an autotuned form's error varies a little
+    # with its per-run kernel choices; the timed run must stay within 1e-5)
     ok = ([c for c in cands if err(c) is not None and err(c) <= 5e-6]
           or [c for c in cands if err(c) is not None and err(c) <= 1e-5])
     if ok:
@@ -146,7 +195,8 @@ def main():
     else:  # nothing within the bar (or no reference): the most accurate variant
         prec, tune = min(cands, key=lambda c: (err(c) if err(c) is not None else 0.0, trial[key(c)]["ms"]))
     form["precision"], form["autotune"] = prec, tune
-    log(f"[bench] measuring form {key((prec, tune))} ({trial[key((prec, tune))]['launches']} launches per contraction)")
+    form_name = key((prec, tune))
+    log(f"[bench] measuring form {form_name} ({trial[form_name]['launches']} launches per contraction)")
 
     out = torch.empty([], dtype=torch.complex64, device=dev)
     for w in range(args.warmup):
@@ -168,46 +218,30 @@ def main():
     got = complex(out.cpu())
     rel_err = abs(got - ref) / abs(ref) if ref is not None else None
     if ref is not None:
-        log(f"[bench] amplitude {got:.6e} vs complex128 {ref:.6e}: rel err {rel_err:.2e}")
+        log(f"[bench] amplitude {got:.9e} vs reference {ref:.9e}: rel err {rel_err:.2e}")
 
-    roofline = None
-    kernels = {}
+    roofline, kernels = None, {}
     if not args.skip_profile:
+        # one instrumented eager pass of the same form (HIP events around every op on its stream)
         kernels = profile(plan, x64, 2, **form)
+        for name, k in kernels.items():
+            k["roofline"] = roofline_of(name, k)
         dom = max(kernels, key=lambda k: kernels[k]["ms"])
-        k = kernels[dom]
-        if dom in PEAKS:
-            ach = k["flops"] / (k["ms"] * 1e-3)
-            roofline = {"bound": "mfma", "achieved": ach / 1e12, "peak": PEAKS[dom] / 1e12, "unit": "TFLOP/s",
-                        "frac": ach / PEAKS[dom], "traffic": None, "kernel": dom,
-                        "avg_launch_ms": k["ms"] / k["calls"], "launches": k["calls"],
-                        "kernel_launches": k["launches"], "algorithmic_bytes_per_launch": k["bytes"] / k["calls"]}
-            tf = os.path.join(ROOT, "profiles", f"traffic_{dom}_{args.workload}.json")
-            if os.path.exists(tf):
-                with open(tf) as f:
-                    t = json.load(f)
-                roofline["traffic"] = t["hbm_bytes_per_launch"]
-                roofline["traffic_source"] = os.path.relpath(tf, ROOT)
-        else:
-            ach = k["bytes"] / (k["ms"] * 1e-3)
-            roofline = {"bound": "hbm", "achieved": ach / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                        "frac": ach / HBM_PEAK, "traffic": None, "kernel": dom,
-                        "avg_launch_ms": k["ms"] / k["calls"], "launches": k["calls"],
-                        "kernel_launches": k["launches"]}
-        log(f"[bench] dominant kernel {dom}: {roofline['achieved']:.1f} {roofline['unit']} "
-            f"({100 * roofline['frac']:.1f}% of peak)")
-        # whole-plan bound: each kernel kind at max(FLOPs / its MFMA peak, bytes / HBM peak)
-        t_min = 0.0
-        for name, kk in kernels.items():
-            peak = PEAKS.get(name, 157.3e12)
-            if name in ("gemm_mfma_bf16x3", "gemm_mfma_fp16x2"):
-                peak = PEAKS[name] / 3  # three 16-bit products per algorithmic product
-            t_min += max(kk["flops"] / peak, kk["bytes"] / HBM_PEAK)
-        prof_ms = sum(kk["ms"] for kk in kernels.values())
-        roofline["plan"] = {"t_min_ms": t_min * 1e3, "t_kernels_ms": prof_ms, "frac": t_min * 1e3 / prof_ms,
-                            "model": "sum over kernel kinds of max(FLOPs / MFMA peak, bytes / 8 TB/s)"}
-        log(f"[bench] plan roofline: {t_min * 1e3:.1f} ms bound vs {prof_ms:.1f} ms measured "
-            f"({100 * t_min * 1e3 / prof_ms:.1f}%)")
+        roofline = dict(kernels[dom]["roofline"])
+        roofline["traffic"] = None
+        tf = os.path.join(ROOT, "profiles", f"traffic_{dom}_{args.workload}_{form_name}.json")
+        if os.path.exists(tf):
+            with open(tf) as f:
+                t = json.load(f)
+            roofline["traffic"] = t["hbm_bytes_per_launch"]
+            roofline["traffic_source"] = os.path.relpath(tf, ROOT)
+        log(f"[bench] dominant kernel {dom}: {roofline['bound']}-bound, {roofline['achieved']:.1f} "
+            f"{roofline['unit']} ({100 * roofline['frac']:.1f}% of its roof)")
+        t_min = sum(max(k["roofline"]["t_mfma_ms"], k["roofline"]["t_hbm_ms"]) for k in kernels.values())
+        prof_ms = sum(k["ms"] for k in kernels.values())
+        roofline["plan"] = {"t_min_ms": t_min, "t_kernels_ms": prof_ms, "frac": t_min / prof_ms,
+                            "model": "sum over kernel kinds of max(FLOPs x r / peak, bytes / 8 TB/s)"}
+        log(f"[bench] plan roofline: {t_min:.1f} ms bound vs {prof_ms:.1f} ms measured ({100 * t_min / prof_ms:.1f}%)")
     plan.release()
 
     plain = {}
@@ -228,22 +262,30 @@ def main():
         plain["plain_fp32_permuted_ms"] = rep_p["elapsed_ms"]
         plain["speedup_vs_plain_permuted"] = rep_p["elapsed_ms"] / ms
         log(f"[bench] plain per-slice FP32 path with permuted B operands: {rep_p['elapsed_ms']:.1f} ms")
+        plan.release()
 
     cpu = None
     if not args.skip_cpu:
         from oracle import pyoracle as O
 
-        cores = min(16, os.cpu_count() or 1)
-        n = args.cpu_slices or 3 * cores
-        log(f"[bench] CPU oracle sample: {n} slices on {cores} threads")
+        threads, host = host_cpu()
+        n = args.cpu_slices or 3 * threads
+        log(f"[bench] CPU oracle sample: slices [0, {n}) on {threads} threads ({host})")
         t0 = time.perf_counter()
-        O.contract_slice_range(net, arrays, [tuple(p) for p in doc["ssa_path"]], doc["sliced_legs"], 0, n,
-                               threads=cores)
+        cpu_part = O.contract_slice_range(net, arrays, [tuple(p) for p in doc["ssa_path"]], doc["sliced_legs"], 0, n,
+                                          threads=threads)
         dt = time.perf_counter() - t0
-        cpu = {"value": n / dt, "unit": "slices/s", "cores": cores, "kind": "port",
-               "sample": f"slices 0..{n - 1} of {info['n_slices']} (complex64), oracle CPU restatement, "
-                         f"{dt:.1f} s", "tflops": 8.0 * info["macs_per_slice"] * n / dt / 1e12}
-        log(f"[bench] CPU: {n / dt:.3f} slices/s")
+        # the same sample on the GPU in the measured form: the CPU run is checked, not only timed
+        g_part, _ = plan.run(x64, 2, slice_range=(0, n), **form)
+        g_part = complex(g_part.cpu())
+        plan.release()
+        sample_err = abs(g_part - complex(cpu_part)) / abs(complex(cpu_part))
+        cpu = {"value": n / dt, "unit": "slices/s", "cores": threads, "kind": "port",
+               "sample": f"slices 0..{n - 1} of {info['n_slices']} (complex64, the executed arithmetic), oracle "
+                         f"CPU restatement with the reference's chunked slice sum, {dt:.1f} s",
+               "tflops": 8.0 * info["macs_per_slice"] * n / dt / 1e12, "host": host,
+               "gpu_vs_cpu_sample_rel_err": sample_err}
+        log(f"[bench] CPU: {n / dt:.3f} slices/s; GPU form vs CPU on the sample: rel err {sample_err:.2e}")
 
     line = {
         "metric": METRIC, "value": round(value, 3), "unit": "TFLOP/s", "n_gpus": 1, "steps": args.steps,
@@ -251,11 +293,11 @@ def main():
         "vs_baseline": None, "dtype": "complex64", "data": "synthetic (seeded random circuit, random-init gates)",
         "config": {"workload": args.workload, "qubits": 30, "grid": "5x6", "cycles": 14,
                    "n_slices": info["n_slices"], "target_size": 2 ** 24,
-                   "form": "batched" + ("" if args.no_graph else "+hipgraph") + ("+autotune" if form["autotune"] else ""),
-                   "precision": form["precision"], "flops_per_contraction": flops},
+                   "form": "batched" + ("" if args.no_graph else "+hipgraph") + ("+autotune" if tune else ""),
+                   "precision": prec, "flops_per_contraction": flops},
         "precision_trials": trial,
         "slices_per_s": round(info["n_slices"] / (ms * 1e-3), 2),
-        "rel_err": rel_err,
+        "rel_err": rel_err, "rel_err_reference": ref_kind,
         "roofline": roofline,
         "cpu_baseline": cpu,
         "kernels": kernels,
