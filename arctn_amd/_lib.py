@@ -77,6 +77,10 @@ _SIGS = {
     "atn_step_adjoint": (C.c_int, [_VP, C.c_size_t, C.c_int, _VP, _VP, _VP, _VP, _VP, _VP]),
     "atn_finalize_forward": (C.c_int, [_VP, C.c_int, _VP, _VP, _VP]),
     "atn_finalize_adjoint": (C.c_int, [_VP, C.c_int, _VP, _VP, _VP]),
+    "atn_adjoint_create": (C.c_int, [_VP, C.c_int, C.c_int, C.c_size_t, C.POINTER(_VP)]),
+    "atn_adjoint_destroy": (None, [_VP]),
+    "atn_adjoint_forward": (C.c_int, [_VP, C.POINTER(_VP), _VP, _VP]),
+    "atn_adjoint_backward": (C.c_int, [_VP, C.POINTER(_VP), _VP, C.POINTER(_VP), _VP]),
     "atn_tensor_op": (C.c_int, [C.c_int, C.c_int, C.c_size_t, C.POINTER(C.c_size_t), C.POINTER(C.c_size_t),
                                 C.c_size_t, _VP, _VP, _VP]),
     "atn_gemm": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_size_t, C.c_size_t, C.c_size_t, C.c_size_t, _VP,

@@ -182,9 +182,41 @@ class Plan:
             self._executors[key] = ex
         return ex
 
+    def check_device_args(self, dev_inputs, dtype_code, out=None):
+        """Count, dtype, shape, contiguity and device of the device tensors (and of a
+        caller-supplied output) before any pointer reaches the C ABI; the reference
+        checks every operand before its GEMMs (src/compiled.rs:386-398)."""
+        torch = _torch()
+        if dtype_code not in DTYPE_NAME:
+            raise ValueError(f"unsupported dtype code {dtype_code!r}")
+        tdt = _torch_dtype(dtype_code)
+        if len(dev_inputs) != len(self.inputs):
+            raise ValueError(f"张量个数 {len(dev_inputs)} 与计划输入数 {len(self.inputs)} 不符")
+        dev = None
+        for i, (t, legs) in enumerate(zip(dev_inputs, self.inputs)):
+            if not isinstance(t, torch.Tensor) or not t.is_cuda:
+                raise ValueError(f"张量 {i} is not a GPU tensor")
+            dev = dev or t.device
+            if t.device != dev:
+                raise ValueError(f"张量 {i} is on {t.device}, tensor 0 on {dev}")
+            if t.dtype != tdt:
+                raise ValueError(f"张量 {i} has dtype {t.dtype}, the run's dtype is {DTYPE_NAME[dtype_code]}")
+            want = tuple(self.size_dict[l] for l in legs)
+            if tuple(t.shape) != want:
+                raise ValueError(f"张量 {i} shape {list(t.shape)} 与网络预期 {list(want)} 不一致")
+            if not t.is_contiguous():
+                raise ValueError(f"张量 {i} is not contiguous")
+        if out is not None:
+            want = tuple(self.size_dict[l] for l in self.output)
+            if not isinstance(out, torch.Tensor) or not out.is_cuda or out.device != dev:
+                raise ValueError("out must be a GPU tensor on the inputs' device")
+            if out.dtype != tdt or tuple(out.shape) != want or not out.is_contiguous():
+                raise ValueError(f"out must be a contiguous {DTYPE_NAME[dtype_code]} tensor of shape {list(want)}")
+
     def run(self, dev_inputs, dtype_code, out=None, report=False, stream=None, **opts):
         """Execute on device tensors; returns (output tensor, report dict or None)."""
         torch = _torch()
+        self.check_device_args(dev_inputs, dtype_code, out)
         ex = self.executor(dtype_code, **opts)
         if out is None:
             shape = [self.size_dict[l] for l in self.output]
@@ -200,6 +232,7 @@ class Plan:
 def profile(plan, dev_inputs, dtype_code, out=None, stream=None, **opts):
     """Per-kernel-kind device time and algorithmic work of one eager execution."""
     torch = _torch()
+    plan.check_device_args(dev_inputs, dtype_code, out)
     ex = plan.executor(dtype_code, **opts)
     if out is None:
         shape = [plan.size_dict[l] for l in plan.output]

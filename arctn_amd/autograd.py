@@ -1,24 +1,29 @@
 """Differentiable contraction on the GPU: a torch.autograd.Function over the
-executor's stage entry points.
+library's reverse-mode runner (atn_adjoint_* in include/arctn_hip.h).
 
-The forward pass runs the compiled plan stage by stage (input preps, pair
-steps in SSA order, finalize), the same sequence as CompiledContraction's
-stage API (src/compiled.rs:7-10, 60-63, 143-153, 206-209). The backward pass
-runs the hand-written adjoint of every stage in reverse order:
+The forward pass is one library call: every pair step of the compiled plan
+runs level by level of the contraction tree (the small steps of a level in one
+launch), the output is written and the checkpoint slots stay on the device.
+The backward pass is one more call: the hand-written adjoint of every stage in
+reverse order,
 
 * pair step  C = perm_a(sum_a(A)) . perm_b(sum_b(B)) (batched GEMM):
   dA2 = dC . B2^H and dB2 = A2^H . dC (conjugate-transposed GEMMs), then the
   inverse permutations and a broadcast over every summed leg;
 * input prep (trace / diagonal): the gradient is placed on the diagonal;
-* finalize: inverse permutation and broadcast over residual summed legs.
+* finalize: inverse permutation and broadcast over residual summed legs,
+
+recomputing the intermediates of each checkpoint segment first. The stage hook
+of the reference (src/compiled.rs:7-10, 60-63, 143-153, 206-209) defines the
+per-stage semantics; the reference itself has no gradient code.
 
 Gradients follow PyTorch's complex convention (the incoming gradient is
 dL/d(conj out)), so real losses of complex contractions differentiate through
 torch's own autograd around this function.
 
-Checkpointing: with ``checkpoint_every=k`` only the live SSA slots at every
-k-th step boundary are kept; the backward pass recomputes each segment's
-intermediates from its boundary before running the segment's adjoints.
+Checkpointing: with ``checkpoint_every=k`` the steps, in tree-level order, form
+segments of k; slots produced and consumed inside a segment are recomputed in
+the backward pass, every other slot is kept from the forward pass.
 """
 
 from __future__ import annotations
@@ -31,127 +36,124 @@ from .api import Plan
 _CODE = {torch.float32: 0, torch.float64: 1, torch.complex64: 2, torch.complex128: 3}
 
 
-class _Stages:
-    """Shapes and slot bookkeeping of a plan's per-slice stages."""
-
-    def __init__(self, plan: Plan):
-        d = plan.describe()
-        self.n_inputs = d["n_inputs"]
-        self.preps = d["preps"]
-        self.steps = d["steps"]
-        self.fin = d["finalize"]
+def _shapes(plan: Plan):
+    """(input shapes, output shape) of an unsliced plan, cached on the plan."""
+    got = plan.__dict__.get("_grad_shapes")
+    if got is None:
         if plan.info()["n_slices"] != 1:
             raise ValueError("autograd contraction needs an unsliced plan")
+        d = plan.describe()
+        got = plan.__dict__["_grad_shapes"] = ([tuple(p["input_shape"]) for p in d["preps"]],
+                                               tuple(d["finalize"]["output_shape"]))
+    return got
 
 
-def _stream(t):
-    return torch.cuda.current_stream(t.device).cuda_stream
+def check_inputs(plan: Plan, inputs):
+    """Count, dtype, device and shape of every input, checked before any pointer
+    crosses the ABI (the reference checks every operand before its GEMMs,
+    src/compiled.rs:386-398). Returns the dtype code."""
+    shapes, _ = _shapes(plan)
+    if len(inputs) != len(shapes):
+        raise ValueError(f"张量个数 {len(inputs)} 与编译计划 {len(shapes)} 不符")
+    code = _CODE.get(inputs[0].dtype) if inputs else None
+    if code is None:
+        raise ValueError(f"unsupported dtype {inputs[0].dtype if inputs else None}")
+    dev = inputs[0].device
+    for i, (t, want) in enumerate(zip(inputs, shapes)):
+        if not isinstance(t, torch.Tensor):
+            raise ValueError(f"张量 {i} is not a torch tensor")
+        if _CODE.get(t.dtype) != code:
+            raise ValueError(f"张量 {i} has dtype {t.dtype}, tensor 0 has {inputs[0].dtype}")
+        if t.device != dev:
+            raise ValueError(f"张量 {i} is on {t.device}, tensor 0 on {dev}")
+        if tuple(t.shape) != want:
+            raise ValueError(f"张量 {i} shape {list(t.shape)} does not match the plan's {list(want)}")
+    if dev.type != "cuda":
+        raise ValueError("the inputs must be GPU tensors")
+    return code
 
 
-def _prep_fwd(plan, i, x):
-    st = plan._stages.preps[i]
-    if not st["traces"]:
-        return x
-    out = torch.empty(st["shape"], dtype=x.dtype, device=x.device)
-    _lib.check(_lib.lib().atn_prep_forward(plan.handle, i, _CODE[x.dtype], x.data_ptr(), out.data_ptr(), _stream(x)))
-    return out
+class _Runner:
+    """One atn_adjoint object: a plan's forward / backward for a dtype, precision
+    and checkpoint spacing, with its device workspace."""
+
+    def __init__(self, plan: Plan, code: int, precision: str, every: int):
+        self.input_shapes, self.output_shape = _shapes(plan)
+        self.code = code
+        self.h = _lib.C.c_void_p()
+        _lib.check(_lib.lib().atn_adjoint_create(plan.handle, code, _lib.PREC[precision], int(every),
+                                                 _lib.C.byref(self.h)))
+        self.generation = 0  # forward passes run so far
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib._lib is not None:
+            _lib.lib().atn_adjoint_destroy(self.h)
+            self.h = None
+
+    def forward(self, inputs, stream):
+        out = torch.empty(self.output_shape, dtype=inputs[0].dtype, device=inputs[0].device)
+        ptrs = _lib.pointer_array([t.data_ptr() for t in inputs])
+        _lib.check(_lib.lib().atn_adjoint_forward(self.h, ptrs, out.data_ptr(), stream))
+        self.generation += 1
+        return out
+
+    def backward(self, inputs, gout, stream):
+        # one buffer for all input gradients, handed out as views
+        sizes = [t.numel() for t in inputs]
+        flat = torch.empty(sum(sizes), dtype=gout.dtype, device=gout.device)
+        grads, ptrs, off = [], [], 0
+        es = flat.element_size()
+        for t, n in zip(inputs, sizes):
+            grads.append(flat[off:off + n].view(t.shape))
+            ptrs.append(flat.data_ptr() + off * es)
+            off += n
+        _lib.check(_lib.lib().atn_adjoint_backward(self.h, _lib.pointer_array([t.data_ptr() for t in inputs]),
+                                                   gout.data_ptr(), _lib.pointer_array(ptrs), stream))
+        return grads
 
 
-def _step_fwd(plan, s, a, b, prec):
-    st = plan._stages.steps[s]
-    out = torch.empty(st["result_shape"], dtype=a.dtype, device=a.device)
-    _lib.check(_lib.lib().atn_step_forward(plan.handle, s, _CODE[a.dtype], _lib.PREC[prec], a.data_ptr(),
-                                           b.data_ptr(), out.data_ptr(), _stream(a)))
-    return out
-
-
-def _fin_fwd(plan, x):
-    out = torch.empty(plan._stages.fin["output_shape"], dtype=x.dtype, device=x.device)
-    _lib.check(_lib.lib().atn_finalize_forward(plan.handle, _CODE[x.dtype], x.data_ptr(), out.data_ptr(), _stream(x)))
-    return out
+def _runner(plan, code, precision, every):
+    cache = plan.__dict__.setdefault("_grad_runners", {})
+    key = (code, precision, every)
+    r = cache.get(key)
+    if r is None:
+        r = cache[key] = _Runner(plan, code, precision, every)
+    return r
 
 
 class ContractFunction(torch.autograd.Function):
     @staticmethod
     def forward(ctx, plan, precision, checkpoint_every, *inputs):
-        if not hasattr(plan, "_stages"):
-            plan._stages = _Stages(plan)
-        S = plan._stages
+        code = check_inputs(plan, inputs)
         # materialise lazy conjugate / negative views before handing raw pointers over
         inputs = [t.resolve_conj().resolve_neg().contiguous() for t in inputs]
-        slots = {i: _prep_fwd(plan, i, t) for i, t in enumerate(inputs)}
-        keep = checkpoint_every and checkpoint_every > 0
-        boundaries = {0: dict(slots)} if keep else None
-        saved = dict(slots) if not keep else None
-        for s, st in enumerate(S.steps):
-            if keep and s > 0 and s % checkpoint_every == 0:
-                boundaries[s] = dict(slots)
-            c = _step_fwd(plan, s, slots.pop(st["li"]), slots.pop(st["ri"]), precision)
-            slots[st["out"]] = c
-            if saved is not None:
-                saved[st["out"]] = c
-        final = slots[S.fin["slot"]]
-        out = _fin_fwd(plan, final)
-        ctx.plan, ctx.precision, ctx.every = plan, precision, checkpoint_every
-        ctx.boundaries, ctx.saved = boundaries, saved
-        ctx.final = final
+        r = _runner(plan, code, precision, int(checkpoint_every or 0))
+        stream = torch.cuda.current_stream(inputs[0].device).cuda_stream
+        out = r.forward(inputs, stream)
+        ctx.runner, ctx.generation = r, r.generation
         ctx.save_for_backward(*inputs)
         return out
 
     @staticmethod
     def backward(ctx, gout):
-        plan, S = ctx.plan, ctx.plan._stages
-        L = _lib.lib()
-        inputs = ctx.saved_tensors
-        gout = gout.contiguous()
-        code = _CODE[gout.dtype]
-        grads = {}
-        gfin = torch.empty(S.fin["input_shape"], dtype=gout.dtype, device=gout.device)
-        _lib.check(L.atn_finalize_adjoint(plan.handle, code, gout.data_ptr(), gfin.data_ptr(), _stream(gout)))
-        grads[S.fin["slot"]] = gfin
-        n_steps = len(S.steps)
-        if ctx.saved is not None:
-            segments = [(0, n_steps, ctx.saved)]
-        else:
-            starts = sorted(ctx.boundaries)
-            segments = [(a, (starts[i + 1] if i + 1 < len(starts) else n_steps), None) for i, a in enumerate(starts)]
-        for a, b, vals in reversed(segments):
-            if vals is None:  # recompute the segment from its boundary
-                live = dict(ctx.boundaries[a])
-                vals = dict(live)
-                for s in range(a, b):
-                    st = S.steps[s]
-                    c = _step_fwd(plan, s, live.pop(st["li"]), live.pop(st["ri"]), ctx.precision)
-                    live[st["out"]] = c
-                    vals[st["out"]] = c
-            for s in range(b - 1, a - 1, -1):
-                st = S.steps[s]
-                g = grads.pop(st["out"], None)
-                if g is None:
-                    continue
-                left, right = vals[st["li"]], vals[st["ri"]]
-                gl = torch.empty(st["left_shape"], dtype=g.dtype, device=g.device)
-                gr = torch.empty(st["right_shape"], dtype=g.dtype, device=g.device)
-                _lib.check(L.atn_step_adjoint(plan.handle, s, code, left.data_ptr(), right.data_ptr(), g.data_ptr(),
-                                              gl.data_ptr(), gr.data_ptr(), _stream(g)))
-                grads[st["li"]] = gl
-                grads[st["ri"]] = gr
-        out = []
-        for i, t in enumerate(inputs):
-            g = grads.get(i)
-            if g is None:
-                out.append(torch.zeros_like(t))
-                continue
-            if S.preps[i]["traces"]:
-                gi = torch.empty(S.preps[i]["input_shape"], dtype=g.dtype, device=g.device)
-                _lib.check(L.atn_prep_adjoint(plan.handle, i, code, g.data_ptr(), gi.data_ptr(), _stream(g)))
-                g = gi
-            out.append(g)
-        return (None, None, None, *out)
+        r = ctx.runner
+        inputs = list(ctx.saved_tensors)
+        stream = torch.cuda.current_stream(gout.device).cuda_stream
+        if r.generation != ctx.generation:
+            # another forward ran on this runner since: rebuild this one's checkpoints
+            r.forward(inputs, stream)
+            ctx.generation = r.generation
+        gout = gout.resolve_conj().resolve_neg().contiguous()
+        if tuple(gout.shape) != r.output_shape or _CODE.get(gout.dtype) != r.code:
+            raise ValueError("output gradient shape or dtype does not match the contraction's output")
+        grads = r.backward(inputs, gout, stream)
+        return (None, None, None, *grads)
 
 
 def contract(plan: Plan, *tensors, precision="native", checkpoint_every=None):
     """Differentiable contraction of device tensors with an unsliced plan."""
     if precision not in _lib.PREC:
         raise ValueError(f"unknown precision {precision!r}")
+    if not isinstance(plan, Plan):
+        raise ValueError("plan must be an arctn_amd.Plan")
     return ContractFunction.apply(plan, precision, checkpoint_every, *tensors)

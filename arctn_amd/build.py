@@ -13,7 +13,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libarctn_hip.so")
-SOURCES = ["plan.cpp", "movement.hip", "gemm.hip", "grouped.hip", "executor.cpp", "abi.cpp"]
+SOURCES = ["plan.cpp", "movement.hip", "gemm.hip", "grouped.hip", "executor.cpp", "adjoint.hip", "abi.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17", "-Wall", "-Wno-unused-result",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]

@@ -373,6 +373,40 @@ int atn_gemm_noalloc(int dtype, int precision, int kernel, size_t batch, size_t 
   });
 }
 
+struct atn_adjoint {
+  std::unique_ptr<GradRunner> r;
+};
+
+int atn_adjoint_create(const atn_plan* plan, int dtype, int precision, size_t checkpoint_every, atn_adjoint** out) {
+  return guard([&] {
+    if (!plan || !out) throw Error("NULL argument");
+    *out = nullptr;
+    check_dtype(dtype);
+    if (precision < 0 || precision > 3) throw Error("unsupported precision code");
+    auto r = std::make_unique<GradRunner>(plan->p, dtype, precision, checkpoint_every);
+    *out = new atn_adjoint{std::move(r)};
+  });
+}
+
+void atn_adjoint_destroy(atn_adjoint* a) { delete a; }
+
+int atn_adjoint_forward(atn_adjoint* a, const void* const* d_inputs, void* d_out, void* stream) {
+  return guard([&] {
+    if (!a || !d_out || (!d_inputs && a->r->n_inputs())) throw Error("NULL argument");
+    a->r->forward(d_inputs, d_out, static_cast<hipStream_t>(stream));
+  });
+}
+
+int atn_adjoint_backward(atn_adjoint* a, const void* const* d_inputs, const void* d_grad_out, void* const* d_grad_inputs,
+                         void* stream) {
+  return guard([&] {
+    if (!a || !d_grad_out || ((!d_inputs || !d_grad_inputs) && a->r->n_inputs())) throw Error("NULL argument");
+    for (size_t i = 0; i < a->r->n_inputs(); ++i)
+      if (!d_inputs[i] || !d_grad_inputs[i]) throw Error(cat("NULL input or gradient pointer ", i));
+    a->r->backward(d_inputs, d_grad_out, d_grad_inputs, static_cast<hipStream_t>(stream));
+  });
+}
+
 struct atn_grouped {
   std::unique_ptr<GroupedExecutor> ex;
 };

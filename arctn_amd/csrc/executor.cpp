@@ -995,24 +995,28 @@ void Executor::run(const void* const* inputs, void* out, hipStream_t s, Report* 
 }
 
 // --------------------------------------------------------------- stages ------
-namespace {
-struct Scratch {
-  hipStream_t s;
-  std::vector<void*> ptrs;
-  explicit Scratch(hipStream_t st) : s(st) {}
-  void* get(size_t bytes) {
-    void* p = nullptr;
-    ATN_HIP(hipMallocAsync(&p, std::max<size_t>(bytes, 256), s));
-    ptrs.push_back(p);
+Scratch::~Scratch() {
+  for (void* p : ptrs) (void)hipFreeAsync(p, s);
+}
+
+void* Scratch::get(size_t bytes) {
+  bytes = align_up(std::max<size_t>(bytes, 256), 256);
+  if (pool && pool->base) {
+    if (used + bytes > pool->cap) throw Error("stage scratch pool too small", -3);
+    void* p = pool->base + used;
+    used += bytes;
     return p;
   }
-  ~Scratch() {
-    for (void* p : ptrs) (void)hipFreeAsync(p, s);
-  }
-};
+  void* p = nullptr;
+  ATN_HIP(hipMallocAsync(&p, bytes, s));
+  ptrs.push_back(p);
+  used += bytes;
+  if (pool) pool->peak = std::max(pool->peak, used);
+  return p;
+}
 
 // sums then permute of one operand; returns the pointer holding the GEMM operand
-const void* operand(Scratch& sc, int dt, const void* x, std::vector<size_t> shape, const std::vector<size_t>& sums,
+const void* stage_operand(Scratch& sc, int dt, const void* x, std::vector<size_t> shape, const std::vector<size_t>& sums,
                     const std::vector<size_t>& perm) {
   int es = (int)dtype_size(dt);
   for (size_t ax : sums) {
@@ -1031,8 +1035,8 @@ const void* operand(Scratch& sc, int dt, const void* x, std::vector<size_t> shap
   return dst;
 }
 
-// inverse of `operand`: g has the permuted shape; writes the gradient of x.
-void operand_adjoint(Scratch& sc, int dt, const void* g, std::vector<size_t> shape0, const std::vector<size_t>& sums,
+// inverse of `stage_operand`: g has the permuted shape; writes the gradient of x.
+void stage_operand_adjoint(Scratch& sc, int dt, const void* g, std::vector<size_t> shape0, const std::vector<size_t>& sums,
                      const std::vector<size_t>& perm, void* out) {
   int es = (int)dtype_size(dt);
   std::vector<std::vector<size_t>> shapes{shape0};
@@ -1068,7 +1072,6 @@ void operand_adjoint(Scratch& sc, int dt, const void* g, std::vector<size_t> sha
     cur = dst;
   }
 }
-}  // namespace
 
 void stage_prep_forward(const Plan& p, size_t i, int dt, const void* in, void* out, hipStream_t s) {
   if (i >= p.cp.n_inputs) throw Error("input index out of range");
@@ -1133,8 +1136,8 @@ void stage_step_forward(const Plan& p, size_t step, int dt, int prec, const void
   if (step >= p.cp.steps.size()) throw Error("step index out of range");
   const PairStep& st = p.cp.steps[step];
   Scratch sc(s);
-  const void* a = operand(sc, dt, l, st.left_shape, st.sum_a, st.a_perm);
-  const void* b = operand(sc, dt, r, st.right_shape, st.sum_b, st.b_perm);
+  const void* a = stage_operand(sc, dt, l, st.left_shape, st.sum_a, st.a_perm);
+  const void* b = stage_operand(sc, dt, r, st.right_shape, st.sum_b, st.b_perm);
   GemmArgs g{};
   g.a = a, g.b = b, g.c = out;
   g.m = st.m, g.k = st.k, g.n = st.n;
@@ -1152,8 +1155,8 @@ void stage_step_adjoint(const Plan& p, size_t step, int dt, const void* l, const
   const PairStep& st = p.cp.steps[step];
   int es = (int)dtype_size(dt);
   Scratch sc(s);
-  const void* a2 = operand(sc, dt, l, st.left_shape, st.sum_a, st.a_perm);
-  const void* b2 = operand(sc, dt, r, st.right_shape, st.sum_b, st.b_perm);
+  const void* a2 = stage_operand(sc, dt, l, st.left_shape, st.sum_a, st.a_perm);
+  const void* b2 = stage_operand(sc, dt, r, st.right_shape, st.sum_b, st.b_perm);
   int64_t bt = st.bt, m = st.m, k = st.k, n = st.n;
   if (gl) {
     // gA2[bt, m, k] = G[bt, m, n] . B2^H  (B2 stored [k, n])
@@ -1165,7 +1168,7 @@ void stage_step_adjoint(const Plan& p, size_t step, int dt, const void* l, const
     g.nbo = 1, g.nbi = bt;
     g.sai = m * n, g.sbi = k * n, g.sci = m * k;
     launch_gemm_adjoint(dt, 0, 1, g, s);
-    operand_adjoint(sc, dt, ga2, st.left_shape, st.sum_a, st.a_perm, gl);
+    stage_operand_adjoint(sc, dt, ga2, st.left_shape, st.sum_a, st.a_perm, gl);
   }
   if (gr) {
     // gB2[bt, k, n] = A2^H . G  (A2 stored [m, k])
@@ -1177,7 +1180,7 @@ void stage_step_adjoint(const Plan& p, size_t step, int dt, const void* l, const
     g.nbo = 1, g.nbi = bt;
     g.sai = m * k, g.sbi = m * n, g.sci = k * n;
     launch_gemm_adjoint(dt, 1, 0, g, s);
-    operand_adjoint(sc, dt, gb2, st.right_shape, st.sum_b, st.b_perm, gr);
+    stage_operand_adjoint(sc, dt, gb2, st.right_shape, st.sum_b, st.b_perm, gr);
   }
 }
 
@@ -1185,14 +1188,14 @@ void stage_finalize_forward(const Plan& p, int dt, const void* in, void* out, hi
   const Finalize& f = p.cp.fin;
   Scratch sc(s);
   int es = (int)dtype_size(dt);
-  const void* x = operand(sc, dt, in, f.input_shape, f.sum, f.perm);
+  const void* x = stage_operand(sc, dt, in, f.input_shape, f.sum, f.perm);
   launch_copy(prod(f.output_shape) * es, x, out, s);
 }
 
 void stage_finalize_adjoint(const Plan& p, int dt, const void* g, void* gin, hipStream_t s) {
   const Finalize& f = p.cp.fin;
   Scratch sc(s);
-  operand_adjoint(sc, dt, g, f.input_shape, f.sum, f.perm, gin);
+  stage_operand_adjoint(sc, dt, g, f.input_shape, f.sum, f.perm, gin);
 }
 
 }  // namespace atn

@@ -181,6 +181,32 @@ class Executor {
   void sync_inputs(const void* const* inputs);
 };
 
+// Scratch of one stage operation: bump allocation from a preallocated pool, or,
+// without a pool base, stream-ordered hipMallocAsync (freed when the object
+// dies); a pool without a base records the operation's peak need instead.
+struct ScratchPool {
+  char* base = nullptr;
+  size_t cap = 0, peak = 0;
+};
+struct Scratch {
+  hipStream_t s;
+  ScratchPool* pool;
+  size_t used = 0;
+  std::vector<void*> ptrs;
+  explicit Scratch(hipStream_t st, ScratchPool* p = nullptr) : s(st), pool(p) {}
+  Scratch(const Scratch&) = delete;
+  Scratch& operator=(const Scratch&) = delete;
+  ~Scratch();
+  void* get(size_t bytes);
+};
+// sum_axis over `sums` then permute by `perm` (the GEMM operand of a pair step);
+// returns x itself when nothing moves.
+const void* stage_operand(Scratch& sc, int dt, const void* x, std::vector<size_t> shape, const std::vector<size_t>& sums,
+                          const std::vector<size_t>& perm);
+// The adjoint of stage_operand: inverse permutation, then broadcast over each summed axis.
+void stage_operand_adjoint(Scratch& sc, int dt, const void* g, std::vector<size_t> shape0,
+                           const std::vector<size_t>& sums, const std::vector<size_t>& perm, void* out);
+
 // Stage execution with stream-ordered scratch (src/compiled.rs:60-63, 143-153, 206-209).
 void stage_prep_forward(const Plan& p, size_t i, int dt, const void* in, void* out, hipStream_t s);
 void stage_prep_adjoint(const Plan& p, size_t i, int dt, const void* g, void* gin, hipStream_t s);
@@ -189,6 +215,67 @@ void stage_step_adjoint(const Plan& p, size_t step, int dt, const void* l, const
                         hipStream_t s);
 void stage_finalize_forward(const Plan& p, int dt, const void* in, void* out, hipStream_t s);
 void stage_finalize_adjoint(const Plan& p, int dt, const void* g, void* gin, hipStream_t s);
+
+// Reverse-mode differentiation of an unsliced plan (no reference counterpart):
+// forward() computes the output and keeps the checkpoint slots on the device;
+// backward() then takes dL/d(conj out) and writes dL/d(conj input) of every
+// input, recomputing the intermediates of each checkpoint segment. The inputs
+// must be the forward's and unchanged. Tiny steps of one tree level share one
+// launch; intermediates live in one planned workspace.
+class GradRunner {
+ public:
+  GradRunner(std::shared_ptr<const Plan> plan, int dt, int prec, size_t checkpoint_every);
+  ~GradRunner();
+  GradRunner(const GradRunner&) = delete;
+  GradRunner& operator=(const GradRunner&) = delete;
+  void forward(const void* const* inputs, void* out, hipStream_t s);
+  void backward(const void* const* inputs, const void* grad_out, void* const* grad_inputs, hipStream_t s);
+  size_t workspace_bytes() const { return ws_bytes_; }
+  size_t n_ops() const { return ops_.size(); }
+  size_t n_inputs() const { return n_in_; }
+
+ private:
+  struct Loc;
+  struct Buf {
+    size_t bytes;
+    int first, last;
+  };
+  struct GOp {
+    enum Kind { PREP, FWD_GROUP, FWD_ONE, FIN, FIN_ADJ, ADJ_GROUP, ADJ_ONE, PREP_ADJ } kind;
+    int t;
+    size_t index;               // step (FWD_ONE / ADJ_ONE) or input (PREP / PREP_ADJ)
+    std::vector<size_t> items;  // steps of a grouped launch
+    int vals = 0;               // 0: forward values, 1: backward-pass values
+    size_t desc_first = 0, n_desc = 0;
+    uint64_t blocks = 0;
+  };
+  int newbuf(size_t bytes);
+  void* at(const Loc& l) const;
+  void set_bases(const void* const* inputs, void* const* grad_inputs, void* out, const void* grad_out, hipStream_t s);
+  void run_ops(size_t begin, size_t end, hipStream_t s);
+  void ensure_pool(hipStream_t s);
+
+  std::shared_ptr<const Plan> P_;
+  int dt_, prec_, es_ = 4;
+  size_t n_in_ = 0;
+  std::vector<Loc> val_f_, val_r_, grad_;
+  std::vector<Buf> bufs_;
+  std::vector<size_t> offs_;
+  std::vector<GOp> ops_;
+  size_t fwd_end_ = 0;
+  std::vector<int32_t> tab_;
+  char* ws_ = nullptr;
+  size_t ws_bytes_ = 0;
+  void* descs_dev_ = nullptr;
+  int32_t* tab_dev_ = nullptr;
+  char** bases_dev_ = nullptr;
+  char** bases_host_ = nullptr;
+  std::vector<char*> bases_;
+  size_t n_bases_ = 0;
+  ScratchPool pool_;
+  bool have_forward_ = false;
+  std::vector<const void*> forward_inputs_;
+};
 
 // Many independent unsliced plans run in lockstep grouped launches. Planned,
 // packed into memory-bounded groups and uploaded once; run() replays them.

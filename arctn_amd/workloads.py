@@ -263,45 +263,40 @@ def random_arrays(net, dtype, seed):
 def brickwork_torch_tensors(slots, angles):
     """Input tensors of the doubled brickwork network as torch tensors built
     from ``angles`` (shape (layers, n_qubits, 2): RY angle, RZ angle), so that
-    gradients flow back to the angles through torch's autograd."""
+    gradients flow back to the angles through torch's autograd.
+
+    Every single-qubit gate U = RZ(phi) . RY(theta) is built at once for all
+    layers and qubits by a handful of batched torch operations; the inputs are
+    views of that batch (and of its conjugate for the bra half), so the tensor
+    construction and its backward cost a few launches, not a few per gate."""
     import torch
 
     cdt = torch.complex128 if angles.dtype == torch.float64 else torch.complex64
     dev = angles.device
-    one = torch.ones((), dtype=angles.dtype, device=dev)
-    zero = torch.zeros((), dtype=angles.dtype, device=dev)
+    th, ph = angles[..., 0], angles[..., 1]
+    c, s = torch.cos(0.5 * th).to(cdt), torch.sin(0.5 * th).to(cdt)
+    e = torch.polar(torch.ones_like(ph), -0.5 * ph)  # exp(-i phi / 2)
+    ec = e.conj()
+    # RZ . RY = [[e c, -e s], [conj(e) s, conj(e) c]]
+    u = torch.stack([torch.stack([e * c, -e * s], -1), torch.stack([ec * s, ec * c], -1)], -2)
+    uc = u.conj().resolve_conj()
     ket0 = torch.tensor([1.0, 0.0], dtype=cdt, device=dev)
     cz = torch.diag(torch.tensor([1, 1, 1, -1], dtype=cdt, device=dev)).reshape(2, 2, 2, 2)
     z = torch.diag(torch.tensor([1.0, -1.0], dtype=cdt, device=dev))
     eye = torch.eye(2, dtype=cdt, device=dev)
-    cache = {}
-
-    def gate(layer, q):
-        key = (layer, q)
-        if key not in cache:
-            th, ph = angles[layer, q, 0], angles[layer, q, 1]
-            c, s = torch.cos(th / 2), torch.sin(th / 2)
-            ry = torch.stack([torch.stack([c, -s]), torch.stack([s, c])]).to(cdt)
-            e = torch.exp(torch.complex(zero, -0.5 * ph * one))
-            rz = torch.diag(torch.stack([e, e.conj()]))
-            cache[key] = rz @ ry
-        return cache[key]
-
     out = []
-    for s in slots:
-        if s[0] == "ket0":
+    for sl in slots:
+        if sl[0] == "ket0":
             out.append(ket0)
-        elif s[0] == "u":
-            u = gate(s[1], s[2])
-            # resolve_conj materialises torch's lazy conjugate view
-            out.append(u.conj().resolve_conj() if s[3] else u)
-        elif s[0] == "cz":
+        elif sl[0] == "u":
+            out.append((uc if sl[3] else u)[sl[1], sl[2]])
+        elif sl[0] == "cz":
             out.append(cz)
-        elif s[0] == "z":
+        elif sl[0] == "z":
             out.append(z)
         else:
             out.append(eye)
-    return [t.contiguous() for t in out]
+    return out
 
 
 def uneven_networks(n_inst, seed=0, n_lo=40, n_hi=120, dims=(2, 3, 4), dtype=np.float32):

@@ -179,6 +179,23 @@ int atn_step_adjoint(const atn_plan* plan, size_t step, int dtype, const void* l
 int atn_finalize_forward(const atn_plan* plan, int dtype, const void* in, void* out, void* stream);
 int atn_finalize_adjoint(const atn_plan* plan, int dtype, const void* grad_out, void* grad_in, void* stream);
 
+/* Reverse-mode differentiation of an unsliced plan in two calls (no reference
+ * counterpart: the reference has no gradient code, and its stage hook,
+ * src/compiled.rs:7-10, is what a tape would drive). forward writes the output
+ * and keeps the checkpoint slots on the device; backward takes dL/d(conj out)
+ * (PyTorch's complex convention) and writes dL/d(conj input) for every input,
+ * recomputing the intermediates of each checkpoint segment (checkpoint_every
+ * steps in contraction-tree level order; 0 keeps every intermediate). backward
+ * needs the inputs of the last forward, unchanged. Pair steps run level by level,
+ * the small ones of a level in one launch; precision applies to the large
+ * forward products (the adjoint products run in the dtype's native MFMA). */
+typedef struct atn_adjoint atn_adjoint;
+int atn_adjoint_create(const atn_plan* plan, int dtype, int precision, size_t checkpoint_every, atn_adjoint** out);
+void atn_adjoint_destroy(atn_adjoint* a);
+int atn_adjoint_forward(atn_adjoint* a, const void* const* d_inputs, void* d_out, void* stream);
+int atn_adjoint_backward(atn_adjoint* a, const void* const* d_inputs, const void* d_grad_out, void* const* d_grad_inputs,
+                         void* stream);
+
 /* Element-level kernels, exported for parity tests of the data-movement path
  * (src/tensor.rs:453-669): op 0 permute(arg = axes), 1 sum_axis(arg[0]),
  * 2 trace_pair(arg[0], arg[1]), 3 diag_pair(arg[0], arg[1]),

@@ -179,3 +179,30 @@ def test_slice_range_validated_before_device_work():
     o = _lib.ExecOptions(1, 0, 1, 0, 0, 0, 0, 0, 2, 9)  # beyond the 4 slices
     assert L.atn_executor_create(p.handle, C.byref(o), C.byref(h)) != 0
     assert b"out of bounds" in L.atn_last_error()
+
+
+def test_device_arguments_checked_before_the_abi():
+    """Plan.run / profile and the autograd contraction check count, dtype, shape,
+    contiguity and device of every tensor (and of a caller's output) before any
+    pointer reaches the C ABI; these checks need no GPU."""
+    import torch
+
+    from arctn_amd import autograd
+
+    net = {"inputs": [[0, 1], [1, 2]], "output": [0, 2], "size_dict": {0: 2, 1: 3, 2: 4}}
+    plan = Plan(net["inputs"], net["output"], net["size_dict"], [(0, 1)])
+    a, b = torch.zeros(2, 3, dtype=torch.complex64), torch.zeros(3, 4, dtype=torch.complex64)
+    with pytest.raises(ValueError, match="张量个数"):
+        plan.check_device_args([a], 2)
+    with pytest.raises(ValueError, match="not a GPU tensor"):
+        plan.check_device_args([a, b], 2)
+    with pytest.raises(ValueError, match="张量个数"):
+        autograd.check_inputs(plan, [a])
+    with pytest.raises(ValueError, match="dtype"):
+        autograd.check_inputs(plan, [a, b.to(torch.complex128)])
+    with pytest.raises(ValueError, match="shape"):
+        autograd.check_inputs(plan, [a, torch.zeros(4, 3, dtype=torch.complex64)])
+    with pytest.raises(ValueError, match="GPU"):
+        autograd.check_inputs(plan, [a, b])
+    with pytest.raises(ValueError, match="unsupported dtype code"):
+        plan.check_device_args([a, b], 7)
