@@ -142,6 +142,8 @@ def main():
         return bench_precision(args)
     if args.workload == "brickwork_20q_10l":
         return bench_gradient(args)
+    if args.workload == "permute_dim2":
+        return bench_permute(args)
 
     from arctn_amd import Plan
     from arctn_amd.api import profile
@@ -400,16 +402,13 @@ def bench_grouped(args):
     plans = [Plan(n["inputs"], n["output"], n["size_dict"], p) for n, _, p in insts]
     arrs = [[torch.from_numpy(x).to(dev) for x in a] for _, a, _ in insts]
     flops = sum(2.0 * p.info()["macs_per_slice"] for p in plans)
-    # float64 oracle (and the same plans in float32 on the CPU) on the instances the CPU finishes quickly
-    cheap = [i for i, c in enumerate(doc["log10_mults"]) if c <= 8.0]
-    t0 = time.perf_counter()
-    refs = {}
-    for i in cheap:
-        net, a, path = insts[i]
-        m64, e64 = O.contract(net, [x.astype(np.float64) for x in a], path, mode="stripped")
-        m32, e32 = O.contract(net, a, path, mode="stripped")
-        refs[i] = (float(m64), e64, abs(float(m32) * 10 ** (e32 - e64) - float(m64)) / abs(float(m64)))
-    oracle_s = time.perf_counter() - t0
+    # every instance against the committed float64 CPU-oracle values; per-instance bound
+    # max(1e-5, 8 x the error of the same contraction in float32 on the CPU)
+    path = os.path.join(ROOT, "tests", "golden", args.workload + ".json")
+    with open(path) as f:
+        fx = json.load(f)
+    ref64, ref32 = fx["f64"], fx["f32"]
+    bound = [max(1e-5, 8.0 * abs(b - a) / abs(a)) for a, b in zip(ref64, ref32)]
     precs = ["native", "bf16x3", "fp16x2"] if args.precision == "auto" else [args.precision]
     variants, best = {}, None
     stream = torch.cuda.current_stream(dev)
@@ -429,21 +428,16 @@ def bench_grouped(args):
         torch.cuda.synchronize()
         ms = e0.elapsed_time(e1) / args.steps
         _, _, rep = G.run(arrs, report=True)
-        vals = [(float(o.cpu()), e) for o, e in zip(outs, exps)]
-        errs = []
-        for i, (ref, e64, err32) in refs.items():
-            m, e = vals[i]
-            errs.append(abs(m * 10 ** (e - e64) - ref) / abs(ref))
-        # the scalar of some instances comes out of cancellation, where float32
-        # itself is off by ~1e-4: the bar is the median at 1e-5 and the worst case
-        # within twice the worst error of the same plans in float32 on the CPU
-        ok = bool(np.median(errs) <= 1e-5 and max(errs) <= 2 * max(v[2] for v in refs.values()))
+        errs = [abs(float(o.cpu()) * 10.0 ** e - r) / abs(r) for o, e, r in zip(outs, exps, ref64)]
+        n_ok = sum(e <= b for e, b in zip(errs, bound))
+        ok = n_ok == len(errs)
         variants[prec] = {"ms": ms, "tflops": flops / ms / 1e9, "n_launches": rep["n_launches"], "groups": rep["batch"],
                           "workspace_bytes": rep["workspace_bytes"], "build_s": build_s,
                           "max_rel_err_vs_f64": max(errs), "median_rel_err_vs_f64": float(np.median(errs)),
+                          "instances_within_bound": n_ok, "max_err_over_bound": max(e / b for e, b in zip(errs, bound)),
                           "within_bar": ok}
         log(f"[bench] {prec}: {ms:.1f} ms/step, {flops / ms / 1e9:.2f} TFLOP/s, {rep['n_launches']} launches, "
-            f"max rel err {max(errs):.2e} (median {np.median(errs):.1e}), within bar: {ok}")
+            f"max rel err {max(errs):.2e} (median {np.median(errs):.1e}), {n_ok}/{len(errs)} within bound")
         if ok and (best is None or ms < variants[best]["ms"]):
             best = prec
         del G
@@ -462,20 +456,25 @@ def bench_grouped(args):
         log(f"[bench] per-instance executor loop: {pms:.1f} ms -> {pms / r['ms']:.2f}x")
     cpu = None
     if not args.skip_cpu:
-        # CPU oracle (float32, the executed arithmetic) on the cheapest instances, about 10 s of work
+        # CPU oracle (float32, the executed arithmetic, stripped) on the cheapest
+        # instances, one instance per host thread (ctypes releases the GIL), ~10 s
+        import concurrent.futures as cf
+
+        threads, host = host_cpu()
         order = sorted(range(len(insts)), key=lambda i: doc["log10_mults"][i])
         done, mults, t0 = 0, 0.0, time.perf_counter()
-        for i in order:
-            net, a, path = insts[i]
-            O.contract(net, a, path, mode="stripped")
-            done += 1
-            mults += 10 ** doc["log10_mults"][i]
-            if time.perf_counter() - t0 > 10.0:
-                break
+        with cf.ThreadPoolExecutor(max_workers=threads) as ex:
+            for start in range(0, len(order), threads):
+                batch = order[start:start + threads]
+                list(ex.map(lambda i: O.contract(insts[i][0], insts[i][1], insts[i][2], mode="stripped"), batch))
+                done += len(batch)
+                mults += sum(10 ** doc["log10_mults"][i] for i in batch)
+                if time.perf_counter() - t0 > 10.0:
+                    break
         dt = time.perf_counter() - t0
-        cpu = {"value": 2.0 * mults / dt / 1e12, "unit": "TFLOP/s", "cores": 1, "kind": "port",
-               "sample": f"the {done} cheapest instances (float32, stripped), oracle CPU restatement, {dt:.1f} s",
-               "instances_per_s": done / dt}
+        cpu = {"value": 2.0 * mults / dt / 1e12, "unit": "TFLOP/s", "cores": threads, "kind": "port", "host": host,
+               "sample": f"the {done} cheapest instances (float32, stripped), oracle CPU restatement, one instance "
+                         f"per thread, {dt:.1f} s", "instances_per_s": done / dt}
     line = {
         "metric": "grouped contraction TFLOP/s of uneven instances", "value": round(r["tflops"], 3),
         "unit": "TFLOP/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(r["ms"], 3),
@@ -485,9 +484,11 @@ def bench_grouped(args):
                    "bond_dims": [2, 3, 4], "paths": "random greedy, seed = instance id", "strip": True,
                    "precision": best, "flops_per_step": flops},
         "instances_per_s": round(len(plans) / (r["ms"] * 1e-3), 2),
-        "accuracy": {"instances_checked": len(cheap), "oracle_seconds": oracle_s,
-                     "bar": "median rel err <= 1e-5 and max rel err <= 2 x the float32 CPU oracle's max error",
-                     "cpu_f32_max_rel_err_vs_f64": max(v[2] for v in refs.values())},
+        "accuracy": {"instances_checked": len(ref64), "reference": "float64 CPU oracle (tests/golden)",
+                     "bar": "every instance: rel err <= max(1e-5, 8 x the float32 CPU oracle's rel err)",
+                     "cpu_f32_max_rel_err_vs_f64": max(abs(b - a) / abs(a) for a, b in zip(ref64, ref32)),
+                     "cpu_f32_median_rel_err_vs_f64": float(np.median([abs(b - a) / abs(a)
+                                                                      for a, b in zip(ref64, ref32)]))},
         "variants": variants,
         "cpu_baseline": cpu,
     }
@@ -578,9 +579,68 @@ def bench_gradient(args):
     print(json.dumps(line), flush=True)
 
 
+def bench_permute(args):
+    """The leg permutation kernel (atn_tensor_op op 0) on high-rank dim-2 tensors of
+    the sizes the 30-qubit circuit's intermediates have (2^24 to 2^28 complex64
+    elements, rank 24 to 28), random axis orders and a full reversal: HBM GB/s
+    (2 x bytes per launch / HIP-event time per launch) against the 8 TB/s peak."""
+    import ctypes as C
+
+    import numpy as np
+    import torch
+
+    from arctn_amd import _lib
+
+    L = _lib.lib()
+    rng = np.random.default_rng(7)
+    cases = []
+    for rank in (24, 26, 28):
+        perm = [int(x) for x in rng.permutation(rank)]
+        cases.append((rank, perm, "random"))
+        cases.append((rank, list(range(rank))[::-1], "reversed"))
+        # the circuit's common case: a few legs move across, most stay in order
+        p = list(range(rank))
+        p = p[:rank - 6] + p[rank - 3:] + p[rank - 6:rank - 3]
+        cases.append((rank, [p[-1]] + p[:-1], "rotate+swap"))
+    res = []
+    s = torch.cuda.current_stream()
+    for rank, perm, kind in cases:
+        n = 1 << rank
+        x = torch.randn(n, dtype=torch.complex64, device="cuda")
+        y = torch.empty_like(x)
+        sh = (C.c_size_t * rank)(*([2] * rank))
+        ar = (C.c_size_t * rank)(*perm)
+        call = lambda: _lib.check(L.atn_tensor_op(2, 0, rank, sh, ar, rank, x.data_ptr(), y.data_ptr(), s.cuda_stream))
+        for _ in range(args.warmup):
+            call()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        e0.record(s)
+        for _ in range(args.steps):
+            call()
+        e1.record(s)
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / args.steps
+        gbs = 2.0 * n * 8 / (ms * 1e-3) / 1e9
+        res.append({"rank": rank, "elements": n, "kind": kind, "perm": perm, "ms": ms, "GB_per_s": gbs,
+                    "frac_of_hbm_peak": gbs / (HBM_PEAK / 1e9)})
+        log(f"[bench] permute rank {rank} ({kind}): {ms:.3f} ms, {gbs:.0f} GB/s")
+        del x, y
+        torch.cuda.empty_cache()
+    best = max(r["GB_per_s"] for r in res)
+    print(json.dumps({"metric": "leg permutation HBM GB/s (rank 24-28, dim 2, complex64)", "value": round(best, 1),
+                      "unit": "GB/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+                      "higher_is_better": True, "dtype": "complex64", "data": "synthetic",
+                      "config": {"workload": "permute_dim2"}, "cases": res}), flush=True)
+
+
 def bench_precision(args):
-    """36-qubit 6x6 circuit, 12 cycles, 64 amplitudes (output-sliced): FP32 MFMA,
-    BF16x3 and FP16x2 on a slice subset against the complex128 CPU oracle."""
+    """36-qubit 6x6 circuit, 12 cycles, six open qubits output-sliced into 64 blocks of
+    256 internal slices. Every precision variant (FP32 MFMA, 3M, BF16x3, FP16x2; the
+    split ones also with split products forced on every step) on internal slices
+    [0, 16) of eight blocks against the complex128 CPU oracle (committed fixture);
+    then the whole plan (16384 slices) in the fastest variant within 1e-5 and in
+    complex128 on the GPU, compared amplitude by amplitude."""
     import numpy as np
     import torch
 
@@ -590,66 +650,89 @@ def bench_precision(args):
 
     net, arrays, doc = load(args.workload)
     plan = Plan(net["inputs"], net["output"], net["size_dict"], doc["ssa_path"], doc["sliced_legs"])
-    info, d = plan.info(), plan.describe()
-    S = min(args.subset_slices, info["n_slices"] // max(info["n_blocks"], 1))
+    info = plan.info()
     dev = torch.device("cuda", 0)
     x64 = [torch.from_numpy(a).to(dev) for a in arrays]
-    x128 = [t.to(torch.complex128) for t in x64]
-    log(f"[bench] plan: {info['n_slices']} slices in {info['n_blocks']} output blocks; subset: slices 0..{S - 1} "
-        f"of block 0 (outer legs {d['outer']} = 0)")
-    # CPU oracle on the subset: outer legs fixed to index 0, internal slices [0, S)
-    outer = set(d["outer"])
-    sub_inputs, sub_arrays = [], []
-    for legs, a in zip(net["inputs"], arrays):
-        a = a.astype(np.complex128)
-        for ax in reversed(range(len(legs))):
-            if legs[ax] in outer:
-                a = np.take(a, 0, axis=ax)
-        sub_inputs.append([l for l in legs if l not in outer])
-        sub_arrays.append(np.array(a, order="C", copy=True))
-    sub_net = {"inputs": sub_inputs, "output": [l for l in net["output"] if l not in outer],
-               "size_dict": {k: v for k, v in net["size_dict"].items() if k not in outer}}
-    cores = min(16, os.cpu_count() or 1)
-    t0 = time.perf_counter()
-    ref = O.contract_slice_range(sub_net, sub_arrays, [tuple(p) for p in doc["ssa_path"]], d["internal"], 0, S,
-                                 threads=cores)
-    cpu_s = time.perf_counter() - t0
-    log(f"[bench] CPU complex128 oracle on {S} slices: {cpu_s:.1f} s on {cores} threads")
-    # GPU block-0 entries: outer legs at index 0 in the full output
-    sel = tuple(0 if l in outer else slice(None) for l in net["output"])
-    ref = ref.reshape(-1)
-    results = {}
-    best = None
-    for prec, code, xs in (("native", 2, x64), ("bf16x3", 2, x64), ("fp16x2", 2, x64), ("complex128", 3, x128)):
-        opts = dict(batched=True, use_graph=True, slice_range=(0, S))
-        if code == 2:
-            opts["precision"] = prec
-        for _ in range(2):
-            out, rep = plan.run(xs, code, report=True, **opts)
-        got = out.cpu().numpy()[sel].reshape(-1).astype(np.complex128)
-        rel = np.abs(got - ref) / np.abs(ref)
-        results[prec] = {"max_rel_err": float(rel.max()), "median_rel_err": float(np.median(rel)),
-                         "norm_rel_err": float(np.linalg.norm(got - ref) / np.linalg.norm(ref)),
-                         "ms": rep["elapsed_ms"], "slices_per_s": S / (rep["elapsed_ms"] * 1e-3),
-                         "tflops": rep["tflops"]}
+    n_int = info["n_slices"] // info["n_blocks"]
+    fx = fixture(args.workload)
+    if fx is None:
+        raise SystemExit("the CPU-oracle fixture of this workload is missing (tests/golden/make_circuit_fixtures.py)")
+    lo, hi = fx["internal_slices"]
+    blocks = sorted(int(b) for b in fx["blocks"])
+    log(f"[bench] plan: {info['n_slices']} slices = {info['n_blocks']} output blocks x {n_int} internal slices; "
+        f"subset: internal slices [{lo}, {hi}) of blocks {blocks}")
+    variants = [("native", None), ("c3m", None), ("bf16x3", None), ("fp16x2", None),
+                ("bf16x3", 4), ("bf16x3", 8), ("fp16x2", 4), ("fp16x2", 8)]
+    subset = {}
+    for prec, kern in variants:
+        name = prec + (f"+forced_kernel{kern}" if kern is not None else "")
+        errs, ms = [], 0.0
+        for b in blocks:
+            for _ in range(2):  # build + timed
+                out, rep = plan.run(x64, 2, report=True, batched=True, use_graph=True, precision=prec,
+                                    gemm_kernel=kern, slice_range=(b * n_int + lo, b * n_int + hi))
+            want = complex(*fx["blocks"][str(b)])
+            errs.append(abs(complex(out.cpu().numpy().reshape(-1)[b]) - want) / abs(want))
+            ms += rep["elapsed_ms"]
+            plan.release()
+        subset[name] = {"max_rel_err": max(errs), "median_rel_err": float(np.median(errs)), "ms_8_ranges": ms}
+        log(f"[bench] {name}: max rel err {max(errs):.2e} over {len(blocks)} blocks x {hi - lo} slices")
+    ok = [p for p, k in variants if k is None and subset[p]["max_rel_err"] <= 1e-5]
+    # the fastest variant within the bar on the subset's timing, then the whole plan in it,
+    # block by block (64 slice ranges of 256 internal slices; progress per block)
+    best = min(ok, key=lambda p: subset[p]["ms_8_ranges"]) if ok else "native"
+    amps, ms_total, flops_total = np.zeros(info["n_blocks"], dtype=np.complex128), 0.0, 0.0
+    for b in range(info["n_blocks"]):
+        for _ in range(2 if b == 0 else 1):  # the first block also builds the executor shape
+            out, rep = plan.run(x64, 2, report=True, batched=True, use_graph=True, precision=best,
+                                slice_range=(b * n_int, (b + 1) * n_int))
+        amps[b] = complex(out.cpu().numpy().reshape(-1)[b])
+        ms_total += rep["elapsed_ms"]
+        flops_total += rep["flops"]
         plan.release()
-        log(f"[bench] {prec}: max rel err {rel.max():.2e} over {len(ref)} amplitudes, {rep['elapsed_ms']:.1f} ms "
-            f"for {S} slices")
-        if code == 2 and results[prec]["max_rel_err"] <= 1e-5 and (best is None or rep["elapsed_ms"] < results[best]["ms"]):
-            best = prec
-    best = best or "native"
-    r = results[best]
+        if b % 8 == 7:
+            log(f"[bench] full plan in {best}: {b + 1}/{info['n_blocks']} blocks, {ms_total / 1e3:.1f} s so far")
+    full = {"precision": best, "ms": ms_total, "tflops": flops_total / (ms_total * 1e-3) / 1e12,
+            "slices_per_s": info["n_slices"] / (ms_total * 1e-3), "amplitudes": [[z.real, z.imag] for z in amps]}
+    # complex128 on the GPU for the fixture's eight blocks, all their internal slices
+    if not args.skip_ref:
+        x128 = [t.to(torch.complex128) for t in x64]
+        rel = []
+        for b in blocks:
+            out, _ = plan.run(x128, 3, batched=True, slice_range=(b * n_int, (b + 1) * n_int))
+            z = complex(out.cpu().numpy().reshape(-1)[b])
+            rel.append(abs(amps[b] - z) / abs(z))
+            plan.release()
+            log(f"[bench] block {b}: {best} vs complex128 GPU, all {n_int} slices: rel err {rel[-1]:.2e}")
+        full.update({"blocks_vs_gpu_complex128": blocks, "max_rel_err_vs_gpu_complex128": max(rel)})
+        del x128
+    # CPU baseline: the oracle on block 0's first internal slices, on the host's CPU share
+    threads, host = host_cpu()
+    cpu = None
+    if not args.skip_cpu:
+        from tests.golden.make_circuit_fixtures import block_network
+
+        n = min(args.cpu_slices or threads, n_int)
+        bnet, barr = block_network(net, arrays, 0)
+        internal = [l for l in doc["sliced_legs"] if l not in net["output"]]
+        t0 = time.perf_counter()
+        O.contract_slice_range(bnet, barr, [tuple(p) for p in doc["ssa_path"]], internal, 0, n, threads=threads)
+        dt = time.perf_counter() - t0
+        cpu = {"value": n / dt, "unit": "slices/s", "cores": threads, "kind": "port", "host": host,
+               "sample": f"internal slices 0..{n - 1} of block 0 (complex64), oracle CPU restatement, {dt:.1f} s"}
+        log(f"[bench] CPU oracle: {n / dt:.3f} slices/s on {threads} threads")
+    r = full
     line = {
-        "metric": "max relative error per precision variant vs complex128 CPU oracle (slice subset)",
-        "value": round(r["tflops"], 3), "unit": "TFLOP/s", "n_gpus": 1, "steps": 2, "warmup": 1,
-        "ms_per_step": round(r["ms"], 3), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "metric": "max relative error per precision variant vs complex128 CPU oracle; whole-plan TFLOP/s",
+        "value": round(r.get("tflops", 0.0), 3), "unit": "TFLOP/s", "n_gpus": 1, "steps": 1, "warmup": 1,
+        "ms_per_step": round(r.get("ms", 0.0), 3), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "complex64", "data": "synthetic (seeded random circuit, random-init gates)",
         "config": {"workload": args.workload, "qubits": 36, "grid": "6x6", "cycles": 12, "open_qubits": 6,
-                   "n_slices": info["n_slices"], "n_blocks": info["n_blocks"], "subset_slices": S,
-                   "fastest_within_1e-5": best},
-        "variants": results,
-        "cpu_baseline": {"value": S / cpu_s, "unit": "slices/s", "cores": cores, "kind": "port",
-                         "sample": f"slices 0..{S - 1} of block 0 (complex128), oracle CPU restatement, {cpu_s:.1f} s"},
+                   "n_slices": info["n_slices"], "n_blocks": info["n_blocks"], "internal_slices_per_block": n_int,
+                   "subset": f"internal slices [{lo}, {hi}) of blocks {blocks}", "fastest_within_1e-5": best},
+        "subset_vs_cpu_oracle": subset,
+        "full_plan": full,
+        "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
 

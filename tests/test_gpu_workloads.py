@@ -1,0 +1,139 @@
+"""The benchmark workloads at their real sizes against the committed CPU-oracle
+fixtures (tests/golden/, written by make_circuit_fixtures.py and
+make_uneven_fixture.py from the oracle in complex128 / float64):
+
+* 30-qubit 5x6 circuit, 14 cycles, 256 slices: the whole amplitude in every
+  precision form of the batched executor, and slices [0, 16) in complex128;
+* 36-qubit 6x6 circuit, 12 cycles, six open qubits output-sliced into 64
+  blocks of 256 internal slices: internal slices [0, 16) of several blocks in
+  every precision, split products forced on every step;
+* 512 uneven random 3-regular networks: every instance's scalar, grouped
+  execution in float32, against float64 with a per-instance bound.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from tests.golden import fixtures as F
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = os.path.join(os.path.dirname(F.__file__))
+
+
+def _fixture(name, key):
+    path = os.path.join(GOLDEN, name)
+    if not os.path.exists(path):
+        pytest.skip(f"{name} not generated")
+    with open(path) as f:
+        doc = json.load(f)
+    if key not in doc:
+        pytest.skip(f"{key} missing from {name}")
+    return doc[key]
+
+
+def _circuit(name):
+    import torch
+    from arctn_amd import Plan
+    from arctn_amd.plans.make_plans import load
+
+    net, arrays, doc = load(name)
+    plan = Plan(net["inputs"], net["output"], net["size_dict"], doc["ssa_path"], doc["sliced_legs"])
+    return net, plan, [torch.from_numpy(a).cuda() for a in arrays]
+
+
+# (precision, forced GEMM kernel or None, bound on the relative error)
+B_FORMS = [("native", None, 1e-5), ("fp16x2", None, 1e-5), ("c3m", None, 1e-5), ("bf16x3", None, 1e-4)]
+
+
+@pytest.mark.parametrize("form", range(len(B_FORMS)))
+def test_circuit_5x6_amplitude_vs_oracle_fixture(cuda, form):
+    """All 256 slices, batched + hipGraph, against the complex128 CPU oracle's amplitude."""
+    fx = _fixture("circuit_fixtures.json", "circuit_5x6_c14_t24")
+    want = complex(*fx["amplitude"])
+    net, plan, xs = _circuit("circuit_5x6_c14_t24")
+    prec, kern, tol = B_FORMS[form]
+    print(f"circuit_5x6 form {prec} kernel {kern}", flush=True)
+    got, _ = plan.run(xs, 2, batched=True, use_graph=True, precision=prec, gemm_kernel=kern)
+    err = abs(complex(got.cpu()) - want) / abs(want)
+    assert err <= tol, (prec, err)
+    if prec == "fp16x2":  # the autotuned form as well
+        got, _ = plan.run(xs, 2, batched=True, use_graph=True, precision=prec, autotune=True)
+        assert abs(complex(got.cpu()) - want) / abs(want) <= tol
+    plan.release()
+
+
+def test_circuit_5x6_slice_range_complex128_vs_oracle(cuda):
+    """Slices [0, 16) in complex128 (FP64 MFMA, FP64 sums) against the oracle's
+    contract_slice_range over the same slices."""
+    fx = _fixture("circuit_fixtures.json", "circuit_5x6_c14_t24")
+    lo, hi = fx["partial_slices"]
+    want = complex(*fx["partial"])
+    net, plan, xs = _circuit("circuit_5x6_c14_t24")
+    x128 = [x.to(dtype=__import__("torch").complex128) for x in xs]
+    got, _ = plan.run(x128, 3, batched=True, slice_range=(lo, hi))
+    assert abs(complex(got.cpu()) - want) / abs(want) <= 1e-12
+    got, _ = plan.run(xs, 2, batched=True, slice_range=(lo, hi))
+    assert abs(complex(got.cpu()) - want) / abs(want) <= 1e-5
+    plan.release()
+
+
+# forced split products on every step: BF16X3's 2^-16 relative product error adds
+# up over the ~700 steps of a slice (4.4e-4 on block 63), far above the 1e-5 bar,
+# which is why BF16X3 is never the selected form; FP16X2 forced everywhere stays within it
+C_FORMS = [("native", None, 1e-5), ("c3m", None, 1e-5), ("fp16x2", None, 1e-5), ("fp16x2", 4, 1e-5),
+           ("fp16x2", 8, 1e-5), ("bf16x3", None, 1e-4), ("bf16x3", 4, 1e-3), ("bf16x3", 8, 1e-3)]
+
+
+@pytest.mark.parametrize("form", range(len(C_FORMS)))
+def test_circuit_6x6_output_blocks_vs_oracle_fixture(cuda, form):
+    """Internal slices [0, 16) of output blocks 0, 27 and 63 of the 36-qubit circuit
+    (six open qubits, 64 output blocks) against the complex128 CPU oracle."""
+    fx = _fixture("circuit_fixtures.json", "circuit_6x6_c12_o6_t26")
+    lo, hi = fx["internal_slices"]
+    n_int = fx["n_internal"]
+    net, plan, xs = _circuit("circuit_6x6_c12_o6_t26")
+    assert plan.info()["n_blocks"] == fx["n_blocks"] == 64
+    prec, kern, tol = C_FORMS[form]
+    for b in (0, 27, 63):
+        print(f"circuit_6x6 form {prec} kernel {kern} block {b}", flush=True)
+        want = complex(*fx["blocks"][str(b)])
+        got, _ = plan.run(xs, 2, batched=True, use_graph=True, precision=prec, gemm_kernel=kern,
+                          slice_range=(b * n_int + lo, b * n_int + hi))
+        got = got.cpu().numpy().reshape(-1)
+        assert abs(complex(got[b]) - want) / abs(want) <= tol, (prec, kern, b)
+        others = np.delete(got, b)
+        assert not np.any(others), "only the block of the slice range is written"
+        plan.release()
+
+
+def test_uneven_512_instances_vs_float64_oracle(cuda):
+    """Every one of the 512 uneven 3-regular networks (n in [40, 120], bond dims
+    {2, 3, 4}), grouped execution in float32 with the strip, against the float64
+    CPU oracle. Per-instance bound: max(1e-5, 8 x the relative error the same
+    contraction makes in float32 on the CPU) - the scalar of a random network can
+    come out of heavy cancellation, where float32 arithmetic itself is off by 1e-4."""
+    import torch
+    from arctn_amd import GroupedContraction, Plan
+    from arctn_amd.plans.make_plans import load_uneven
+
+    fx = _fixture("uneven_3regular_512.json", "f64")
+    f32 = _fixture("uneven_3regular_512.json", "f32")
+    if any(v is None for v in fx):
+        pytest.skip("fixture incomplete")
+    insts, _ = load_uneven("uneven_3regular_512")
+    plans = [Plan(n["inputs"], n["output"], n["size_dict"], p) for n, _, p in insts]
+    arrs = [[torch.from_numpy(x).cuda() for x in a] for _, a, _ in insts]
+    g = GroupedContraction(plans, np.float32, strip=True, precision="native")
+    outs, exps = g.run(arrs)
+    bad = []
+    for i, (o, e) in enumerate(zip(outs, exps)):
+        got = float(o.cpu()) * 10.0 ** e
+        ref, cpu32 = fx[i], f32[i]
+        bound = max(1e-5, 8.0 * abs(cpu32 - ref) / abs(ref))
+        err = abs(got - ref) / abs(ref)
+        if not err <= bound:
+            bad.append((i, err, bound))
+    assert not bad, bad[:10]
