@@ -144,6 +144,8 @@ def main():
         return bench_gradient(args)
     if args.workload == "permute_dim2":
         return bench_permute(args)
+    if args.workload == "gemm_steps":
+        return bench_gemm_steps(args)
 
     from arctn_amd import Plan
     from arctn_amd.api import profile
@@ -577,6 +579,46 @@ def bench_gradient(args):
                                    "(the chain to the angles excluded)"},
     }
     print(json.dumps(line), flush=True)
+
+
+def bench_gemm_steps(args):
+    """The split-precision GEMM shapes of the 30-qubit circuit's contraction (complex64,
+    64 slices per batched launch, A shared by the batch where the plan shares it,
+    row-major operands) on each applicable atn_gemm kernel: 1 split while staging,
+    4 pre-split operands, 8 A pre-split and B split while staging; FP16X2."""
+    import torch
+
+    from arctn_amd import _lib
+
+    L = _lib.lib()
+    shapes = [(1024, 512, 8192, True), (512, 128, 32768, True), (512, 512, 8192, True), (128, 128, 131072, False),
+              (32, 128, 131072, True), (8, 256, 32768, True)]
+    bt = 64
+    s = torch.cuda.current_stream()
+    res = []
+    for m, k, n, shared in shapes:
+        a = torch.randn(1 if shared else bt, m, k, dtype=torch.complex64, device="cuda")
+        b = torch.randn(bt, k, n, dtype=torch.complex64, device="cuda")
+        c = torch.empty(bt, m, n, dtype=torch.complex64, device="cuda")
+        for kern in (1, 4, 8):
+            call = lambda: _lib.check(L.atn_gemm(2, 3, kern, bt, m, k, n, a.data_ptr(), 0 if shared else m * k,
+                                                 b.data_ptr(), k * n, c.data_ptr(), m * n, s.cuda_stream))
+            call()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            torch.cuda.synchronize()
+            e0.record(s)
+            for _ in range(args.steps):
+                call()
+            e1.record(s)
+            torch.cuda.synchronize()
+            ms = e0.elapsed_time(e1) / args.steps
+            tf = 8.0 * bt * m * k * n / (ms * 1e-3) / 1e12
+            res.append({"m": m, "k": k, "n": n, "batch": bt, "a_shared": shared, "kernel": kern, "ms": ms,
+                        "algorithmic_tflops": tf})
+            log(f"[bench] m={m} k={k} n={n} x{bt} kernel {kern}: {ms:.3f} ms, {tf:.1f} TFLOP/s")
+        del a, b, c
+        torch.cuda.empty_cache()
+    print(json.dumps({"metric": "split-precision GEMM steps (FP16X2, complex64)", "cases": res}), flush=True)
 
 
 def bench_permute(args):

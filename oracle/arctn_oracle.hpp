@@ -363,16 +363,24 @@ std::vector<T> batch_gemm(const std::vector<T>& a, const std::vector<T>& b, size
   gemm_geometry(bt, m, k, n);
   if (a.size() != bt * m * k || b.size() != bt * k * n) throw Error("GEMM operand length mismatch");
   std::vector<T> c(bt * m * n, Num<T>::zero());
+  // Column blocks of JB elements keep a block of C's row in L1 while p runs; every
+  // C[i,j] still accumulates A[i,p]*B[p,j] for p = 0..k-1 in order, from zero.
+  constexpr size_t JB = 4096 / sizeof(T);
   for (size_t ib = 0; ib < bt; ++ib) {
     const T* A = a.data() + ib * m * k;
     const T* B = b.data() + ib * k * n;
     T* C = c.data() + ib * m * n;
-    // i-p-j order: every C[i,j] still accumulates A[i,p]*B[p,j] for p = 0..k-1 in order.
-    for (size_t i = 0; i < m; ++i)
-      for (size_t p = 0; p < k; ++p) {
-        T aip = A[i * k + p];
-        for (size_t j = 0; j < n; ++j) C[i * n + j] = add(C[i * n + j], Num<T>::mul(aip, B[p * n + j]));
+    for (size_t j0 = 0; j0 < n; j0 += JB) {
+      const size_t j1 = std::min(n, j0 + JB);
+      for (size_t i = 0; i < m; ++i) {
+        T* Ci = C + i * n;
+        for (size_t p = 0; p < k; ++p) {
+          const T aip = A[i * k + p];
+          const T* Bp = B + p * n;
+          for (size_t j = j0; j < j1; ++j) Ci[j] = add(Ci[j], Num<T>::mul(aip, Bp[j]));
+        }
       }
+    }
   }
   return c;
 }
