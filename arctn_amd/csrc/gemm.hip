@@ -1056,8 +1056,14 @@ constexpr size_t SPLIT_LDS = 4 * 128 * (64 + 8) * 2;
 // Measured on gfx950 (complex64 1024x512x8192 x64, BF16x3): BK 64 with one LDS
 // buffer (two workgroups per CU) 302 TFLOP/s algorithmic, BK 32 with two 277.
 constexpr int PRESPLIT_BK = 64;
-// (hi, lo) halves of a BM x BK tile of A and a 128 x BK tile of B'
-constexpr size_t presplit_lds(int bm) { return 2 * (size_t)(bm + 128) * PRESPLIT_BK * 2; }
+// (hi, lo) halves of a BM x BK tile of A and a 128 x BK tile of B', then the
+// raw-B row table (rows words)
+constexpr size_t presplit_lds(int bm, int64_t rows = 0) {
+  return 2 * (size_t)(bm + 128) * PRESPLIT_BK * 2 + (size_t)(rows + 3) / 4 * 16;
+}
+// The most raw-B rows a row table holds (16 KiB: 80 KiB of LDS with 128-row
+// tiles, two workgroups per CU); longer k runs with pre-split B.
+constexpr int64_t PRESPLIT_MAX_TABLE = 4096;
 
 // ------------------------------------------------------------------ dispatch --
 // ------------------------------------------------ pre-split operand GEMM --
@@ -1172,6 +1178,34 @@ __device__ inline uint4 neg_im(uint4 v) { return make_uint4(v.x ^ 0x80000000u, v
 __device__ inline uint32_t swap16(uint32_t x) { return (x >> 16) | (x << 16); }
 __device__ inline uint4 swap_pairs(uint4 v) { return make_uint4(swap16(v.x), swap16(v.y), swap16(v.z), swap16(v.w)); }
 
+// (hi, lo) 16-bit pairs of two floats, x ~ hi + lo, as packed 32-bit words: one
+// packed conversion for hi, the residuals x - hi (exact in FP32), one for lo;
+// the same values as split16 on each float.
+typedef float v2f __attribute__((ext_vector_type(2)));
+template <bool F16> struct HalfPair;
+template <> struct HalfPair<false> { typedef __bf16 T __attribute__((ext_vector_type(2))); };
+template <> struct HalfPair<true> { typedef _Float16 T __attribute__((ext_vector_type(2))); };
+template <bool F16>
+__device__ inline void split_pair(v2f x, uint32_t& hi, uint32_t& lo) {
+  using P = typename HalfPair<F16>::T;
+  const P h = __builtin_convertvector(x, P);
+  const P l = __builtin_convertvector(x - __builtin_convertvector(h, v2f), P);
+  hi = __builtin_bit_cast(uint32_t, h), lo = __builtin_bit_cast(uint32_t, l);
+}
+
+// Buffer resources: a load at a byte offset at or past the resource's size
+// returns zeros, so rows past a matrix and the sentinel offsets below read 0
+// with no branch per load. Raw B offsets (row + column, bytes) stay below
+// RAW_LIMIT; a row past k carries ROW_OOB, a column past n COL_OOB, and any
+// sum with a sentinel lands at or past RAW_LIMIT without wrapping 32 bits.
+__device__ inline __amdgpu_buffer_rsrc_t buf_rsrc(const void* base, uint32_t bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, (int)bytes, 0x00020000);
+}
+constexpr uint32_t RAW_LIMIT = 1u << 30, ROW_OOB = 1u << 31, COL_OOB = 1u << 30;
+__device__ inline uint32_t tile_bytes(int64_t rows_left, int64_t row_bytes) {
+  return (uint32_t)std::min<int64_t>(rows_left * row_bytes, 0x7fffffff);
+}
+
 // LDS tiles are 128 rows of BK halves (no padding), one buffer (64 KiB, two
 // workgroups per CU); the next k tile waits in registers for a barrier. The
 // 16-byte chunk c of row r sits at chunk c ^ f(r) (f(r) = (r >> 1) & 7 for 8
@@ -1197,34 +1231,40 @@ __device__ inline int swz(int row, int ch) {
 // to four tiles per workgroup, measured the same on the circuit's steps: the
 // two workgroups per CU already overlap one's prologue with the other's MFMAs.)
 // BM: rows of A per tile. 128: four 64x64 wave tiles (two workgroups per CU);
-// 32: four 32x32 wave tiles side by side (40 KiB of LDS, three workgroups per
-// CU) for m <= 32, where B streams from HBM once and the MFMA work per B byte
-// is a quarter of the 128-row tile's. GATHER: raw B read through the gather
-// tables (always with lanes along k; a separate instantiation with lanes along
-// columns, so that row-major B carries no row-offset loads).
+// 32: four 32x32 wave tiles side by side (three workgroups per CU) for m <= 32,
+// where B streams from HBM once and the MFMA work per B byte is a quarter of
+// the 128-row tile's. GATHER: raw B read through the gather tables.
+// Every operand load is a buffer load at a per-thread offset fixed for the
+// whole kernel (A, pre-split B: the k tile goes into the scalar offset) or
+// one add away from it (raw B: the row's byte offset from an LDS table filled
+// once per workgroup, plus the thread's column offset, fixed per tile).
+// Layout of the dynamic LDS: ahi, alo [BM][BK]; bhi, blo [128][BK]; raw B: the
+// row table, kp / (CPLX ? 2 : 1) words (presplit_lds).
 template <bool CPLX, bool F16, int BK, int BMODE, int BM, bool GATHER>
 __global__ __launch_bounds__(256, BM == 32 ? 3 : 2) void gemm_presplit_kernel(GemmArgs g, PreSplit ps) {
   constexpr bool BRAW = BMODE != 0;
   using H = typename Half<F16>::H;
   using V = typename Half<F16>::V;
-  static_assert(BK == 32 || BK == 64, "k tile");
-  static_assert(BM == 128 || (BM == 32 && BK == 64), "row tile");
-  static_assert(GATHER == (BMODE == 2) || BMODE == 1, "lanes along k read gathered B; pre-split B no tables");
+  static_assert(BK == 64, "k tile (kp is a multiple of 64 halves)");
+  static_assert(BM == 128 || BM == 32, "row tile");
+  static_assert(!GATHER || BRAW, "pre-split B has no tables");
   constexpr int BN = 128, CH = BK / 8;
   constexpr int WM = BM == 128 ? 64 : 32, WN = WM, TM = WM / 16, TN = WN / 16;
   constexpr int A_Q = BM * CH / 256;                    // 16-byte A chunks per thread
   constexpr int B_Q = (CPLX ? BN / 2 : BN) * CH / 256;  // 16-byte B chunks per thread
   constexpr int TILE_A = BM * BK, TILE_B = BN * BK;     // halves per LDS array
-  // raw B: chunk q of a k tile is (tile column q % BC, k group q / BC), so the
-  // lanes of a wave read consecutive columns of one B row; a chunk is BE rows
+  // raw B with lanes along columns: chunk q of a k tile is (tile column q % BC,
+  // k group q / BC), so the lanes of a wave read consecutive columns of one B
+  // row; a chunk is BE rows
   constexpr int BC = CPLX ? BN / 2 : BN, BE = CPLX ? 4 : 8;
   static_assert(256 % BC == 0, "a thread's raw B loads share one column");
   using BT = typename std::conditional<CPLX, float2, float>::type;
   extern __shared__ __align__(16) char smem_split[];
-  H* const ahi = reinterpret_cast<H*>(smem_split);  // ahi, alo [TILE_A]; bhi, blo [TILE_B]
+  H* const ahi = reinterpret_cast<H*>(smem_split);
   H* const alo = ahi + TILE_A;
   H* const bhi = alo + TILE_A;
   H* const blo = bhi + TILE_B;
+  uint32_t* const tab = reinterpret_cast<uint32_t*>(blo + TILE_B);
 
   float unscale = 1.f, sb = 1.f;
   float mx = 0.f;
@@ -1239,38 +1279,27 @@ __global__ __launch_bounds__(256, BM == 32 ? 3 : 2) void gemm_presplit_kernel(Ge
   const int wm0 = BM == 128 ? (wave >> 1) * WM : 0, wn0 = BM == 128 ? (wave & 1) * WN : wave * WN;
   const uint64_t nz = (uint64_t)g.nbo * g.nbi, ntile = (uint64_t)tiles_m * tiles_n;
   const int64_t ainner = g.sai ? g.nbi : 1, binner = g.sbi ? g.nbi : 1;
-  // Gather offsets of the B rows a thread loads in a k tile (the same in every
-  // tile and batch entry): read one k tile ahead of their B loads, and those of
-  // the first two k tiles once per workgroup where registers allow. Vector
-  // loads, also where the row is wave-uniform: a scalar load in flight would
-  // make every LDS wait of the MFMA loop wait for it as well (scalar and LDS
-  // reads share one counter, and scalar ones return out of order), which cost a
-  // third of the kernel's rate on gathered operands.
-  constexpr int NR = BMODE == 2 ? B_Q : BMODE == 1 ? B_Q * BE : 1;
-  struct RowOffs {
-    int32_t v[NR];
-  };
-  auto rows_at = [&](int64_t k0) {
-    RowOffs r{};
-    if constexpr (BMODE == 2) {
+  if constexpr (BRAW) {  // byte offset of every B row (the same in every tile and batch entry)
+    const int64_t kr = kp / (CPLX ? 2 : 1);
+    for (int64_t r = tid; r < kr; r += 256)
+      tab[r] = r < g.k ? (uint32_t)((GATHER ? (int64_t)g.brow[r] : r * g.ldb) * (int64_t)sizeof(BT)) : ROW_OOB;
+    __syncthreads();
+  }
+  // per-thread byte offsets of the 16-byte chunks of a k tile within a row tile
+  // ([rows][kp] halves); the k tile's offset goes into the scalar offset
+  uint32_t a_off[A_Q], b_off[BRAW ? 1 : B_Q];
 #pragma unroll
-        for (int u = 0; u < B_Q; ++u) {
-          const int64_t r0 = (k0 + ((tid + 256 * u) % CH) * 8) / (CPLX ? 2 : 1);
-          r.v[u] = r0 < g.k ? g.brow[r0] : 0;
-        }
-    } else if constexpr (BMODE == 1 && GATHER) {
+  for (int u = 0; u < A_Q; ++u) {
+    const int q = tid + 256 * u;
+    a_off[u] = (uint32_t)(((q / CH) * kp + (q % CH) * 8) * 2);
+  }
+  if constexpr (!BRAW) {
 #pragma unroll
-        for (int u = 0; u < B_Q; ++u) {
-          const int64_t r0 = (k0 + (int64_t)((tid + 256 * u) / BC) * 8) / (CPLX ? 2 : 1);
-#pragma unroll
-          for (int e = 0; e < BE; ++e) r.v[u * BE + e] = r0 + e < g.k ? g.brow[r0 + e] : 0;
-        }
+    for (int u = 0; u < B_Q; ++u) {
+      const int q = tid + 256 * u;
+      b_off[u] = (uint32_t)(((q / CH) * kp + (q % CH) * 8) * 2);
     }
-    return r;
-  };
-  constexpr bool HOIST = GATHER && NR <= 8 && (BM == 32 || (BMODE == 2 && CPLX));
-  RowOffs rr0{}, rr1{};
-  if constexpr (HOIST) rr0 = rows_at(0), rr1 = rows_at(BK);
+  }
 
   for (uint64_t z = blockIdx.y; z < nz; z += gridDim.y) {
     const int64_t zo = (int64_t)(z / g.nbi), zi = (int64_t)(z - zo * g.nbi);
@@ -1282,78 +1311,79 @@ __global__ __launch_bounds__(256, BM == 32 ? 3 : 2) void gemm_presplit_kernel(Ge
     const H* Bl = BRAW ? nullptr : (const H*)ps.blo + bmat * g.n * kp;
     const BT* Braw = (const BT*)g.b + zo * g.sbo + zi * g.sbi;
     float* C = (float*)g.c + (zo * g.sco + zi * g.sci) * (CPLX ? 2 : 1);
+    const auto rb = buf_rsrc(Braw, RAW_LIMIT);
     for (uint64_t t = blockIdx.x; t < ntile; t += gridDim.x) {
       int64_t m0, n0;
       tile_origin(t, ntile, tiles_m, tiles_n, BM, BN, m0, n0);
+      const int64_t nc0 = CPLX ? n0 / 2 : n0;  // first compact column of B
       v4f acc[TM][TN];
 #pragma unroll
       for (int i = 0; i < TM; ++i)
 #pragma unroll
         for (int j = 0; j < TN; ++j) acc[i][j] = v4f{0, 0, 0, 0};
-      uint4 rah[A_Q], ral[A_Q], rbh[BRAW ? 1 : B_Q], rbl[BRAW ? 1 : B_Q];
-      BT rbr[BRAW ? B_Q * BE : 1];
-      // raw B: the thread's column (one per chunk with lanes along k), its gather
-      // offset hoisted out of the k loop
-      const int64_t bcn = (CPLX ? n0 / 2 : n0) + tid % BC;
-      const bool bcn_ok = BMODE == 1 && bcn < g.n;
-      const int64_t bco = bcn_ok ? (GATHER ? (int64_t)g.bcol[bcn] : bcn) : 0;
-      int64_t kco[BMODE == 2 ? B_Q : 1];
-      bool kok[BMODE == 2 ? B_Q : 1];
+      // this tile's rows of A and of pre-split B: rows past the end read zeros
+      const uint32_t a_bytes = tile_bytes(M - m0, kp * 2);
+      const auto rah_ = buf_rsrc(Ah + m0 * kp, a_bytes), ral_ = buf_rsrc(Al + m0 * kp, a_bytes);
+      const uint32_t b_bytes = BRAW ? 0u : tile_bytes(g.n - nc0, kp * 2);
+      const auto rbh_ = buf_rsrc(BRAW ? (const void*)Braw : (const void*)(Bh + nc0 * kp), b_bytes);
+      const auto rbl_ = buf_rsrc(BRAW ? (const void*)Braw : (const void*)(Bl + nc0 * kp), b_bytes);
+      // raw B: the byte offset of the thread's column (lanes along columns), or of
+      // the column of each chunk (lanes along k)
+      uint32_t bco = 0, kco[BMODE == 2 ? B_Q : 1];
+      if constexpr (BMODE == 1) {
+        const int64_t j = nc0 + tid % BC;
+        bco = j < g.n ? (uint32_t)((GATHER ? (int64_t)g.bcol[j] : j) * (int64_t)sizeof(BT)) : COL_OOB;
+      }
       if constexpr (BMODE == 2) {
 #pragma unroll
         for (int u = 0; u < B_Q; ++u) {
-          const int64_t j = (CPLX ? n0 / 2 : n0) + (tid + 256 * u) / CH;
-          kok[u] = j < g.n;
-          kco[u] = kok[u] ? (int64_t)g.bcol[j] : 0;
+          const int64_t j = nc0 + (tid + 256 * u) / CH;
+          kco[u] = j < g.n ? (uint32_t)((int64_t)g.bcol[j] * (int64_t)sizeof(BT)) : COL_OOB;
         }
       }
-      auto load = [&](int64_t k0, const RowOffs& ro) {
+      uint4 rah[A_Q], ral[A_Q], rbh[BRAW ? 1 : B_Q], rbl[BRAW ? 1 : B_Q];
+      BT rbr[BRAW ? B_Q * BE : 1];
+      auto load = [&](int64_t k0) {
+        const int ks = (int)(k0 * 2);
 #pragma unroll
         for (int u = 0; u < A_Q; ++u) {
-          const int q = tid + 256 * u, row = q / CH, ch = (q % CH) * 8;
-          const int64_t gr = m0 + row, gc = k0 + ch;
-          const bool ok = gr < M && gc < kp;
-          rah[u] = ok ? *reinterpret_cast<const uint4*>(Ah + gr * kp + gc) : make_uint4(0, 0, 0, 0);
-          ral[u] = ok ? *reinterpret_cast<const uint4*>(Al + gr * kp + gc) : make_uint4(0, 0, 0, 0);
+          rah[u] = __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(rah_, a_off[u], ks, 0));
+          ral[u] = __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(ral_, a_off[u], ks, 0));
         }
-        if constexpr (BMODE == 2) {
+        if constexpr (BMODE == 0) {
 #pragma unroll
           for (int u = 0; u < B_Q; ++u) {
-            // BE consecutive k of one column: within one run of consecutive row
-            // offsets (brow_run is a multiple of BE, chunks start at multiples of BE)
-            const int64_t r0 = (k0 + ((tid + 256 * u) % CH) * 8) / (CPLX ? 2 : 1);
-            const BT* bp = Braw + kco[u] + ro.v[u];
-            static_assert(BE * sizeof(BT) == 32, "a chunk is 32 bytes of B");
-            if (kok[u] && r0 + BE <= g.k && ((uintptr_t)bp & 15) == 0) {  // two 16-byte loads
-              const uint4* bw = reinterpret_cast<const uint4*>(bp);
-              uint4* dst = reinterpret_cast<uint4*>(&rbr[u * BE]);
-              dst[0] = bw[0], dst[1] = bw[1];
-            } else {
-#pragma unroll
-              for (int e = 0; e < BE; ++e) rbr[u * BE + e] = kok[u] && r0 + e < g.k ? bp[e] : BT{};
-            }
+            rbh[u] = __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(rbh_, b_off[u], ks, 0));
+            rbl[u] = __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(rbl_, b_off[u], ks, 0));
           }
         } else if constexpr (BMODE == 1) {
-          const BT* bp = Braw + bco;
 #pragma unroll
           for (int u = 0; u < B_Q; ++u) {
-            // first B row of the chunk (complex rows for complex B; k0 is even). BC is
-            // a multiple of the wave size, so the row is the same on every lane of a wave.
-            const int kc = __builtin_amdgcn_readfirstlane((tid + 256 * u) / BC);
-            const int64_t r0 = (k0 + kc * 8) / (CPLX ? 2 : 1);
+            // first B row of the chunk (complex rows for complex B); the same on
+            // every lane of a wave (BC is a multiple of the wave size)
+            const int kc = (tid + 256 * u) / BC;
+            const int r0 = (int)((k0 + kc * 8) / (CPLX ? 2 : 1));
+            uint32_t ro[BE];
+            *reinterpret_cast<uint4*>(ro) = *reinterpret_cast<const uint4*>(tab + r0);
+            if constexpr (BE == 8) *reinterpret_cast<uint4*>(ro + 4) = *reinterpret_cast<const uint4*>(tab + r0 + 4);
 #pragma unroll
-            for (int e = 0; e < BE; ++e)
-              rbr[u * BE + e] =
-                  bcn_ok && r0 + e < g.k ? bp[GATHER ? (int64_t)ro.v[u * BE + e] : (r0 + e) * g.ldb] : BT{};
+            for (int e = 0; e < BE; ++e) {
+              if constexpr (CPLX)
+                rbr[u * BE + e] = __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(rb, ro[e] + bco, 0, 0));
+              else
+                rbr[u * BE + e] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rb, ro[e] + bco, 0, 0));
+            }
           }
         } else {
 #pragma unroll
           for (int u = 0; u < B_Q; ++u) {
-            const int q = tid + 256 * u, row = q / CH, ch = (q % CH) * 8;
-            const int64_t gr = (CPLX ? n0 / 2 : n0) + row, gc = k0 + ch;
-            const bool ok = gr < g.n && gc < kp;
-            rbh[u] = ok ? *reinterpret_cast<const uint4*>(Bh + gr * kp + gc) : make_uint4(0, 0, 0, 0);
-            rbl[u] = ok ? *reinterpret_cast<const uint4*>(Bl + gr * kp + gc) : make_uint4(0, 0, 0, 0);
+            // BE consecutive k of one column, 32 contiguous bytes (brow_run and k
+            // are multiples of BE, chunks start at multiples of BE)
+            const int r0 = (int)((k0 + ((tid + 256 * u) % CH) * 8) / (CPLX ? 2 : 1));
+            const uint32_t o = tab[r0] + kco[u];
+            uint4* dst = reinterpret_cast<uint4*>(&rbr[u * BE]);
+            dst[0] = __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(rb, o, 0, 0));
+            dst[1] = __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(rb, o + 16, 0, 0));
           }
         }
       };
@@ -1367,56 +1397,51 @@ __global__ __launch_bounds__(256, BM == 32 ? 3 : 2) void gemm_presplit_kernel(Ge
 #pragma unroll
         for (int u = 0; u < B_Q; ++u) {
           const int q = tid + 256 * u;
-          int row, ch;
-          uint4 vh, vl;
           if constexpr (BRAW) {
+            int row, ch;
             if constexpr (BMODE == 2) row = q / CH, ch = (q % CH) * 8;
             else row = q % BC, ch = (q / BC) * 8;
-            H h[8], l[8];
+            uint32_t h[4], l[4];
+            if constexpr (CPLX) {
+              uint32_t h2[4], l2[4];
 #pragma unroll
-            for (int e = 0; e < BE; ++e) {
-              if constexpr (CPLX) {
-                split16<F16>(rbr[u * BE + e].x * sb, h[2 * e], l[2 * e]);
-                split16<F16>(rbr[u * BE + e].y * sb, h[2 * e + 1], l[2 * e + 1]);
-              } else {
-                split16<F16>(rbr[u * BE + e] * sb, h[e], l[e]);
+              for (int e = 0; e < 4; ++e) {
+                const float re = rbr[u * 4 + e].x * sb, im = rbr[u * 4 + e].y * sb;
+                split_pair<F16>(v2f{re, -im}, h[e], l[e]);
+                split_pair<F16>(v2f{im, re}, h2[e], l2[e]);
               }
+              *reinterpret_cast<uint4*>(bhi + swz<BK>(row, ch)) = *reinterpret_cast<const uint4*>(h);
+              *reinterpret_cast<uint4*>(blo + swz<BK>(row, ch)) = *reinterpret_cast<const uint4*>(l);
+              *reinterpret_cast<uint4*>(bhi + swz<BK>(row + 64, ch)) = *reinterpret_cast<const uint4*>(h2);
+              *reinterpret_cast<uint4*>(blo + swz<BK>(row + 64, ch)) = *reinterpret_cast<const uint4*>(l2);
+            } else {
+#pragma unroll
+              for (int e = 0; e < 4; ++e)
+                split_pair<F16>(v2f{rbr[u * 8 + 2 * e] * sb, rbr[u * 8 + 2 * e + 1] * sb}, h[e], l[e]);
+              *reinterpret_cast<uint4*>(bhi + swz<BK>(row, ch)) = *reinterpret_cast<const uint4*>(h);
+              *reinterpret_cast<uint4*>(blo + swz<BK>(row, ch)) = *reinterpret_cast<const uint4*>(l);
             }
-            vh = *reinterpret_cast<const uint4*>(h), vl = *reinterpret_cast<const uint4*>(l);
           } else {
-            row = q / CH, ch = (q % CH) * 8;
-            vh = rbh[u], vl = rbl[u];
-          }
-          if constexpr (CPLX) {
-            *reinterpret_cast<uint4*>(bhi + swz<BK>(row, ch)) = neg_im(vh);
-            *reinterpret_cast<uint4*>(blo + swz<BK>(row, ch)) = neg_im(vl);
-            *reinterpret_cast<uint4*>(bhi + swz<BK>(row + 64, ch)) = swap_pairs(vh);
-            *reinterpret_cast<uint4*>(blo + swz<BK>(row + 64, ch)) = swap_pairs(vl);
-          } else {
-            *reinterpret_cast<uint4*>(bhi + swz<BK>(row, ch)) = vh;
-            *reinterpret_cast<uint4*>(blo + swz<BK>(row, ch)) = vl;
+            const int row = q / CH, ch = (q % CH) * 8;
+            if constexpr (CPLX) {
+              *reinterpret_cast<uint4*>(bhi + swz<BK>(row, ch)) = neg_im(rbh[u]);
+              *reinterpret_cast<uint4*>(blo + swz<BK>(row, ch)) = neg_im(rbl[u]);
+              *reinterpret_cast<uint4*>(bhi + swz<BK>(row + 64, ch)) = swap_pairs(rbh[u]);
+              *reinterpret_cast<uint4*>(blo + swz<BK>(row + 64, ch)) = swap_pairs(rbl[u]);
+            } else {
+              *reinterpret_cast<uint4*>(bhi + swz<BK>(row, ch)) = rbh[u];
+              *reinterpret_cast<uint4*>(blo + swz<BK>(row, ch)) = rbl[u];
+            }
           }
         }
       };
-      const int64_t nk = (kp + BK - 1) / BK;
+      const int64_t nk = kp / BK;
       const int cl = lane & 15;
-      RowOffs rn{};  // row offsets of k tile kt + 1
-      if constexpr (HOIST) {
-        load(0, rr0);
-        rn = rr1;
-      } else {
-        load(0, rows_at(0));
-        if constexpr (GATHER)
-          if (nk > 1) rn = rows_at(BK);
-      }
+      load(0);
       store();
       __syncthreads();
       for (int64_t kt = 0; kt < nk; ++kt) {
-        if (kt + 1 < nk) {
-          load((kt + 1) * BK, rn);
-          if constexpr (GATHER)
-            if (kt + 2 < nk) rn = rows_at((kt + 2) * BK);
-        }
+        if (kt + 1 < nk) load((kt + 1) * BK);
 #pragma unroll
         for (int ks = 0; ks < BK; ks += 32) {
           const int kg = ks + (lane >> 4) * 8;
@@ -1771,12 +1796,21 @@ struct PreSplitSizes {
 static PreSplitSizes presplit_sizes(int dt, const GemmArgs& g) {
   PreSplitSizes r;
   const int64_t K = dt == C64 ? 2 * g.k : g.k;
-  r.kp = (K + 7) / 8 * 8;
+  r.kp = (K + PRESPLIT_BK - 1) / PRESPLIT_BK * PRESPLIT_BK;  // whole k tiles, zero padded
   auto r256 = [](size_t b) { return (b + 255) / 256 * 256; };
   r.a_half = r256((size_t)n_mats(g.sao, g.sai, g.nbo, g.nbi) * (size_t)g.m * (size_t)r.kp * 2);
   r.b_half = r256((size_t)n_mats(g.sbo, g.sbi, g.nbo, g.nbi) * (size_t)g.n * (size_t)r.kp * 2);
   r.total = 2 * r.a_half + 2 * r.b_half;
   return r;
+}
+
+// The pre-split kernel reads raw B (GEMM_PRESPLIT_A) when its row table fits
+// LDS and its byte offsets stay below 1 GiB (gather tables address one k x n
+// batch entry); otherwise B is pre-split as well.
+static bool raw_b_fits(int dt, const GemmArgs& g) {
+  const int64_t b_span = g.bcol ? g.k * g.n : (g.k - 1) * g.ldb + g.n;
+  return presplit_sizes(dt, g).kp / (dt == C64 ? 2 : 1) <= PRESPLIT_MAX_TABLE &&
+         b_span * (dt == C64 ? 8 : 4) < (int64_t)(1u << 30);
 }
 
 // Split-precision products pay off on MFMA-bound shapes only; memory-bound
@@ -1843,10 +1877,12 @@ static void presplit_kernel_launch(const GemmArgs& g, const PreSplit& ps, dim3 g
   static bool attr_set = false;
   if (!attr_set) {
     ATN_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&gemm_presplit_kernel<C, F, PRESPLIT_BK, R, M, G>),
-                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)presplit_lds(M)));
+                                hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)presplit_lds(M, PRESPLIT_MAX_TABLE)));
     attr_set = true;
   }
-  LAUNCH((gemm_presplit_kernel<C, F, PRESPLIT_BK, R, M, G>), grid, dim3(256), presplit_lds(M), s, g, ps);
+  const size_t lds = presplit_lds(M, R ? ps.kp / (C ? 2 : 1) : 0);
+  LAUNCH((gemm_presplit_kernel<C, F, PRESPLIT_BK, R, M, G>), grid, dim3(256), lds, s, g, ps);
 }
 template <bool C, bool F, int R, int M>
 static void presplit_gemm(const GemmArgs& g, const PreSplit& ps, dim3 grid, hipStream_t s) {
@@ -1888,7 +1924,9 @@ static int launch_gemm_kernels(int dt, int prec, int kernel, const GemmArgs& g0,
   // the explicit row-tile kernels are the FP32 / FP64 MFMA kernels in any mode
   if (split_prec && (kernel == GEMM_MFMA || kernel == GEMM_PRESPLIT || kernel == GEMM_PRESPLIT_A)) {
     const PreSplitSizes psz = presplit_sizes(dt, g);
-    const bool raw_b = kernel == GEMM_PRESPLIT_A;
+    const bool raw_ok = raw_b_fits(dt, g);
+    const bool raw_b = kernel == GEMM_PRESPLIT_A && raw_ok;
+    if (kernel == GEMM_PRESPLIT_A && !raw_ok) kernel = GEMM_PRESPLIT;
     const bool pre = kernel != GEMM_MFMA && g.scratch &&
                      g.scratch_bytes >= 256 + (raw_b ? 2 * psz.a_half : psz.total);
     int64_t N = dt == C64 ? 2 * g.n : g.n;
@@ -1939,7 +1977,8 @@ static int launch_gemm_kernels(int dt, int prec, int kernel, const GemmArgs& g0,
       if (raw_b) {
         // lanes along k where the gathered B has runs of consecutive k offsets
         // (its contracted legs innermost in memory) and short column runs
-        if (g.bcol && g.brow_run % (cx ? 4 : 8) == 0 && g.bcol_run < 16) PRESPLIT_MODES(2);
+        if (g.bcol && g.brow_run % (cx ? 4 : 8) == 0 && g.k % (cx ? 4 : 8) == 0 && g.bcol_run < 16)
+          PRESPLIT_MODES(2);
         else PRESPLIT_MODES(1);
         return GEMM_PRESPLIT_A;
       }
@@ -2024,7 +2063,8 @@ size_t gemm_scratch_bytes(int dt, int prec, int kernel, const GemmArgs& g) {
   size_t part = ks.n_split > 1 ? (size_t)ks.n_split * dtype_size(dt) * (size_t)(g.nbo * g.nbi * g.m * g.n) : 0;
   if ((prec == BF16X3 || prec == FP16X2) && (dt == F32 || dt == C64)) {
     if (kernel == GEMM_PRESPLIT) part = presplit_sizes(dt, g).total;
-    if (kernel == GEMM_PRESPLIT_A) part = 2 * presplit_sizes(dt, g).a_half;
+    if (kernel == GEMM_PRESPLIT_A)
+      part = raw_b_fits(dt, g) ? 2 * presplit_sizes(dt, g).a_half : presplit_sizes(dt, g).total;
   }
   return 256 + part;
 }

@@ -210,7 +210,9 @@ int atn_tensor_op(int dtype, int op, size_t rank, const size_t* shape, const siz
  * 3 split-K kernel (workgroups split k; partial sums added in a fixed order),
  * 4 split-precision MFMA kernel on operands pre-split into 16-bit (hi, lo) arrays
  *   by a separate pass (for native precision: 1), 5 / 6 / 7 native MFMA kernel
- *   with a 32 / 64 / 128-row tile (1 picks the row tile from m). */
+ *   with a 32 / 64 / 128-row tile (1 picks the row tile from m), 8 the kernel of 4
+ *   with only A pre-split and B split as it is staged (B stays in one read; 4 where
+ *   B's layout does not allow it). */
 int atn_gemm(int dtype, int precision, int kernel, size_t batch, size_t m, size_t k, size_t n, const void* d_a,
              size_t stride_a, const void* d_b, size_t stride_b, void* d_c, size_t stride_c, void* stream);
 
