@@ -1,4 +1,5 @@
-"""Loader helpers for reference_fixtures.json (inputs + expected values only)."""
+"""Loader helpers for reference_fixtures.json (inputs + expected values only),
+and the per-instance bar of the uneven-networks fixture."""
 import json
 import os
 
@@ -37,4 +38,22 @@ def case_arrays(net, dtype, seed=871):
         if np.issubdtype(np.dtype(dtype), np.complexfloating):
             data = data + 1j * rng.standard_normal(shape)
         out.append(np.array(data, dtype=dtype, order="C"))
+    return out
+
+
+def uneven_bounds(doc):
+    """Per-instance bar of uneven_3regular_512.json: max(1e-5, 16 s_i), s_i the
+    root mean square of the relative errors of four CPU float32 contractions of
+    instance i (the fixture's float32 run and three with rescaled inputs, each a
+    fresh sample of where float32 rounding falls; the scalar of a random
+    network comes out of heavy cancellation on some instances, where float32
+    arithmetic itself is off by 1e-4 and more). A float32 contraction with its
+    own summation order is one more sample of that error: under a normal model
+    with s_i estimated from four samples it exceeds 16 s_i with probability
+    about 1e-4 per instance, while a wrong step or a dropped term errs by orders
+    of magnitude more."""
+    out = []
+    for v64, v32, rs in zip(doc["f64"], doc["f32"], doc["f32_rescaled"]):
+        errs = [abs(x - v64) / abs(v64) for x in [v32] + list(rs)]
+        out.append(max(1e-5, 16.0 * float(np.sqrt(np.mean(np.square(errs))))))
     return out

@@ -25,6 +25,15 @@ n_inputs), and multiplies the result back by 2^(s * n_inputs) in float64.
 Power-of-two scaling is exact: away from overflow and subnormals it changes
 no rounding, so the float32 error measured is the float32 arithmetic's own.
 
+Three more float32 runs per instance ("f32_rescaled") contract the inputs
+multiplied by c in {1.15, 1.3, 1.45} (each product x * c rounded to float32,
+then the power-of-two shift) and divide the result by c^n_inputs in float64.
+The exact value is unchanged, but every rounding in the contraction falls
+differently: each run is a fresh sample of the float32 arithmetic's error on
+that instance (plus one rounding of every input, the same size as one step's).
+One sample alone says little about an instance's error scale, four give an
+estimate of it (their root mean square) to within a small factor.
+
 The file is rewritten after every instance, so an interrupted run resumes.
 """
 import argparse
@@ -67,6 +76,31 @@ def _values(net, arrays, path, sliced, threads):
     return vals[0], vals[1], shift
 
 
+RESCALE = (1.15, 1.3, 1.45)
+
+
+def _rescaled(net, arrays, path, sliced, threads, v64):
+    """float32 values of the instance with the inputs multiplied by each c."""
+    out = []
+    n = len(arrays)
+    for c in RESCALE:
+        shift = int(round((np.log2(abs(v64)) + n * np.log2(c)) / n))
+        a = [np.ldexp((x.astype(np.float32) * np.float32(c)).astype(np.float32), -shift).astype(np.float32)
+             for x in arrays]
+        if sliced:
+            v = O.contract(net, a, path, "sliced", sliced=sliced, threads=threads)
+        else:
+            v = O.contract(net, a, path, "compiled")
+        out.append(float(np.ldexp(np.float64(v), shift * n) / np.float64(c) ** n))
+    return out
+
+
+def _rescaled_job(i, v64):
+    inst, _ = load_uneven(NAME)
+    net, arrays, path = inst[i]
+    return i, _rescaled(net, arrays, path, [], 1, v64)
+
+
 def _unsliced_job(i):
     inst, _ = load_uneven(NAME)
     net, arrays, path = inst[i]
@@ -81,7 +115,8 @@ def main():
     inst, _ = load_uneven(NAME)
     n = len(inst)
     doc = {"workload": NAME, "slice_target": SLICE_TARGET, "f64": [None] * n, "f32": [None] * n,
-           "f32_input_shift": [None] * n, "sliced_legs": [None] * n, "cpu_s": [None] * n}
+           "f32_input_shift": [None] * n, "sliced_legs": [None] * n, "cpu_s": [None] * n,
+           "f32_rescaled": [None] * n}
     if os.path.exists(OUT):
         with open(OUT) as f:
             old = json.load(f)
@@ -121,6 +156,26 @@ def main():
             net, arrays, path = inst[i]
             t0 = time.time()
             record(i, _values(net, arrays, path, sliced[i], args.threads), time.time() - t0)
+
+    def record_rescaled(i, vals):
+        doc["f32_rescaled"][i] = vals
+        with open(OUT + ".tmp", "w") as f:
+            json.dump(doc, f, separators=(",", ":"))
+        os.replace(OUT + ".tmp", OUT)
+        v = doc["f64"][i]
+        errs = [abs(doc["f32"][i] - v) / abs(v)] + [abs(x - v) / abs(v) for x in vals]
+        print(f"instance {i}: f32 rel errs " + " ".join(f"{e:.1e}" for e in errs)
+              + f", total {time.time() - t_all:.0f} s", flush=True)
+
+    todo = [i for i in sorted(range(n), key=lambda i: cost[i]) if doc["f32_rescaled"][i] is None]
+    with cf.ProcessPoolExecutor(max_workers=args.threads) as ex:
+        futs = [ex.submit(_rescaled_job, i, doc["f64"][i]) for i in todo if not sliced[i]]
+        for fut in cf.as_completed(futs):
+            record_rescaled(*fut.result())
+    for i in todo:
+        if sliced[i]:
+            net, arrays, path = inst[i]
+            record_rescaled(i, _rescaled(net, arrays, path, sliced[i], args.threads, doc["f64"][i]))
 
 
 if __name__ == "__main__":
