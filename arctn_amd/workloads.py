@@ -260,15 +260,53 @@ def random_arrays(net, dtype, seed):
     return out
 
 
+def _rows_function():
+    import torch
+
+    class Rows(torch.autograd.Function):
+        """Rows ``base[rows[i]]`` of a stacked tensor as separate tensors (one
+        gather); the backward adds all their gradients into one tensor of the
+        base's shape with one stack and one index_add."""
+
+        @staticmethod
+        def forward(ctx, base, rows):
+            ctx.base_shape = base.shape
+            ctx.save_for_backward(rows)
+            return tuple(base.index_select(0, rows).unbind(0))
+
+        @staticmethod
+        def backward(ctx, *grads):
+            (rows,) = ctx.saved_tensors
+            ref = next(g for g in grads if g is not None)
+            g = torch.stack([x if x is not None else torch.zeros_like(ref) for x in grads])
+            out = torch.zeros(ctx.base_shape, dtype=g.dtype, device=g.device)
+            return out.index_add_(0, rows, g), None
+
+    return Rows
+
+
+_ROWS = []
+
+
+def _rows(base, rows):
+    """``Rows.apply`` (the Function class is made on first use: torch is
+    imported lazily by this module)."""
+    if not _ROWS:
+        _ROWS.append(_rows_function())
+    return _ROWS[0].apply(base, rows)
+
+
 def brickwork_torch_tensors(slots, angles):
     """Input tensors of the doubled brickwork network as torch tensors built
     from ``angles`` (shape (layers, n_qubits, 2): RY angle, RZ angle), so that
     gradients flow back to the angles through torch's autograd.
 
     Every single-qubit gate U = RZ(phi) . RY(theta) is built at once for all
-    layers and qubits by a handful of batched torch operations; the inputs are
-    views of that batch (and of its conjugate for the bra half), so the tensor
-    construction and its backward cost a few launches, not a few per gate."""
+    layers and qubits by a handful of batched torch operations, and the 2x2
+    inputs are handed out by one ``Rows`` node (ket and bra halves alike), so
+    the construction and its backward cost a few launches, not a few per gate
+    (per-gate indexing made autograd build and add one zero-filled copy of the
+    whole batch per gate in the backward pass)."""
     import torch
 
     cdt = torch.complex128 if angles.dtype == torch.float64 else torch.complex64
@@ -279,7 +317,10 @@ def brickwork_torch_tensors(slots, angles):
     ec = e.conj()
     # RZ . RY = [[e c, -e s], [conj(e) s, conj(e) c]]
     u = torch.stack([torch.stack([e * c, -e * s], -1), torch.stack([ec * s, ec * c], -1)], -2)
-    uc = u.conj().resolve_conj()
+    uu = torch.stack([u, u.conj()])  # [ket / bra, layer, qubit, 2, 2]
+    layers, nq = u.shape[0], u.shape[1]
+    rows = [(sl[3] * layers + sl[1]) * nq + sl[2] for sl in slots if sl[0] == "u"]
+    gates = iter(_rows(uu.reshape(-1, 2, 2), torch.tensor(rows, device=dev)))
     ket0 = torch.tensor([1.0, 0.0], dtype=cdt, device=dev)
     cz = torch.diag(torch.tensor([1, 1, 1, -1], dtype=cdt, device=dev)).reshape(2, 2, 2, 2)
     z = torch.diag(torch.tensor([1.0, -1.0], dtype=cdt, device=dev))
@@ -289,7 +330,7 @@ def brickwork_torch_tensors(slots, angles):
         if sl[0] == "ket0":
             out.append(ket0)
         elif sl[0] == "u":
-            out.append((uc if sl[3] else u)[sl[1], sl[2]])
+            out.append(next(gates))
         elif sl[0] == "cz":
             out.append(cz)
         elif sl[0] == "z":

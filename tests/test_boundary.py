@@ -206,3 +206,25 @@ def test_device_arguments_checked_before_the_abi():
         autograd.check_inputs(plan, [a, b])
     with pytest.raises(ValueError, match="unsupported dtype code"):
         plan.check_device_args([a, b], 7)
+
+
+def test_brickwork_gate_tensors_and_their_gradient():
+    """The torch-built inputs of the doubled brickwork network equal the NumPy
+    ones, and the gradient through the batched gate construction (one node hands
+    out every 2x2 gate, ket and bra halves) matches finite differences
+    (torch.autograd.gradcheck, float64, CPU)."""
+    import torch
+
+    net, slots = W.brickwork_structure(4, 3)
+    ang = W.brickwork_angles(4, 3, 1)
+    th = torch.tensor(ang, dtype=torch.float64, requires_grad=True)
+    got = W.brickwork_torch_tensors(slots, th)
+    for t, want in zip(got, W.brickwork_arrays(slots, ang)):
+        np.testing.assert_array_equal(t.detach().numpy(), want)
+    gen = torch.Generator().manual_seed(3)
+    w = [torch.randn(t.shape, dtype=torch.complex128, generator=gen) for t in got]
+
+    def f(a):
+        return sum((wi * t).real.sum() for wi, t in zip(w, W.brickwork_torch_tensors(slots, a)))
+
+    assert torch.autograd.gradcheck(f, (th,))
