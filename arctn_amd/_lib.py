@@ -14,7 +14,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libarctn_hip.so")
 
 ATN_F32, ATN_F64, ATN_C64, ATN_C128 = 0, 1, 2, 3
-PREC = {"native": 0, "fp32": 0, "c3m": 1, "bf16x3": 2, "fp16x2": 3}
+PREC = {"native": 0, "fp32": 0, "c3m": 1, "bf16x3": 2, "fp16x2": 3, "bf16x3_3m": 4, "fp16x2_3m": 5}
 
 
 class NetworkDesc(C.Structure):

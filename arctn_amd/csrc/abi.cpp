@@ -55,7 +55,7 @@ static ExecOptions to_opts(const atn_exec_options* o) {
   ExecOptions e;
   if (!o) return e;
   if (o->dtype < 0 || o->dtype > 3) throw Error("unsupported dtype code");
-  if (o->precision < 0 || o->precision > 3) throw Error("unsupported precision code");
+  if (o->precision < 0 || o->precision > 5) throw Error("unsupported precision code");
   e.dtype = o->dtype;
   e.precision = o->precision;
   e.batched = o->batched != 0;
@@ -305,7 +305,7 @@ static GemmArgs gemm_args(int dtype, int precision, int kernel, size_t batch, si
   check_dtype(dtype);
   if (!batch || !m || !k || !n) throw Error("GEMM dimensions must be positive");
   if (kernel < -1 || kernel > 8) throw Error("unknown GEMM kernel");
-  if (precision < 0 || precision > 3) throw Error("unsupported precision code");
+  if (precision < 0 || precision > 5) throw Error("unsupported precision code");
   if (precision == C3M && dtype != C64) throw Error("precision c3m applies to complex64 only");
   GemmArgs g{};
   g.a = d_a, g.b = d_b, g.c = d_c;
@@ -382,7 +382,7 @@ int atn_adjoint_create(const atn_plan* plan, int dtype, int precision, size_t ch
     if (!plan || !out) throw Error("NULL argument");
     *out = nullptr;
     check_dtype(dtype);
-    if (precision < 0 || precision > 3) throw Error("unsupported precision code");
+    if (precision < 0 || precision > 5) throw Error("unsupported precision code");
     auto r = std::make_unique<GradRunner>(plan->p, dtype, precision, checkpoint_every);
     *out = new atn_adjoint{std::move(r)};
   });
@@ -417,7 +417,7 @@ int atn_grouped_create(const atn_plan* const* plans, size_t n_plans, int dtype, 
     if (!out || (n_plans && !plans)) throw Error("NULL argument");
     *out = nullptr;
     check_dtype(dtype);
-    if (precision < 0 || precision > 3) throw Error("unsupported precision code");
+    if (precision < 0 || precision > 5) throw Error("unsupported precision code");
     std::vector<std::shared_ptr<const Plan>> ps;
     for (size_t i = 0; i < n_plans; ++i) {
       if (!plans[i]) throw Error("NULL plan");
@@ -452,7 +452,7 @@ int atn_grouped_execute(const atn_plan* const* plans, size_t n_plans, int dtype,
                         atn_report* report) {
   return guard([&] {
     check_dtype(dtype);
-    if (precision < 0 || precision > 3) throw Error("unsupported precision code");
+    if (precision < 0 || precision > 5) throw Error("unsupported precision code");
     std::vector<std::shared_ptr<const Plan>> ps;
     std::vector<const void* const*> ins;
     std::vector<void*> outs;

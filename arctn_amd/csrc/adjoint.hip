@@ -164,7 +164,7 @@ GradRunner::GradRunner(std::shared_ptr<const Plan> plan, int dt, int prec, size_
   const Plan& P = *P_;
   if (P.n_slices != 1) throw Error("differentiation needs an unsliced plan");
   if (prec == C3M && dt != C64) throw Error("precision c3m applies to complex64 only");
-  if ((prec == BF16X3 || prec == FP16X2) && !(dt == F32 || dt == C64))
+  if (split_precision(prec) && !(dt == F32 || dt == C64))
     throw Error("split precision applies to float32 / complex64 only");
   es_ = (int)dtype_size(dt);
   const size_t n_in = P.cp.n_inputs, n_st = P.cp.steps.size(), nslot = n_in + n_st;

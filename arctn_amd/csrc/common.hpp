@@ -43,7 +43,12 @@ inline std::string vstr(const std::vector<size_t>& v) {
 
 // dtype codes of the C ABI
 enum DType { F32 = 0, F64 = 1, C64 = 2, C128 = 3 };
-enum Prec { NATIVE = 0, C3M = 1, BF16X3 = 2, FP16X2 = 3 };
+enum Prec { NATIVE = 0, C3M = 1, BF16X3 = 2, FP16X2 = 3, BF16X3_3M = 4, FP16X2_3M = 5 };
+// The split precisions with complex products as three real ones (3M) where the
+// 3M kernel applies: their half type is that of BF16X3 / FP16X2.
+inline bool split_3m(int prec) { return prec == BF16X3_3M || prec == FP16X2_3M; }
+inline int split_halves(int prec) { return prec == BF16X3_3M ? BF16X3 : prec == FP16X2_3M ? FP16X2 : prec; }
+inline bool split_precision(int prec) { return split_halves(prec) == BF16X3 || split_halves(prec) == FP16X2; }
 
 inline size_t dtype_size(int dt) {
   switch (dt) {

@@ -564,7 +564,7 @@ void Executor::build(size_t B) {
       }
   }
   want_max_.assign(nslot, 0);
-  if (o_.precision == FP16X2 && !o_.strip)
+  if (split_halves(o_.precision) == FP16X2 && !o_.strip)
     for (const PairStep& st : P.cp.steps)
       if (split_products_apply(dt, o_.precision, (int64_t)st.m, (int64_t)st.k, (int64_t)st.n)) {
         if (st.sum_a.empty()) want_max_[st.li] = 1;
@@ -692,7 +692,7 @@ Executor::Executor(std::shared_ptr<const Plan> plan, const ExecOptions& o) : P_(
   es_ = (int)dtype_size(o_.dtype);
   if (o_.precision != NATIVE) {
     if (o_.precision == C3M && o_.dtype != C64) throw Error("precision c3m applies to complex64 only");
-    if ((o_.precision == BF16X3 || o_.precision == FP16X2) && !(o_.dtype == C64 || o_.dtype == F32))
+    if (split_precision(o_.precision) && !(o_.dtype == C64 || o_.dtype == F32))
       throw Error("split precision applies to float32 / complex64 only");
   }
   if (o_.strip && P.n_slices > 1) throw Error("strip is available for unsliced plans (execute_stripped)");

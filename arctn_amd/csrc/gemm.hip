@@ -1208,9 +1208,13 @@ __device__ inline uint32_t tile_bytes(int64_t rows_left, int64_t row_bytes) {
 
 // LDS tiles are 128 rows of BK halves (no padding), one buffer (64 KiB, two
 // workgroups per CU); the next k tile waits in registers for a barrier. The
-// 16-byte chunk c of row r sits at chunk c ^ f(r) (f(r) = (r >> 1) & 7 for 8
-// chunks per row, (r >> 2) & 3 for 4): the 16 rows of one MFMA fragment read
-// and the rows of one 16-lane store each cover all 64 banks once.
+// 16-byte chunk c of row r sits at chunk c ^ f(r), f(r) = r & 7 for 8 chunks per
+// row and (r >> 1) & 3 for 4. Measured on gfx950 with a probe of every XOR map of
+// the row bits: 16-byte LDS stores are served 8 lanes (128 bytes) at a time, so
+// the 8 rows of a store group need distinct chunks, and of the maps that meet
+// that, these two also read an MFMA fragment (16 rows, one chunk per 16 lanes)
+// at full rate; (r >> 1) & 7, used before, read at full rate but conflicted
+// 2-way on the stores (and (r >> 2) & 3 on both).
 // Complex B: LDS row r (< 64) holds (re, -im) of compact row r and produces
 // output column 2r (real part); row 64 + r holds (im, re) and produces 2r + 1.
 // BRAW: B is not pre-split; the FP32 (or complex64) B is read through the gather
@@ -1219,7 +1223,7 @@ __device__ inline uint32_t tile_bytes(int64_t rows_left, int64_t row_bytes) {
 template <int BK>
 __device__ inline int swz(int row, int ch) {
   constexpr int CH = BK / 8;  // 16-byte chunks per row
-  const int f = CH == 8 ? ((row >> 1) & 7) : ((row >> 2) & 3);
+  const int f = CH == 8 ? (row & 7) : ((row >> 1) & 3);
   return row * BK + (((ch >> 3) ^ f) << 3);
 }
 
@@ -1512,6 +1516,262 @@ __global__ __launch_bounds__(256, BM == 32 ? 3 : 2) void gemm_presplit_kernel(Ge
         const int sw = cswz(tr);
         float4 v = *reinterpret_cast<const float4*>(&cs[tr * BN + (c4 ^ (sw & ~3))]);
         if (sw & 1) v = make_float4(v.y, v.x, v.w, v.z);
+        const int64_t row = m0 + tr, col = n0 + c4;
+        if (row < M) {
+          v.x *= unscale, v.y *= unscale, v.z *= unscale, v.w *= unscale;
+          if (c16 && col + 3 < N) {
+            *reinterpret_cast<float4*>(C + row * ldc + col) = v;
+            if (g.cmax) fold_max(mx, v.x), fold_max(mx, v.y), fold_max(mx, v.z), fold_max(mx, v.w);
+          } else {
+            const float e4[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+            for (int e = 0; e < 4; ++e)
+              if (col + e < N) {
+                C[row * ldc + col + e] = e4[e];
+                if (g.cmax) fold_max(mx, e4[e]);
+              }
+          }
+        }
+      }
+      __syncthreads();  // the next tile stages into the same LDS
+    }
+  }
+  if (g.cmax) publish_max((double)mx, g.cmax);
+}
+
+// ------------------------------------------- 3M split-precision GEMM --
+// complex64 C = A.B as Gauss's three real products on the 16-bit MFMA:
+//   T1 = Ar.Br, T2 = Ai.Bi, T3 = (Ar + Ai).(Br + Bi);  Cr = T1 - T2, Ci = T3 - T1 - T2
+// each real product as lo.hi + hi.lo + hi.hi of 16-bit halves accumulated in
+// FP32: 9 MFMA products per block of complex multiply-adds where the 2x2
+// expansion of gemm_presplit_kernel issues 12. A comes split by
+// split_a3m_kernel into three planes (Ar, Ai, Ar + Ai) of hi and lo halves,
+// [za][3][m][kq] (kq = k rounded up to 32, zero padded); B is read in complex64,
+// row-major or through the gather tables, lanes along columns (a thread loads 8
+// consecutive complex rows of one column), and split into its three planes while
+// staging. FP16 halves use per-operand scales 2^(s-1), one below the 2x2 form's,
+// since the sum planes reach twice the operand max.
+// Tile: 128 rows x 64 complex columns x 32 complex k (one MFMA k step); 4 waves
+// in a 2x2 arrangement, 64 rows x 32 complex columns each, three accumulator
+// planes. LDS: A planes [3][128][32] hi and lo, B planes [3][64][32] (B
+// transposed) hi and lo, rows of 32 halves whose 16-byte chunk c sits at
+// c ^ ((row >> 1) & 3), then the row table of raw B (kq words); the epilogue
+// reuses the first 64 KiB as a [128][128] float tile.
+constexpr int S3M_BK = 32;
+constexpr size_t s3m_lds(int64_t kq) { return (size_t)(3 * 128 + 3 * 64) * S3M_BK * 2 * 2 + (size_t)kq * 4; }
+__device__ inline int swz32(int row, int ch) { return row * 32 + ((ch ^ ((row >> 1) & 3)) << 3); }
+
+// A (complex64 [m][k]) -> planes [za][3][m][kq] hi / lo; one thread per 4
+// consecutive k of a row (all three planes).
+template <bool F16>
+__global__ __launch_bounds__(256) void split_a3m_kernel(GemmArgs g, PreSplit ps) {
+  using H = typename Half<F16>::H;
+  const int64_t kq = ps.kp, inner = g.sai ? g.nbi : 1, nm = n_mats(g.sao, g.sai, g.nbo, g.nbi);
+  const int64_t per_row = kq / 4, total = nm * g.m * per_row;
+  float sa = 1.f;
+  if constexpr (F16) sa = ldexpf(1.f, pow2_shift(*g.amax) - 1);
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t mat = e / (g.m * per_row), r = e - mat * g.m * per_row, row = r / per_row, c = (r - row * per_row) * 4;
+    const int64_t zo = mat / inner, zi = mat - zo * inner;
+    const float2* src = (const float2*)g.a + zo * g.sao + zi * g.sai + row * g.lda;
+    H h[3][4], l[3][4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const float2 x = c + u < g.k ? src[c + u] : make_float2(0.f, 0.f);
+      const float re = x.x * sa, im = x.y * sa;
+      split16<F16>(re, h[0][u], l[0][u]);
+      split16<F16>(im, h[1][u], l[1][u]);
+      split16<F16>(re + im, h[2][u], l[2][u]);
+    }
+#pragma unroll
+    for (int p = 0; p < 3; ++p) {
+      const int64_t off = ((mat * 3 + p) * g.m + row) * kq + c;
+      *reinterpret_cast<uint2*>((H*)ps.ahi + off) = *reinterpret_cast<const uint2*>(h[p]);
+      *reinterpret_cast<uint2*>((H*)ps.alo + off) = *reinterpret_cast<const uint2*>(l[p]);
+    }
+  }
+}
+
+template <bool F16, bool GATHER>
+__global__ __launch_bounds__(256, 2) void gemm_split3m_kernel(GemmArgs g, PreSplit ps) {
+  using H = typename Half<F16>::H;
+  using V = typename Half<F16>::V;
+  constexpr int BM = 128, BNC = 64, BK = S3M_BK, CH = BK / 8;
+  constexpr int TM = 4, TN = 2;                     // 16x16 blocks of a wave tile
+  constexpr int PA = BM * BK, PB = BNC * BK;        // halves per plane
+  constexpr int A_Q = BM * CH / 256;                // 16-byte chunks of one A plane array per thread (2)
+  extern __shared__ __align__(16) char smem_split[];
+  H* const ahi = reinterpret_cast<H*>(smem_split);  // [3][BM][BK]
+  H* const alo = ahi + 3 * PA;
+  H* const bhi = alo + 3 * PA;                      // [3][BNC][BK]
+  H* const blo = bhi + 3 * PB;
+  uint32_t* const tab = reinterpret_cast<uint32_t*>(blo + 3 * PB);
+
+  float unscale = 1.f, sb = 1.f;
+  float mx = 0.f;
+  if constexpr (F16) {
+    const int ea = pow2_shift(*g.amax) - 1, eb = pow2_shift(*g.bmax) - 1;
+    unscale = ldexpf(1.f, -(ea + eb));
+    sb = ldexpf(1.f, eb);
+  }
+  const int64_t M = g.m, N = 2 * g.n, kq = ps.kp, ldc = 2 * g.ldc;
+  const int tiles_n = (int)((g.n + BNC - 1) / BNC), tiles_m = (int)((M + BM - 1) / BM);
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, cl = lane & 15, g4 = lane >> 4;
+  const int wm0 = (wave >> 1) * 64, wn0 = (wave & 1) * 32;
+  const uint64_t nz = (uint64_t)g.nbo * g.nbi, ntile = (uint64_t)tiles_m * tiles_n;
+  const int64_t ainner = g.sai ? g.nbi : 1;
+  // byte offset of every B row (the same in every tile and batch entry)
+  for (int64_t r = tid; r < kq; r += 256)
+    tab[r] = r < g.k ? (uint32_t)((GATHER ? (int64_t)g.brow[r] : r * g.ldb) * 8) : ROW_OOB;
+  __syncthreads();
+  // per-thread byte offsets of the A chunks within a plane's row tile; B: the
+  // thread's column (tid % 64) and group of 8 complex rows (tid / 64, one per wave)
+  uint32_t a_off[A_Q];
+#pragma unroll
+  for (int u = 0; u < A_Q; ++u) {
+    const int q = tid + 256 * u;
+    a_off[u] = (uint32_t)(((q / CH) * kq + (q % CH) * 8) * 2);
+  }
+  const int bj = tid % BNC, bgrp = tid / BNC;
+
+  for (uint64_t z = blockIdx.y; z < nz; z += gridDim.y) {
+    const int64_t zo = (int64_t)(z / g.nbi), zi = (int64_t)(z - zo * g.nbi);
+    const int64_t amat = (g.sao ? zo : 0) * ainner + (g.sai ? zi : 0);
+    const float2* Braw = (const float2*)g.b + zo * g.sbo + zi * g.sbi;
+    float* C = (float*)g.c + (zo * g.sco + zi * g.sci) * 2;
+    const auto rb = buf_rsrc(Braw, RAW_LIMIT);
+    for (uint64_t t = blockIdx.x; t < ntile; t += gridDim.x) {
+      int64_t m0, nc0;
+      tile_origin(t, ntile, tiles_m, tiles_n, BM, BNC, m0, nc0);
+      v4f acc[3][TM][TN];
+#pragma unroll
+      for (int p = 0; p < 3; ++p)
+#pragma unroll
+        for (int i = 0; i < TM; ++i)
+#pragma unroll
+          for (int j = 0; j < TN; ++j) acc[p][i][j] = v4f{0, 0, 0, 0};
+      // the three A planes of this batch entry as one resource each for hi and lo;
+      // plane p's rows of this tile start at scalar byte offset p * plane + tile
+      // (rows past m of a plane read the next plane's rows, which only reach
+      // result rows past m, never stored; reads past the last plane read zeros)
+      const uint32_t plane = (uint32_t)(M * kq * 2), tile = (uint32_t)(m0 * kq * 2);
+      const auto rah = buf_rsrc((const H*)ps.ahi + amat * 3 * M * kq, 3 * plane);
+      const auto ral = buf_rsrc((const H*)ps.alo + amat * 3 * M * kq, 3 * plane);
+      const int64_t jcol = nc0 + bj;
+      const uint32_t bco = jcol < g.n ? (uint32_t)((GATHER ? (int64_t)g.bcol[jcol] : jcol) * 8) : COL_OOB;
+      uint4 ra_h[3][A_Q], ra_l[3][A_Q];
+      float2 rbr[8];
+      auto load = [&](int64_t k0) {
+#pragma unroll
+        for (int p = 0; p < 3; ++p) {
+          const int so = (int)(p * plane + tile + (uint32_t)(k0 * 2));
+#pragma unroll
+          for (int u = 0; u < A_Q; ++u) {
+            ra_h[p][u] = __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(rah, a_off[u], so, 0));
+            ra_l[p][u] = __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(ral, a_off[u], so, 0));
+          }
+        }
+        uint32_t ro[8];
+        *reinterpret_cast<uint4*>(ro) = *reinterpret_cast<const uint4*>(tab + k0 + 8 * bgrp);
+        *reinterpret_cast<uint4*>(ro + 4) = *reinterpret_cast<const uint4*>(tab + k0 + 8 * bgrp + 4);
+#pragma unroll
+        for (int e = 0; e < 8; ++e)
+          rbr[e] = __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(rb, ro[e] + bco, 0, 0));
+      };
+      auto store = [&]() {
+#pragma unroll
+        for (int p = 0; p < 3; ++p)
+#pragma unroll
+          for (int u = 0; u < A_Q; ++u) {
+            const int q = tid + 256 * u;
+            *reinterpret_cast<uint4*>(ahi + p * PA + swz32(q / CH, q % CH)) = ra_h[p][u];
+            *reinterpret_cast<uint4*>(alo + p * PA + swz32(q / CH, q % CH)) = ra_l[p][u];
+          }
+        // B planes: re, im, re + im of 8 consecutive k of column bj
+        uint32_t h[3][4], l[3][4];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          const float r0 = rbr[2 * e].x * sb, i0 = rbr[2 * e].y * sb;
+          const float r1 = rbr[2 * e + 1].x * sb, i1 = rbr[2 * e + 1].y * sb;
+          split_pair<F16>(v2f{r0, r1}, h[0][e], l[0][e]);
+          split_pair<F16>(v2f{i0, i1}, h[1][e], l[1][e]);
+          split_pair<F16>(v2f{r0 + i0, r1 + i1}, h[2][e], l[2][e]);
+        }
+        const int o = swz32(bj, bgrp);
+#pragma unroll
+        for (int p = 0; p < 3; ++p) {
+          *reinterpret_cast<uint4*>(bhi + p * PB + o) = *reinterpret_cast<const uint4*>(h[p]);
+          *reinterpret_cast<uint4*>(blo + p * PB + o) = *reinterpret_cast<const uint4*>(l[p]);
+        }
+      };
+      const int64_t nk = kq / BK;
+      load(0);
+      store();
+      __syncthreads();
+      for (int64_t kt = 0; kt < nk; ++kt) {
+        if (kt + 1 < nk) load((kt + 1) * BK);
+#pragma unroll
+        for (int p = 0; p < 3; ++p) {
+          V ah[TM], al[TM], bh[TN], bl[TN];
+#pragma unroll
+          for (int i = 0; i < TM; ++i) {
+            ah[i] = *reinterpret_cast<const V*>(&ahi[p * PA + swz32(wm0 + i * 16 + cl, g4)]);
+            al[i] = *reinterpret_cast<const V*>(&alo[p * PA + swz32(wm0 + i * 16 + cl, g4)]);
+          }
+#pragma unroll
+          for (int j = 0; j < TN; ++j) {
+            bh[j] = *reinterpret_cast<const V*>(&bhi[p * PB + swz32(wn0 + j * 16 + cl, g4)]);
+            bl[j] = *reinterpret_cast<const V*>(&blo[p * PB + swz32(wn0 + j * 16 + cl, g4)]);
+          }
+          // the three products of one accumulator are TM * TN instructions apart
+#pragma unroll
+          for (int i = 0; i < TM; ++i)
+#pragma unroll
+            for (int j = 0; j < TN; ++j) acc[p][i][j] = Half<F16>::mfma(al[i], bh[j], acc[p][i][j]);
+#pragma unroll
+          for (int i = 0; i < TM; ++i)
+#pragma unroll
+            for (int j = 0; j < TN; ++j) acc[p][i][j] = Half<F16>::mfma(ah[i], bl[j], acc[p][i][j]);
+#pragma unroll
+          for (int i = 0; i < TM; ++i)
+#pragma unroll
+            for (int j = 0; j < TN; ++j) acc[p][i][j] = Half<F16>::mfma(ah[i], bh[j], acc[p][i][j]);
+        }
+        __syncthreads();
+        if (kt + 1 < nk) {
+          store();
+          __syncthreads();
+        }
+      }
+      // Epilogue through LDS (free after the loop's last barrier): (Cr, Ci) of
+      // row r, complex column c at float index r * 128 + 2 * (c ^ 16 * ((r >> 2) & 1)),
+      // so the two 16-lane groups of a 32-lane store half hit distinct banks;
+      // then 16-byte row pieces to C.
+      float* const cs = reinterpret_cast<float*>(smem_split);
+      {
+        const int xr = (g4 & 1) << 4;
+        float* const crow = cs + (wm0 + 4 * g4) * 128;
+#pragma unroll
+        for (int j = 0; j < TN; ++j) {
+          const int pc = 2 * ((wn0 + j * 16 + cl) ^ xr);
+#pragma unroll
+          for (int i = 0; i < TM; ++i)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+              const float t1 = acc[0][i][j][r], t2 = acc[1][i][j][r], t3 = acc[2][i][j][r];
+              *reinterpret_cast<float2*>(&crow[(i * 16 + r) * 128 + pc]) = make_float2(t1 - t2, t3 - t1 - t2);
+            }
+        }
+      }
+      __syncthreads();
+      const bool c16 = ((uintptr_t)C & 15) == 0 && (ldc & 3) == 0;
+      const int64_t n0 = 2 * nc0;  // first float column of the tile
+      constexpr int C_Q = BM * 128 / 4 / 256;  // 16-byte pieces per thread
+#pragma unroll 2
+      for (int u = 0; u < C_Q; ++u) {
+        const int q = tid + 256 * u, tr = q / 32, c4 = (q % 32) * 4;
+        const int pc = 2 * ((c4 / 2) ^ (((tr >> 2) & 1) << 4));
+        float4 v = *reinterpret_cast<const float4*>(&cs[tr * 128 + pc]);
         const int64_t row = m0 + tr, col = n0 + c4;
         if (row < M) {
           v.x *= unscale, v.y *= unscale, v.z *= unscale, v.w *= unscale;
@@ -1832,7 +2092,7 @@ static int native_mfma_kernel(int dt, int64_t m) {
 }
 
 static int auto_kernel(int dt, int prec, const GemmArgs& g) {
-  const bool split_prec = (prec == BF16X3 || prec == FP16X2) && (dt == F32 || dt == C64);
+  const bool split_prec = split_precision(prec) && (dt == F32 || dt == C64);
   if (split_prec && split_worth(dt, g)) {
     // Up to eight row tiles of A for complex operands, two for real ones (32 rows
     // for m <= 32, else 128): A split by a pass, B read in its own precision and
@@ -1894,6 +2154,24 @@ static void presplit_gemm(const GemmArgs& g, const PreSplit& ps, dim3 grid, hipS
   }
 }
 
+// Scratch of one half array of the 3M form's A planes (256-byte aligned).
+static size_t s3m_a_half(const GemmArgs& g) {
+  const int64_t kq = (g.k + S3M_BK - 1) / S3M_BK * S3M_BK;
+  return ((size_t)n_mats(g.sao, g.sai, g.nbo, g.nbi) * 3 * (size_t)g.m * (size_t)kq * 2 + 255) / 256 * 256;
+}
+
+template <bool F, bool G>
+static void split3m_launch(const GemmArgs& g, const PreSplit& ps, dim3 grid, hipStream_t s) {
+  static bool attr_set = false;
+  if (!attr_set) {
+    ATN_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&gemm_split3m_kernel<F, G>),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)s3m_lds(PRESPLIT_MAX_TABLE)));
+    attr_set = true;
+  }
+  if (ps.kp > PRESPLIT_MAX_TABLE) throw Error("3M split GEMM: k beyond the row table");
+  LAUNCH((gemm_split3m_kernel<F, G>), grid, dim3(256), s3m_lds(ps.kp), s, g, ps);
+}
+
 static int launch_gemm_kernels(int dt, int prec, int kernel, const GemmArgs& g0, hipStream_t s) {
   GemmArgs g = g0;
   if (kernel == GEMM_AUTO) kernel = auto_kernel(dt, prec, g);
@@ -1918,7 +2196,8 @@ static int launch_gemm_kernels(int dt, int prec, int kernel, const GemmArgs& g0,
     run_c3m(g, s);
     return GEMM_MFMA;
   }
-  const bool split_prec = (prec == BF16X3 || prec == FP16X2) && (dt == F32 || dt == C64);
+  const bool split_prec = split_precision(prec) && (dt == F32 || dt == C64);
+  const int hp = split_halves(prec);  // the half type (3M forms too)
   if ((kernel == GEMM_PRESPLIT || kernel == GEMM_PRESPLIT_A) && !split_prec) kernel = GEMM_MFMA;
   // split precision runs on GEMM_MFMA (split while staging) and GEMM_PRESPLIT;
   // the explicit row-tile kernels are the FP32 / FP64 MFMA kernels in any mode
@@ -1935,7 +2214,7 @@ static int launch_gemm_kernels(int dt, int prec, int kernel, const GemmArgs& g0,
     uint64_t tiles = (uint64_t)((g.m + bm - 1) / bm) * (uint64_t)((N + 127) / 128);
     dim3 grid((unsigned)std::min<uint64_t>(tiles, 1u << 20), zgrid(g));
     if (g.cmax && g.m <= bm) grid.x = std::min<unsigned>(grid.x, std::max<unsigned>(1, TRACK_WORKGROUPS / grid.y));
-    if (prec == FP16X2) {
+    if (hp == FP16X2) {
       if (!g.scratch) throw Error("FP16X2 GEMM needs 16 bytes of device scratch");
       // operand max magnitudes -> per-GEMM power-of-two scales (measured here
       // unless the caller supplies them)
@@ -1949,7 +2228,7 @@ static int launch_gemm_kernels(int dt, int prec, int kernel, const GemmArgs& g0,
     } else {
       g_last_kind = K_GEMM_MFMA_BF16X3;
     }
-    bool f16 = prec == FP16X2, cx = dt == C64;
+    bool f16 = hp == FP16X2, cx = dt == C64;
     if (pre) {  // operands split once into 16-bit (hi, lo) arrays, then the copy-only GEMM
       PreSplit ps;
       char* base = (char*)g.scratch + 256;
@@ -1977,7 +2256,29 @@ static int launch_gemm_kernels(int dt, int prec, int kernel, const GemmArgs& g0,
       if (raw_b) {
         // lanes along k where the gathered B has runs of consecutive k offsets
         // (its contracted legs innermost in memory) and short column runs
-        if (g.bcol && g.brow_run % (cx ? 4 : 8) == 0 && g.k % (cx ? 4 : 8) == 0 && g.bcol_run < 16)
+        const bool lanes_k = g.bcol && g.brow_run % (cx ? 4 : 8) == 0 && g.k % (cx ? 4 : 8) == 0 && g.bcol_run < 16;
+        if (split_3m(prec) && cx && bm == 128 && !lanes_k && g.scratch_bytes >= 256 + 2 * s3m_a_half(g) &&
+            3 * g.m * ((g.k + S3M_BK - 1) / S3M_BK * S3M_BK) * 2 < ((int64_t)1 << 31)) {
+          PreSplit p3;
+          const size_t ah = s3m_a_half(g);
+          p3.ahi = base, p3.alo = base + ah, p3.bhi = p3.blo = nullptr;
+          p3.kp = (g.k + S3M_BK - 1) / S3M_BK * S3M_BK;
+          const int64_t na3 = n_mats(g.sao, g.sai, g.nbo, g.nbi) * g.m * p3.kp / 4;
+          const dim3 ga3((unsigned)std::min<int64_t>((na3 + 255) / 256, 65536));
+          if (f16) {
+            LAUNCH((split_a3m_kernel<true>), ga3, dim3(256), 0, s, g, p3);
+            g_last_kind = K_GEMM_MFMA_FP16X2;
+            if (g.bcol) split3m_launch<true, true>(g, p3, grid, s);
+            else split3m_launch<true, false>(g, p3, grid, s);
+          } else {
+            LAUNCH((split_a3m_kernel<false>), ga3, dim3(256), 0, s, g, p3);
+            g_last_kind = K_GEMM_MFMA_BF16X3;
+            if (g.bcol) split3m_launch<false, true>(g, p3, grid, s);
+            else split3m_launch<false, false>(g, p3, grid, s);
+          }
+          return GEMM_PRESPLIT_A;
+        }
+        if (lanes_k)
           PRESPLIT_MODES(2);
         else PRESPLIT_MODES(1);
         return GEMM_PRESPLIT_A;
@@ -2045,7 +2346,7 @@ static int launch_gemm_kernels(int dt, int prec, int kernel, const GemmArgs& g0,
 bool split_products_apply(int dt, int prec, int64_t m, int64_t k, int64_t n) {
   GemmArgs g{};
   g.m = m, g.k = k, g.n = n;
-  return (prec == BF16X3 || prec == FP16X2) && (dt == F32 || dt == C64) && split_worth(dt, g);
+  return split_precision(prec) && (dt == F32 || dt == C64) && split_worth(dt, g);
 }
 
 bool gemm_runs_simple(int dt, int prec, int kernel, const GemmArgs& g) {
@@ -2061,10 +2362,12 @@ size_t gemm_scratch_bytes(int dt, int prec, int kernel, const GemmArgs& g) {
     ks = plan_mfma_split(dt, prec == C3M ? NATIVE : prec, g, MAX_PARTIAL_BYTES,
                          prec == C3M ? (g.m <= 64 ? 64 : 128) : mfma_rows_for(dt, kernel, g.m));
   size_t part = ks.n_split > 1 ? (size_t)ks.n_split * dtype_size(dt) * (size_t)(g.nbo * g.nbi * g.m * g.n) : 0;
-  if ((prec == BF16X3 || prec == FP16X2) && (dt == F32 || dt == C64)) {
+  if (split_precision(prec) && (dt == F32 || dt == C64)) {
     if (kernel == GEMM_PRESPLIT) part = presplit_sizes(dt, g).total;
-    if (kernel == GEMM_PRESPLIT_A)
+    if (kernel == GEMM_PRESPLIT_A) {
       part = raw_b_fits(dt, g) ? 2 * presplit_sizes(dt, g).a_half : presplit_sizes(dt, g).total;
+      if (split_3m(prec) && dt == C64 && raw_b_fits(dt, g)) part = std::max(part, 2 * s3m_a_half(g));
+    }
   }
   return 256 + part;
 }

@@ -106,7 +106,7 @@ def _args(argv):
     ap.add_argument("--seed", type=int, default=0, help="seed of the random inputs when --data is absent")
     ap.add_argument("--single", action="store_true", help="f32 / c32 instead of f64 / c64")
     ap.add_argument("--complex", action="store_true", help="complex scalars")
-    ap.add_argument("--precision", default="native", choices=["native", "fp32", "bf16x3", "fp16x2"])
+    ap.add_argument("--precision", default="native", choices=["native", "fp32", "c3m", "bf16x3", "fp16x2", "bf16x3_3m", "fp16x2_3m"])
     ap.add_argument("--plain", action="store_true", help="per-slice executor instead of the batched one")
     ap.add_argument("--graph", action="store_true", help="batched form captured in a hipGraph")
     ap.add_argument("--autotune", action="store_true")

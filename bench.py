@@ -175,7 +175,8 @@ def main():
         log(f"[bench] reference amplitude {ref:.9e} ({ref_kind})")
 
     # Form: the fastest precision variant (optionally with autotune) whose error meets 1e-5.
-    precs = ["native", "c3m", "bf16x3", "fp16x2"] if args.precision == "auto" else [args.precision]
+    precs = (["native", "c3m", "bf16x3", "fp16x2", "bf16x3_3m", "fp16x2_3m"] if args.precision == "auto"
+             else [args.precision])
     tunes = [False, True] if args.autotune else [False]
     cands = [(p, t) for p in precs for t in tunes]
     trial = {}
@@ -592,7 +593,8 @@ def bench_gemm_steps(args):
     """The split-precision GEMM shapes of the 30-qubit circuit's contraction (complex64,
     64 slices per batched launch, A shared by the batch where the plan shares it,
     row-major operands) on each applicable atn_gemm kernel: 1 split while staging,
-    4 pre-split operands, 8 A pre-split and B split while staging; FP16X2."""
+    4 pre-split operands, 8 A pre-split and B split while staging; FP16X2 (precision 3)
+    and FP16X2 with 3M complex products (precision 5, kernel 8)."""
     import torch
 
     from arctn_amd import _lib
@@ -607,8 +609,8 @@ def bench_gemm_steps(args):
         a = torch.randn(1 if shared else bt, m, k, dtype=torch.complex64, device="cuda")
         b = torch.randn(bt, k, n, dtype=torch.complex64, device="cuda")
         c = torch.empty(bt, m, n, dtype=torch.complex64, device="cuda")
-        for kern in (1, 4, 8):
-            call = lambda: _lib.check(L.atn_gemm(2, 3, kern, bt, m, k, n, a.data_ptr(), 0 if shared else m * k,
+        for kern, prec in ((1, 3), (4, 3), (8, 3), (8, 5)):
+            call = lambda: _lib.check(L.atn_gemm(2, prec, kern, bt, m, k, n, a.data_ptr(), 0 if shared else m * k,
                                                  b.data_ptr(), k * n, c.data_ptr(), m * n, s.cuda_stream))
             call()
             e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
@@ -620,9 +622,9 @@ def bench_gemm_steps(args):
             torch.cuda.synchronize()
             ms = e0.elapsed_time(e1) / args.steps
             tf = 8.0 * bt * m * k * n / (ms * 1e-3) / 1e12
-            res.append({"m": m, "k": k, "n": n, "batch": bt, "a_shared": shared, "kernel": kern, "ms": ms,
-                        "algorithmic_tflops": tf})
-            log(f"[bench] m={m} k={k} n={n} x{bt} kernel {kern}: {ms:.3f} ms, {tf:.1f} TFLOP/s")
+            res.append({"m": m, "k": k, "n": n, "batch": bt, "a_shared": shared, "kernel": kern, "precision": prec,
+                        "ms": ms, "algorithmic_tflops": tf})
+            log(f"[bench] m={m} k={k} n={n} x{bt} kernel {kern} precision {prec}: {ms:.3f} ms, {tf:.1f} TFLOP/s")
         del a, b, c
         torch.cuda.empty_cache()
     print(json.dumps({"metric": "split-precision GEMM steps (FP16X2, complex64)", "cases": res}), flush=True)

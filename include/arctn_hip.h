@@ -44,8 +44,16 @@ typedef enum { ATN_F32 = 0, ATN_F64 = 1, ATN_C64 = 2, ATN_C128 = 3 } atn_dtype;
  *           2x2 real expansion of B), FP64 MFMA for f64/complex128.
  *   C3M:    complex 3M (Gauss) on FP32 MFMA; complex64 only.
  *   BF16X3: complex64/f32 products as hi*hi + hi*lo + lo*hi in BF16 MFMA, FP32 accumulate.
- *   FP16X2: same with FP16 hi/lo halves and per-step power-of-two operand rescaling. */
-typedef enum { ATN_PREC_NATIVE = 0, ATN_PREC_C3M = 1, ATN_PREC_BF16X3 = 2, ATN_PREC_FP16X2 = 3 } atn_precision;
+ *   FP16X2: same with FP16 hi/lo halves and per-step power-of-two operand rescaling.
+ *   BF16X3_3M / FP16X2_3M: BF16X3 / FP16X2 whose complex64 products run as
+ *           Gauss's three real products (9 instead of 12 16-bit products per
+ *           complex block) where the 3M split kernel applies (A split by a pass,
+ *           B read in complex64 with lanes along columns, 128-row tiles); other
+ *           steps, and float32 plans, run as BF16X3 / FP16X2. */
+typedef enum {
+  ATN_PREC_NATIVE = 0, ATN_PREC_C3M = 1, ATN_PREC_BF16X3 = 2, ATN_PREC_FP16X2 = 3,
+  ATN_PREC_BF16X3_3M = 4, ATN_PREC_FP16X2_3M = 5
+} atn_precision;
 
 /* Network + plan, as flat arrays (the fields of the execution-plan v2 artifact,
  * src/execution_plan.rs:39-43). Legs are u32 ids (src/network.rs:13). */

@@ -337,7 +337,7 @@ def test_slice_ranges_partial_sums(cuda):
     np.testing.assert_allclose(parts, whole.cpu().numpy(), rtol=1e-12, atol=1e-12)
 
 
-@pytest.mark.parametrize("precision", ["native", "c3m", "bf16x3", "fp16x2"])
+@pytest.mark.parametrize("precision", ["native", "c3m", "bf16x3", "fp16x2", "bf16x3_3m", "fp16x2_3m"])
 def test_autotuned_batched_graph_forms(cuda, precision):
     """The autotuner's per-step kernel / tile choices keep the result within 1e-5."""
     net, arrays, _ = W.random_circuit_amplitude(4, 4, 8, 0)
@@ -380,7 +380,7 @@ def test_gather_tables_match_permuted_operands(cuda, dtype):
     want = complex(O.contract(net, [a.astype(np.complex128) for a in arrays], path, "compiled"))
     plan = Plan(net["inputs"], net["output"], net["size_dict"], path, sliced)
     dev = [torch.from_numpy(a).cuda() for a in arrays]
-    for prec in ("native", "c3m", "bf16x3", "fp16x2"):
+    for prec in ("native", "c3m", "bf16x3", "fp16x2", "bf16x3_3m", "fp16x2_3m"):
         for gather in (True, False):
             got, _ = plan.run(dev, 2, batched=True, use_graph=True, precision=prec, gather=gather)
             assert abs(complex(got.cpu()) - want) / abs(want) < 1e-5, (prec, gather)
@@ -411,7 +411,7 @@ def test_forced_gemm_kernels_with_gather_tables(cuda, dtype):
         # the circuit (the heuristic keeps it to large steps) its error adds up
         forms = [("native", (0, 1, 2, 3, 5, 6, 7, 8), 1e-5), ("bf16x3", (1, 4, 8), 3e-4), ("fp16x2", (1, 4, 8), 1e-5)]
         if dtype == np.complex64:
-            forms.append(("c3m", (1, 5), 1e-5))
+            forms += [("c3m", (1, 5), 1e-5), ("bf16x3_3m", (8,), 3e-4), ("fp16x2_3m", (8,), 1e-5)]
         for prec, kernels, tol in forms:
             for kern in kernels:
                 for gather in (True, False):

@@ -96,7 +96,11 @@ GEMM_CASES = ([(dt, 0, k, U32) for dt in (np.float32, np.complex64) for k in NAT
               + [(dt, 0, k, U64) for dt in (np.float64, np.complex128) for k in NATIVE_KERNELS if k != 7]
               + [(dt, 2, k, U_BF16X3) for dt in (np.float32, np.complex64) for k in (1, 4, 8)]
               + [(dt, 3, k, U_FP16X2) for dt in (np.float32, np.complex64) for k in (1, 4, 8)]
-              + [(np.complex64, 1, 1, U32)])
+              + [(np.complex64, 1, 1, U32)]
+              # split precisions with 3M complex products (kernel 8 with m > 32 runs the
+              # 3M split kernel; the other kernels and float32 run the 2x2 form)
+              + [(dt, p, k, u) for p, u in ((4, U_BF16X3), (5, U_FP16X2)) for dt in (np.float32, np.complex64)
+                 for k in ((1, 4, 8) if dt == np.complex64 else (8,))])
 
 
 def _magnitude_bound(a, b):
@@ -144,7 +148,7 @@ def test_gemm_broadcast_batch(cuda, dtype):
     a = _rand((m, k), dtype, rng)
     b = _rand((bt, k, n), dtype, rng)
     want = np.matmul(a.astype(np.complex128)[None], b.astype(np.complex128))
-    for kernel, prec in ((0, 0), (1, 0), (3, 0), (1, 2), (1, 3), (4, 2), (4, 3), (8, 2), (8, 3)):
+    for kernel, prec in ((0, 0), (1, 0), (3, 0), (1, 2), (1, 3), (4, 2), (4, 3), (8, 2), (8, 3), (8, 4), (8, 5)):
         ta, tb = torch.from_numpy(a).cuda(), torch.from_numpy(b).cuda()
         tc = torch.empty((bt, m, n), dtype=ta.dtype, device="cuda")
         _lib.check(_lib.lib().atn_gemm(code, prec, kernel, bt, m, k, n, ta.data_ptr(), 0, tb.data_ptr(), k * n,

@@ -46,7 +46,8 @@ def _circuit(name):
 
 
 # (precision, forced GEMM kernel or None, bound on the relative error)
-B_FORMS = [("native", None, 1e-5), ("fp16x2", None, 1e-5), ("c3m", None, 1e-5), ("bf16x3", None, 1e-4)]
+B_FORMS = [("native", None, 1e-5), ("fp16x2", None, 1e-5), ("c3m", None, 1e-5), ("bf16x3", None, 1e-4),
+           ("fp16x2_3m", None, 1e-5), ("bf16x3_3m", None, 1e-4)]
 
 
 @pytest.mark.parametrize("form", range(len(B_FORMS)))
@@ -85,7 +86,8 @@ def test_circuit_5x6_slice_range_complex128_vs_oracle(cuda):
 # up over the ~700 steps of a slice (4.4e-4 on block 63), far above the 1e-5 bar,
 # which is why BF16X3 is never the selected form; FP16X2 forced everywhere stays within it
 C_FORMS = [("native", None, 1e-5), ("c3m", None, 1e-5), ("fp16x2", None, 1e-5), ("fp16x2", 4, 1e-5),
-           ("fp16x2", 8, 1e-5), ("bf16x3", None, 1e-4), ("bf16x3", 4, 1e-3), ("bf16x3", 8, 1e-3)]
+           ("fp16x2", 8, 1e-5), ("bf16x3", None, 1e-4), ("bf16x3", 4, 1e-3), ("bf16x3", 8, 1e-3),
+           ("fp16x2_3m", 8, 1e-5), ("bf16x3_3m", 8, 1e-3)]
 
 
 @pytest.mark.parametrize("form", range(len(C_FORMS)))
