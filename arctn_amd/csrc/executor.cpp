@@ -141,8 +141,8 @@ void Executor::gen_inputs(bool body, size_t nb, std::vector<Ref>& slots, std::ve
   }
 }
 
-bool Executor::add_gather(const std::vector<size_t>& shape, const std::vector<size_t>& perm, size_t n_con, int64_t& row,
-                          int64_t& col, int32_t& col_run, int32_t& row_run) {
+bool gather_tables(const std::vector<size_t>& shape, const std::vector<size_t>& perm, size_t n_con,
+                   std::vector<int32_t>& rows, std::vector<int32_t>& cols, int32_t& col_run, int32_t& row_run) {
   const size_t r = shape.size();
   std::vector<int64_t> st(r, 1);
   for (size_t a = r; a-- > 1;) st[a - 1] = st[a] * (int64_t)shape[a];
@@ -162,11 +162,6 @@ bool Executor::add_gather(const std::vector<size_t>& shape, const std::vector<si
       out[e] = (int32_t)off;
     }
   };
-  std::vector<int32_t> rows, cols;
-  size_t nrow = 1, ncol = 1;
-  for (size_t d = 0; d < n_con; ++d) nrow *= shape[perm[d]];
-  for (size_t d = n_con; d < r; ++d) ncol *= shape[perm[d]];
-  if (const_host_.size() + nrow + ncol > ((size_t)64 << 20)) return false;  // 256 MB of tables at most
   table(0, n_con, rows);
   table(n_con, r, cols);
   // innermost free output axes that are also the innermost input axes: runs of
@@ -182,6 +177,17 @@ bool Executor::add_gather(const std::vector<size_t>& shape, const std::vector<si
     if (perm[d] != a) break;
     row_run *= (int32_t)shape[a];
   }
+  return true;
+}
+
+bool Executor::add_gather(const std::vector<size_t>& shape, const std::vector<size_t>& perm, size_t n_con, int64_t& row,
+                          int64_t& col, int32_t& col_run, int32_t& row_run) {
+  size_t nrow = 1, ncol = 1;
+  for (size_t d = 0; d < n_con; ++d) nrow *= shape[perm[d]];
+  for (size_t d = n_con; d < shape.size(); ++d) ncol *= shape[perm[d]];
+  if (const_host_.size() + nrow + ncol > ((size_t)64 << 20)) return false;  // 256 MB of tables at most
+  std::vector<int32_t> rows, cols;
+  if (!gather_tables(shape, perm, n_con, rows, cols, col_run, row_run)) return false;
   row = (int64_t)const_host_.size();
   const_host_.insert(const_host_.end(), rows.begin(), rows.end());
   col = (int64_t)const_host_.size();

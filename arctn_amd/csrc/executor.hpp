@@ -68,6 +68,14 @@ struct KernelStat {
   double ms = 0, flops = 0, bytes = 0;
 };
 
+// Gather tables of the [con][free] view of an operand of `shape` permuted by
+// `perm` (con = the first n_con output axes): rows[p] and cols[j] are element
+// offsets into the unpermuted operand, col_run / row_run the lengths of runs of
+// consecutive offsets (innermost output axes that stay innermost in memory).
+// false when the operand's offsets do not fit int32.
+bool gather_tables(const std::vector<size_t>& shape, const std::vector<size_t>& perm, size_t n_con,
+                   std::vector<int32_t>& rows, std::vector<int32_t>& cols, int32_t& col_run, int32_t& row_run);
+
 class Executor {
  public:
   Executor(std::shared_ptr<const Plan> plan, const ExecOptions& o);

@@ -261,6 +261,9 @@ struct Single {  // an op launched on its own
   bool final = false;  // STRIP: the plan's last step (always normalise)
   PermPlan perm;
   GemmArgs gemm{};
+  // GEMM: B read through gather tables instead of a permute (offsets into the
+  // plan's table vector, then byte offsets into the group's descriptor blob), or -1
+  int64_t brow = -1, bcol = -1;
   size_t A = 0, d = 0, Bm = 0, C = 0;  // SUM: [A, d, C]; DIAG/TRACE: [A, d, Bm, d, C]; COPY: A bytes
 };
 
@@ -281,6 +284,7 @@ struct Buf {
 struct PlanProg {
   std::vector<Round> rounds;
   size_t arena = 0;  // bytes
+  std::vector<int32_t> tables;  // gather tables of the B operands of large GEMMs
 };
 
 // Output-order (dim, input stride) axes of permuting `shape` by `perm`, with
@@ -389,12 +393,42 @@ PlanProg build_plan(const Plan& P, int dt, bool strip, size_t n_rounds) {
       for (size_t ax : sums) cur = sum_axis(R, t, cur, shape, ax);
       return permute(R, t, cur, shape, perm, nullptr);
     };
-    GRef a = operand(st.li, st.sum_a, st.a_perm);
-    GRef b = operand(st.ri, st.sum_b, st.b_perm);
     const uint64_t cnum = (uint64_t)st.bt * st.m * st.n;
+    const bool large = cnum >= LARGE_NUMEL || cnum * st.k >= LARGE_MACS || (int64_t)st.k >= LARGE_K;
+    // A large GEMM reads a B that needs only a permutation (no batch or summed
+    // legs) through gather tables, with its contracted legs taken in B's memory
+    // order (A's contracted legs follow; the result layout does not change)
+    std::vector<size_t> a_perm = st.a_perm, b_perm = st.b_perm;
+    std::vector<int32_t> rows, cols;
+    int32_t col_run = 1, row_run = 1;
+    bool gather_b = large && st.bt == 1 && st.sum_b.empty() && plan_permute(shapes[st.ri], b_perm, 1, es).kind != 0;
+    if (gather_b) {
+      size_t n_con = 0;
+      for (size_t d = 0; d < b_perm.size(); ++d) {
+        const Leg l = st.right_legs[b_perm[d]];
+        if (std::count(st.left_legs.begin(), st.left_legs.end(), l) &&
+            !std::count(st.result_legs.begin(), st.result_legs.end(), l))
+          n_con = d + 1;
+      }
+      if (n_con > 1) {
+        std::vector<size_t> bc(b_perm.begin(), b_perm.begin() + (long)n_con);
+        std::sort(bc.begin(), bc.end());
+        std::vector<Leg> la = st.left_legs;  // A's legs once its summed legs are gone
+        for (size_t ax : st.sum_a) la.erase(la.begin() + (long)ax);
+        const size_t nfa = a_perm.size() - n_con;
+        for (size_t q = 0; q < n_con; ++q) {
+          a_perm[nfa + q] = (size_t)(std::find(la.begin(), la.end(), st.right_legs[bc[q]]) - la.begin());
+          b_perm[q] = bc[q];
+        }
+      }
+      gather_b = gather_tables(shapes[st.ri], b_perm, n_con, rows, cols, col_run, row_run);
+      if (!gather_b) a_perm = st.a_perm, b_perm = st.b_perm;
+    }
+    GRef a = operand(st.li, st.sum_a, a_perm);
+    GRef b = gather_b ? slot[st.ri] : operand(st.ri, st.sum_b, b_perm);
     GRef c = newbuf(cnum * es, t);
     use(a, t), use(b, t);
-    if (cnum >= LARGE_NUMEL || cnum * st.k >= LARGE_MACS || (int64_t)st.k >= LARGE_K) {
+    if (large) {
       Single sg;
       sg.kind = Single::GEMM;
       sg.in = a, sg.in2 = b, sg.out = c;
@@ -403,6 +437,13 @@ PlanProg build_plan(const Plan& P, int dt, bool strip, size_t n_rounds) {
       g.lda = g.k, g.ldb = g.n, g.ldc = g.n;
       g.nbo = 1, g.nbi = (int64_t)st.bt;
       g.sai = g.m * g.k, g.sbi = g.k * g.n, g.sci = g.m * g.n;
+      if (gather_b) {
+        sg.brow = (int64_t)prog.tables.size();
+        prog.tables.insert(prog.tables.end(), rows.begin(), rows.end());
+        sg.bcol = (int64_t)prog.tables.size();
+        prog.tables.insert(prog.tables.end(), cols.begin(), cols.end());
+        g.brow_run = row_run, g.bcol_run = col_run;
+      }
       R.mid.push_back(sg);
     } else {
       R.gemms.push_back(
@@ -536,6 +577,11 @@ GroupedExecutor::GroupedExecutor(std::vector<std::shared_ptr<const Plan>> plans,
         if (src && bytes) std::memcpy(G.blob.data() + o, src, bytes);
         return o;
       };
+      std::vector<size_t> tab_at(ng);  // byte offset of each plan's gather tables in the blob
+      for (size_t i = 0; i < ng; ++i) {
+        const auto& tb = progs[G.g0 + i].tables;
+        tab_at[i] = tb.empty() ? 0 : append(tb.data(), tb.size() * sizeof(int32_t));
+      }
       G.launches.resize(n_rounds);
       for (size_t r = 0; r < n_rounds; ++r) {
         std::vector<GPermDesc> perms;
@@ -561,6 +607,7 @@ GroupedExecutor::GroupedExecutor(std::vector<std::shared_ptr<const Plan>> plans,
           for (Single sg : R.pre) sg.in = fix(sg.in), sg.out = fix(sg.out), L.pre.push_back(sg);
           for (Single sg : R.mid) {
             sg.in = fix(sg.in), sg.in2 = fix(sg.in2), sg.out = fix(sg.out), sg.plan = (uint32_t)i;
+            if (sg.brow >= 0) sg.brow = (int64_t)(tab_at[i] + sg.brow * 4), sg.bcol = (int64_t)(tab_at[i] + sg.bcol * 4);
             L.mid.push_back(sg);
           }
           for (Single sg : R.big_strips) sg.in = fix(sg.in), sg.plan = (uint32_t)i, L.big_strips.push_back(sg);
@@ -662,6 +709,8 @@ void GroupedExecutor::run(const std::vector<const void* const*>& inputs, const s
             g.scratch = dev + G.gemm_scratch_off;
             g.scratch_bytes = G.gemm_scratch;
             if (sg.fold) g.cmax = slots + sg.plan;
+            if (sg.brow >= 0)
+              g.brow = reinterpret_cast<const int32_t*>(dev + sg.brow), g.bcol = reinterpret_cast<const int32_t*>(dev + sg.bcol);
             launch_gemm(dt, prec_, GEMM_AUTO, g, s);
             break;
           }
