@@ -1096,11 +1096,22 @@ __global__ __launch_bounds__(256) void split_a_kernel(GemmArgs g, PreSplit ps) {
   for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
     const int64_t mat = e / (g.m * per_row), r = e - mat * g.m * per_row, row = r / per_row, c = (r - row * per_row) * 4;
     const int64_t zo = mat / inner, zi = mat - zo * inner;
-    const float* src = (const float*)g.a + (zo * g.sao + zi * g.sai) * (CPLX ? 2 : 1) + row * lda;
+    const float* base = (const float*)g.a + (zo * g.sao + zi * g.sai) * (CPLX ? 2 : 1);
+    const float* src = base + row * lda;
     H h[4], l[4];
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
-      float x = c + u < K ? src[c + u] * sa : 0.f;
+      float x = 0.f;
+      if (c + u < K) {
+        if (g.acol) {  // A through gather tables (one batch entry): element (row, p) at arow[row] + acol[p]
+          const int64_t p = CPLX ? (c + u) >> 1 : c + u;
+          const int64_t off = (int64_t)g.arow[row] + (int64_t)g.acol[p];
+          x = CPLX ? base[2 * off + ((c + u) & 1)] : base[off];
+        } else {
+          x = src[c + u];
+        }
+        x *= sa;
+      }
       split16<F16>(x, h[u], l[u]);
     }
     const int64_t off = (mat * g.m + row) * kp + c;
@@ -1573,11 +1584,14 @@ __global__ __launch_bounds__(256) void split_a3m_kernel(GemmArgs g, PreSplit ps)
   for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
     const int64_t mat = e / (g.m * per_row), r = e - mat * g.m * per_row, row = r / per_row, c = (r - row * per_row) * 4;
     const int64_t zo = mat / inner, zi = mat - zo * inner;
-    const float2* src = (const float2*)g.a + zo * g.sao + zi * g.sai + row * g.lda;
+    const float2* base = (const float2*)g.a + zo * g.sao + zi * g.sai;
     H h[3][4], l[3][4];
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
-      const float2 x = c + u < g.k ? src[c + u] : make_float2(0.f, 0.f);
+      // A through gather tables (one batch entry): element (row, p) at arow[row] + acol[p]
+      const float2 x = c + u >= g.k ? make_float2(0.f, 0.f)
+                       : g.acol     ? base[(int64_t)g.arow[row] + (int64_t)g.acol[c + u]]
+                                    : base[row * g.lda + c + u];
       const float re = x.x * sa, im = x.y * sa;
       split16<F16>(re, h[0][u], l[0][u]);
       split16<F16>(im, h[1][u], l[1][u]);
@@ -2176,7 +2190,12 @@ static int launch_gemm_kernels(int dt, int prec, int kernel, const GemmArgs& g0,
   GemmArgs g = g0;
   if (kernel == GEMM_AUTO) kernel = auto_kernel(dt, prec, g);
   if (g.k == 0) kernel = GEMM_SIMPLE;  // empty sums: every kernel but this one assumes k tiles
-  if (g.acol) kernel = GEMM_SIMPLE;    // A through gather tables: only this kernel reads them
+  // A through gather tables: read by the simple kernel and by the split passes of
+  // the pre-split kernels (one batch entry); any other choice runs the simple kernel
+  if (g.acol && !(split_precision(prec) && (dt == F32 || dt == C64) && g.nbo * g.nbi == 1 &&
+                  (kernel == GEMM_PRESPLIT || kernel == GEMM_PRESPLIT_A) && g.scratch &&
+                  g.scratch_bytes >= gemm_scratch_bytes(dt, prec, kernel, g)))
+    kernel = GEMM_SIMPLE;
   if (kernel == GEMM_SPLITK) {
     run_splitk(dt, g, s);
     return GEMM_SPLITK;

@@ -264,6 +264,13 @@ struct Single {  // an op launched on its own
   // GEMM: B read through gather tables instead of a permute (offsets into the
   // plan's table vector, then byte offsets into the group's descriptor blob), or -1
   int64_t brow = -1, bcol = -1;
+  // GEMM: A read through gather tables by the split pass of a split-precision
+  // product instead of a permute (the same offsets), or -1
+  int64_t arow = -1, acol = -1;
+  // GEMM: operand max slots (FP16 scales) of A / B in the plan's table of result
+  // maxima, or -1 (measured by a pass); GEMM without strip, STRIP: the slot that
+  // receives this result's max, or -1
+  int amax = -1, bmax = -1, max_out = -1;
   size_t A = 0, d = 0, Bm = 0, C = 0;  // SUM: [A, d, C]; DIAG/TRACE: [A, d, Bm, d, C]; COPY: A bytes
 };
 
@@ -285,6 +292,7 @@ struct PlanProg {
   std::vector<Round> rounds;
   size_t arena = 0;  // bytes
   std::vector<int32_t> tables;  // gather tables of the B operands of large GEMMs
+  int n_max = 0;                // result-max slots (large results read by later large GEMMs)
 };
 
 // Output-order (dim, input stride) axes of permuting `shape` by `perm`, with
@@ -306,7 +314,7 @@ void merged_axes(const std::vector<size_t>& shape, const std::vector<size_t>& pe
 }
 
 // Plans one plan's ops into `n_rounds` rounds: 0 preps, 1 + s step s, the last finalize.
-PlanProg build_plan(const Plan& P, int dt, bool strip, size_t n_rounds) {
+PlanProg build_plan(const Plan& P, int dt, bool strip, int prec, size_t n_rounds) {
   const int es = (int)dtype_size(dt);
   const Compiled& C = P.cp;
   PlanProg prog;
@@ -359,6 +367,9 @@ PlanProg build_plan(const Plan& P, int dt, bool strip, size_t n_rounds) {
   };
 
   std::vector<GRef> slot(C.n_inputs + C.steps.size());
+  // result-max slot of each plan slot produced by a large GEMM (stripped runs:
+  // the power-of-two rescale writes the max after it), or -1
+  std::vector<int> res_max(slot.size(), -1);
   std::vector<std::vector<size_t>> shapes(slot.size());
   for (size_t i = 0; i < C.n_inputs; ++i) {  // round 0: traces / diagonals (src/compiled.rs:401-404)
     GRef cur{INPUT0 + (uint32_t)i, 0};
@@ -399,17 +410,17 @@ PlanProg build_plan(const Plan& P, int dt, bool strip, size_t n_rounds) {
     // legs) through gather tables, with its contracted legs taken in B's memory
     // order (A's contracted legs follow; the result layout does not change)
     std::vector<size_t> a_perm = st.a_perm, b_perm = st.b_perm;
+    size_t n_con = 0;  // with no batch legs: the leading entries of b_perm, the trailing ones of a_perm
+    for (size_t d = 0; d < b_perm.size(); ++d) {
+      const Leg l = st.right_legs[b_perm[d]];
+      if (std::count(st.left_legs.begin(), st.left_legs.end(), l) &&
+          !std::count(st.result_legs.begin(), st.result_legs.end(), l))
+        n_con = d + 1;
+    }
     std::vector<int32_t> rows, cols;
     int32_t col_run = 1, row_run = 1;
     bool gather_b = large && st.bt == 1 && st.sum_b.empty() && plan_permute(shapes[st.ri], b_perm, 1, es).kind != 0;
     if (gather_b) {
-      size_t n_con = 0;
-      for (size_t d = 0; d < b_perm.size(); ++d) {
-        const Leg l = st.right_legs[b_perm[d]];
-        if (std::count(st.left_legs.begin(), st.left_legs.end(), l) &&
-            !std::count(st.result_legs.begin(), st.result_legs.end(), l))
-          n_con = d + 1;
-      }
       if (n_con > 1) {
         std::vector<size_t> bc(b_perm.begin(), b_perm.begin() + (long)n_con);
         std::sort(bc.begin(), bc.end());
@@ -424,7 +435,15 @@ PlanProg build_plan(const Plan& P, int dt, bool strip, size_t n_rounds) {
       gather_b = gather_tables(shapes[st.ri], b_perm, n_con, rows, cols, col_run, row_run);
       if (!gather_b) a_perm = st.a_perm, b_perm = st.b_perm;
     }
-    GRef a = operand(st.li, st.sum_a, a_perm);
+    // A of a large split-precision product: read through gather tables by the
+    // split pass, which then does the permute's work
+    std::vector<int32_t> arows, acols;
+    int32_t arun = 1, crun = 1;
+    const bool gather_a = large && st.bt == 1 && st.sum_a.empty() &&
+                          split_products_apply(dt, prec, (int64_t)st.m, (int64_t)st.k, (int64_t)st.n) &&
+                          plan_permute(shapes[st.li], a_perm, 1, es).kind != 0 &&
+                          gather_tables(shapes[st.li], a_perm, a_perm.size() - n_con, arows, acols, crun, arun);
+    GRef a = gather_a ? slot[st.li] : operand(st.li, st.sum_a, a_perm);
     GRef b = gather_b ? slot[st.ri] : operand(st.ri, st.sum_b, b_perm);
     GRef c = newbuf(cnum * es, t);
     use(a, t), use(b, t);
@@ -437,6 +456,15 @@ PlanProg build_plan(const Plan& P, int dt, bool strip, size_t n_rounds) {
       g.lda = g.k, g.ldb = g.n, g.ldc = g.n;
       g.nbo = 1, g.nbi = (int64_t)st.bt;
       g.sai = g.m * g.k, g.sbi = g.k * g.n, g.sci = g.m * g.n;
+      if (gather_a) {
+        sg.arow = (int64_t)prog.tables.size();
+        prog.tables.insert(prog.tables.end(), arows.begin(), arows.end());
+        sg.acol = (int64_t)prog.tables.size();
+        prog.tables.insert(prog.tables.end(), acols.begin(), acols.end());
+      }
+      // operand maxima of earlier large results (a permute or a gather keeps the max)
+      if (st.sum_a.empty()) sg.amax = res_max[st.li];
+      if (st.sum_b.empty()) sg.bmax = res_max[st.ri];
       if (gather_b) {
         sg.brow = (int64_t)prog.tables.size();
         prog.tables.insert(prog.tables.end(), rows.begin(), rows.end());
@@ -455,6 +483,7 @@ PlanProg build_plan(const Plan& P, int dt, bool strip, size_t n_rounds) {
       Single sg;
       sg.kind = Single::STRIP;
       sg.in = c, sg.A = cnum, sg.final = last;
+      if (!last) sg.max_out = res_max[st.out] = prog.n_max++;
       R.big_strips.push_back(sg);
     } else if (strip) {
       R.strips.push_back(GStripDesc{c, cnum, 0, 0, last ? 1u : 0u});
@@ -520,6 +549,7 @@ struct GroupedExecutor::Group {
   std::vector<Launch> launches;
   std::vector<char> blob;  // descriptors, strip slots and exponents, GEMM scratch, base table
   size_t slots_off = 0, expo_off = 0, bases_off = 0, gemm_scratch_off = 0, gemm_scratch = 256;
+  size_t maxtab_off = 0;  // result maxima (double bits) read as FP16 operand scales
   std::vector<size_t> in_base;
   size_t out_base = 0, nb = 0, arena = 0;
   char* dev = nullptr;
@@ -538,7 +568,7 @@ GroupedExecutor::GroupedExecutor(std::vector<std::shared_ptr<const Plan>> plans,
   }
   const size_t n_rounds = max_steps + 2;
   std::vector<PlanProg> progs(np);
-  for (size_t p = 0; p < np; ++p) progs[p] = build_plan(*plans_[p], dt, strip, n_rounds);
+  for (size_t p = 0; p < np; ++p) progs[p] = build_plan(*plans_[p], dt, strip, prec, n_rounds);
 
   size_t free_b = 0, total_b = 0;
   ATN_HIP(hipMemGetInfo(&free_b, &total_b));
@@ -577,6 +607,10 @@ GroupedExecutor::GroupedExecutor(std::vector<std::shared_ptr<const Plan>> plans,
         if (src && bytes) std::memcpy(G.blob.data() + o, src, bytes);
         return o;
       };
+      std::vector<int> max_at(ng);  // first result-max slot of each plan in the group's table
+      int n_max = 0;
+      for (size_t i = 0; i < ng; ++i) max_at[i] = n_max, n_max += progs[G.g0 + i].n_max;
+      G.maxtab_off = append(nullptr, (size_t)std::max(n_max, 1) * 8);
       std::vector<size_t> tab_at(ng);  // byte offset of each plan's gather tables in the blob
       for (size_t i = 0; i < ng; ++i) {
         const auto& tb = progs[G.g0 + i].tables;
@@ -608,9 +642,16 @@ GroupedExecutor::GroupedExecutor(std::vector<std::shared_ptr<const Plan>> plans,
           for (Single sg : R.mid) {
             sg.in = fix(sg.in), sg.in2 = fix(sg.in2), sg.out = fix(sg.out), sg.plan = (uint32_t)i;
             if (sg.brow >= 0) sg.brow = (int64_t)(tab_at[i] + sg.brow * 4), sg.bcol = (int64_t)(tab_at[i] + sg.bcol * 4);
+            if (sg.arow >= 0) sg.arow = (int64_t)(tab_at[i] + sg.arow * 4), sg.acol = (int64_t)(tab_at[i] + sg.acol * 4);
+            if (sg.amax >= 0) sg.amax += max_at[i];
+            if (sg.bmax >= 0) sg.bmax += max_at[i];
             L.mid.push_back(sg);
           }
-          for (Single sg : R.big_strips) sg.in = fix(sg.in), sg.plan = (uint32_t)i, L.big_strips.push_back(sg);
+          for (Single sg : R.big_strips) {
+            sg.in = fix(sg.in), sg.plan = (uint32_t)i;
+            if (sg.max_out >= 0) sg.max_out += max_at[i];
+            L.big_strips.push_back(sg);
+          }
         }
         uint64_t b = 0;
         for (auto& g : perms) g.block0 = b, b += blocks_for(g.numel);
@@ -690,6 +731,7 @@ void GroupedExecutor::run(const std::vector<const void* const*>& inputs, const s
       char* dev = G.dev;
       char* const* dbases = reinterpret_cast<char* const*>(dev + G.bases_off);
       auto* slots = reinterpret_cast<unsigned long long*>(dev + G.slots_off);
+      auto* maxtab = reinterpret_cast<unsigned long long*>(dev + G.maxtab_off);
       auto* expo = reinterpret_cast<long long*>(dev + G.expo_off);
       auto* log10_last = reinterpret_cast<double*>(dev + G.expo_off + ng * 8);
       if (strip_) {  // exponents and log10 parts
@@ -711,6 +753,10 @@ void GroupedExecutor::run(const std::vector<const void* const*>& inputs, const s
             if (sg.fold) g.cmax = slots + sg.plan;
             if (sg.brow >= 0)
               g.brow = reinterpret_cast<const int32_t*>(dev + sg.brow), g.bcol = reinterpret_cast<const int32_t*>(dev + sg.bcol);
+            if (sg.arow >= 0)
+              g.arow = reinterpret_cast<const int32_t*>(dev + sg.arow), g.acol = reinterpret_cast<const int32_t*>(dev + sg.acol);
+            if (sg.amax >= 0) g.amax = maxtab + sg.amax;
+            if (sg.bmax >= 0) g.bmax = maxtab + sg.bmax;
             launch_gemm(dt, prec_, GEMM_AUTO, g, s);
             break;
           }
@@ -767,7 +813,7 @@ void GroupedExecutor::run(const std::vector<const void* const*>& inputs, const s
                 launch_strip_exact(dt, at(sg.in), sg.A, (uint64_t*)(slots + sg.plan), log10_last + sg.plan, s);
               else
                 launch_scale_pow2(dt, at(sg.in), sg.A, (const uint64_t*)(slots + sg.plan), (int64_t*)(expo + sg.plan), s,
-                                  STRIP_KEEP);
+                                  STRIP_KEEP, sg.max_out >= 0 ? (uint64_t*)(maxtab + sg.max_out) : nullptr);
             }
             ATN_HIP(hipGetLastError());
           }

@@ -91,8 +91,9 @@ void launch_diag_scatter(int es, bool trace, size_t A, size_t d, size_t Bm, size
 void launch_maxabs(int dt, const void* x, size_t numel, uint64_t* maxbits, hipStream_t s);
 // keep > 0: leave x (and expo) unchanged while the max's exponent e lies in
 // [-keep, keep] — the power-of-two scale is only needed against over/underflow.
+// scaled_max (optional): receives the max of x after the call as double bits.
 void launch_scale_pow2(int dt, void* x, size_t numel, const uint64_t* maxbits, int64_t* expo, hipStream_t s,
-                       int keep = 0);
+                       int keep = 0, uint64_t* scaled_max = nullptr);
 // The last step of a stripped run: divide by the exact max magnitude m (Scalar::abs,
 // f64) and add log10(m) to log10_exp (src/compiled.rs:412-430); maxbits is scratch.
 void launch_strip_exact(int dt, void* x, size_t numel, uint64_t* maxbits, double* log10_exp, hipStream_t s);

@@ -687,54 +687,25 @@ __global__ void maxabs_kernel(const T* __restrict__ x, uint64_t n, unsigned long
   if ((threadIdx.x & 63) == 0 && m > 0.0) atomicMax(maxbits, (unsigned long long)__double_as_longlong(m));
 }
 
+// scaled_max (optional): receives the max after the rescale (max * 2^-e, or the
+// max itself when nothing is rescaled) as double bits, for a later product's
+// FP16 operand scale.
 template <class T>
-__global__ void scale_pow2_kernel(T* __restrict__ x, uint64_t n, const unsigned long long* maxbits, long long* expo, int keep);
-template <>
-__global__ void scale_pow2_kernel<float2>(float2* __restrict__ x, uint64_t n, const unsigned long long* maxbits, long long* expo, int keep) {
-  double m = __longlong_as_double((long long)*maxbits);
-  if (!(m > 0.0) || !isfinite(m)) return;
-  int e;
-  frexp(m, &e);
-  if (keep > 0 && e >= -keep && e <= keep) return;
-  if (blockIdx.x == 0 && threadIdx.x == 0) *expo += e;
-  float s = (float)ldexp(1.0, -e);
+__global__ void scale_pow2_kernel(T* __restrict__ x, uint64_t n, const unsigned long long* maxbits, long long* expo, int keep,
+                                  unsigned long long* scaled_max) {
+  const double m = __longlong_as_double((long long)*maxbits);
+  const bool first = blockIdx.x == 0 && threadIdx.x == 0;
+  int e = 0;
+  if (m > 0.0 && isfinite(m)) {
+    frexp(m, &e);
+    if (keep > 0 && e >= -keep && e <= keep) e = 0;
+  }
+  if (first && scaled_max) *scaled_max = (unsigned long long)__double_as_longlong(e ? ldexp(m, -e) : m);
+  if (e == 0) return;
+  if (first) *expo += e;
+  const typename Arith<T>::R s = (typename Arith<T>::R)ldexp(1.0, -e);
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
-    x[i] = make_float2(x[i].x * s, x[i].y * s);
-}
-template <>
-__global__ void scale_pow2_kernel<double2>(double2* __restrict__ x, uint64_t n, const unsigned long long* maxbits, long long* expo, int keep) {
-  double m = __longlong_as_double((long long)*maxbits);
-  if (!(m > 0.0) || !isfinite(m)) return;
-  int e;
-  frexp(m, &e);
-  if (keep > 0 && e >= -keep && e <= keep) return;
-  if (blockIdx.x == 0 && threadIdx.x == 0) *expo += e;
-  double s = ldexp(1.0, -e);
-  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
-    x[i] = make_double2(x[i].x * s, x[i].y * s);
-}
-
-template <>
-__global__ void scale_pow2_kernel<float>(float* __restrict__ x, uint64_t n, const unsigned long long* maxbits, long long* expo, int keep) {
-  double m = __longlong_as_double((long long)*maxbits);
-  if (!(m > 0.0) || !isfinite(m)) return;
-  int e;
-  frexp(m, &e);
-  if (keep > 0 && e >= -keep && e <= keep) return;
-  if (blockIdx.x == 0 && threadIdx.x == 0) *expo += e;
-  float s = (float)ldexp(1.0, -e);
-  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) x[i] *= s;
-}
-template <>
-__global__ void scale_pow2_kernel<double>(double* __restrict__ x, uint64_t n, const unsigned long long* maxbits, long long* expo, int keep) {
-  double m = __longlong_as_double((long long)*maxbits);
-  if (!(m > 0.0) || !isfinite(m)) return;
-  int e;
-  frexp(m, &e);
-  if (keep > 0 && e >= -keep && e <= keep) return;
-  if (blockIdx.x == 0 && threadIdx.x == 0) *expo += e;
-  double s = ldexp(1.0, -e);
-  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) x[i] *= s;
+    x[i] = Arith<T>::scale(x[i], s);
 }
 
 // Exact normalisation of a stripped run's last step (src/compiled.rs:412-430):
@@ -783,12 +754,13 @@ void launch_maxabs(int dt, const void* x, size_t numel, uint64_t* maxbits, hipSt
     LAUNCH(maxabs_kernel<T>, grid_for(numel, 256, 2048), 256, 0, s, (const T*)x, (uint64_t)numel, (unsigned long long*)maxbits);
   });
 }
-void launch_scale_pow2(int dt, void* x, size_t numel, const uint64_t* maxbits, int64_t* expo, hipStream_t s, int keep) {
+void launch_scale_pow2(int dt, void* x, size_t numel, const uint64_t* maxbits, int64_t* expo, hipStream_t s, int keep,
+                       uint64_t* scaled_max) {
   dispatch_dt(dt, [&](auto tag) {
     using T = decltype(tag);
     g_last_kind = K_STRIP;
     LAUNCH(scale_pow2_kernel<T>, grid_for(numel, 256), 256, 0, s, (T*)x, (uint64_t)numel, (const unsigned long long*)maxbits,
-           (long long*)expo, keep);
+           (long long*)expo, keep, (unsigned long long*)scaled_max);
   });
 }
 
