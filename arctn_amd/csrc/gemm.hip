@@ -666,6 +666,7 @@ __global__ __launch_bounds__(256) void gemm_mfma_kernel(GemmArgs g) {
           if constexpr (GATHER)
             if (kt + 2 < nk) rn = rows_at(kbeg + (kt + 2) * BK);
         }
+        __builtin_amdgcn_s_setprio(1);  // the MFMA run at raised priority (as in gemm_presplit_kernel)
 #pragma unroll
         for (int kk = 0; kk < BK; kk += 4) {
           F af[TM], bf[TN];
@@ -679,6 +680,7 @@ __global__ __launch_bounds__(256) void gemm_mfma_kernel(GemmArgs g) {
 #pragma unroll
             for (int j = 0; j < TN; ++j) acc[i][j] = Mfma<F>::op(af[i], bf[j], acc[i][j]);
         }
+        __builtin_amdgcn_s_setprio(0);
         __syncthreads();
         if (kt + 1 < nk) {
           store();
@@ -1473,7 +1475,11 @@ __global__ __launch_bounds__(256, BM == 32 ? 3 : 2) void gemm_presplit_kernel(Ge
             bl[j] = *reinterpret_cast<const V*>(&blo[swz<BK>(wn0 + j * 16 + cl, kg)]);
           }
           // the three products of one accumulator are TM*TN instructions apart,
-          // so consecutive MFMAs never wait on each other's result
+          // so consecutive MFMAs never wait on each other's result. A wave in its
+          // MFMA run issues at raised priority, so the SIMD's other wave (staging,
+          // waiting on memory) yields the issue slots to it (gfx950, complex64
+          // x64: m=1024 k=512 n=8192 7.44 -> 7.24 ms, m=512 k=512 n=8192 4.04 -> 3.95)
+          __builtin_amdgcn_s_setprio(1);
 #pragma unroll
           for (int i = 0; i < TM; ++i)
 #pragma unroll
@@ -1487,6 +1493,7 @@ __global__ __launch_bounds__(256, BM == 32 ? 3 : 2) void gemm_presplit_kernel(Ge
 #pragma unroll
             for (int j = 0; j < TN; ++j) acc[i][j] = Half<F16>::mfma(ah[i], bh[j], acc[i][j]);
         }
+        __builtin_amdgcn_s_setprio(0);
         __syncthreads();
         if (kt + 1 < nk) {
           store();
@@ -1724,6 +1731,7 @@ __global__ __launch_bounds__(256, 2) void gemm_split3m_kernel(GemmArgs g, PreSpl
       __syncthreads();
       for (int64_t kt = 0; kt < nk; ++kt) {
         if (kt + 1 < nk) load((kt + 1) * BK);
+        __builtin_amdgcn_s_setprio(1);  // as in gemm_presplit_kernel
 #pragma unroll
         for (int p = 0; p < 3; ++p) {
           V ah[TM], al[TM], bh[TN], bl[TN];
@@ -1751,6 +1759,7 @@ __global__ __launch_bounds__(256, 2) void gemm_split3m_kernel(GemmArgs g, PreSpl
 #pragma unroll
             for (int j = 0; j < TN; ++j) acc[p][i][j] = Half<F16>::mfma(ah[i], bh[j], acc[p][i][j]);
         }
+        __builtin_amdgcn_s_setprio(0);
         __syncthreads();
         if (kt + 1 < nk) {
           store();
