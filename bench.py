@@ -312,7 +312,8 @@ def main():
 
 
 # kernels that make up one launch of a split-precision GEMM kind
-SPLIT_KERNELS = ("gemm_presplit_kernel", "split_a_kernel", "split_bt_kernel", "maxabs_kernel", "gemm_split_kernel")
+SPLIT_KERNELS = ("gemm_presplit_kernel", "split_a_kernel", "split_bt_kernel", "maxabs_kernel", "gemm_split_kernel",
+                 "gemm_split3m_kernel", "split_a3m_kernel")
 
 
 def library_gemm():
@@ -378,7 +379,8 @@ def traffic_from(fetch_csv, write_csv, copy_fetch_csv, copy_write_csv, last=0):
     for counter, path in (("FETCH_SIZE", fetch_csv), ("WRITE_SIZE", write_csv)):
         tot, disp = per_kernel(path, counter)
         kib = sum(v for k, v in tot.items() if any(s in k for s in SPLIT_KERNELS))
-        launches = sum(v for k, v in disp.items() if "gemm_presplit_kernel" in k or "gemm_split_kernel" in k)
+        launches = sum(v for k, v in disp.items()
+                       if "gemm_presplit_kernel" in k or "gemm_split_kernel" in k or "gemm_split3m_kernel" in k)
         out[counter] = kib * 1024.0 * scale[counter] / max(launches, 1)
         out[counter + "_launches"] = launches
     res = {"hbm_bytes_per_launch": out["FETCH_SIZE"] + out["WRITE_SIZE"], "fetch_bytes_per_launch": out["FETCH_SIZE"],
