@@ -1420,12 +1420,15 @@ __global__ __launch_bounds__(256, BM == 32 ? 3 : 2) void gemm_presplit_kernel(Ge
             else row = q % BC, ch = (q / BC) * 8;
             uint32_t h[4], l[4];
             if constexpr (CPLX) {
+              // (re, im) split once; the rows (re, -im) and (im, re) are its halves with
+              // the high one negated, and swapped (rounding commutes with negation)
               uint32_t h2[4], l2[4];
 #pragma unroll
               for (int e = 0; e < 4; ++e) {
-                const float re = rbr[u * 4 + e].x * sb, im = rbr[u * 4 + e].y * sb;
-                split_pair<F16>(v2f{re, -im}, h[e], l[e]);
-                split_pair<F16>(v2f{im, re}, h2[e], l2[e]);
+                uint32_t hh, ll;
+                split_pair<F16>(v2f{rbr[u * 4 + e].x * sb, rbr[u * 4 + e].y * sb}, hh, ll);
+                h[e] = hh ^ 0x80000000u, l[e] = ll ^ 0x80000000u;
+                h2[e] = swap16(hh), l2[e] = swap16(ll);
               }
               *reinterpret_cast<uint4*>(bhi + swz<BK>(row, ch)) = *reinterpret_cast<const uint4*>(h);
               *reinterpret_cast<uint4*>(blo + swz<BK>(row, ch)) = *reinterpret_cast<const uint4*>(l);
