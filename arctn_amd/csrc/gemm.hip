@@ -1080,6 +1080,11 @@ constexpr int64_t PRESPLIT_MAX_TABLE = 4096;
 struct PreSplit {
   const void *ahi, *alo, *bhi, *blo;
   int64_t kp;
+  // split k (pre-split B): blockIdx.z takes halves [z * kc, (z + 1) * kc) of kp and
+  // writes its partial product to part[z][batch][m][n] (the result's element type),
+  // summed in range order by splitk_reduce_kernel; kc = 0: all of k, into C
+  int64_t kc = 0;
+  void* part = nullptr;
 };
 
 static __host__ __device__ inline int64_t n_mats(int64_t so, int64_t si, int64_t nbo, int64_t nbi) {
@@ -1290,7 +1295,11 @@ __global__ __launch_bounds__(256, BM == 32 ? 3 : 2) void gemm_presplit_kernel(Ge
     sb = ldexpf(1.f, pow2_shift(*g.bmax));
   }
   const int64_t M = g.m, N = CPLX ? 2 * g.n : g.n, kp = ps.kp;
-  const int64_t ldc = CPLX ? 2 * g.ldc : g.ldc;
+  // k range of this workgroup (split k) and where its product goes
+  const int64_t kb = ps.kc ? (int64_t)blockIdx.z * ps.kc : 0;
+  const int64_t ke = ps.kc && kb + ps.kc < kp ? kb + ps.kc : kp;
+  const bool fold = g.cmax && !ps.part;  // partial products: the reduce kernel folds the max
+  const int64_t ldc = ps.part ? N : CPLX ? 2 * g.ldc : g.ldc;
   const int tiles_n = (int)((N + BN - 1) / BN), tiles_m = (int)((M + BM - 1) / BM);
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm0 = BM == 128 ? (wave >> 1) * WM : 0, wn0 = BM == 128 ? (wave & 1) * WN : wave * WN;
@@ -1327,7 +1336,8 @@ __global__ __launch_bounds__(256, BM == 32 ? 3 : 2) void gemm_presplit_kernel(Ge
     const H* Bh = BRAW ? nullptr : (const H*)ps.bhi + bmat * g.n * kp;
     const H* Bl = BRAW ? nullptr : (const H*)ps.blo + bmat * g.n * kp;
     const BT* Braw = (const BT*)g.b + zo * g.sbo + zi * g.sbi;
-    float* C = (float*)g.c + (zo * g.sco + zi * g.sci) * (CPLX ? 2 : 1);
+    float* C = ps.part ? (float*)ps.part + (((int64_t)blockIdx.z * (int64_t)nz + (int64_t)z) * M * N)
+                       : (float*)g.c + (zo * g.sco + zi * g.sci) * (CPLX ? 2 : 1);
     const auto rb = buf_rsrc(Braw, RAW_LIMIT);
     for (uint64_t t = blockIdx.x; t < ntile; t += gridDim.x) {
       int64_t m0, n0;
@@ -1455,13 +1465,13 @@ __global__ __launch_bounds__(256, BM == 32 ? 3 : 2) void gemm_presplit_kernel(Ge
           }
         }
       };
-      const int64_t nk = kp / BK;
+      const int64_t nk = (ke - kb) / BK;
       const int cl = lane & 15;
-      load(0);
+      load(kb);
       store();
       __syncthreads();
       for (int64_t kt = 0; kt < nk; ++kt) {
-        if (kt + 1 < nk) load((kt + 1) * BK);
+        if (kt + 1 < nk) load(kb + (kt + 1) * BK);
 #pragma unroll
         for (int ks = 0; ks < BK; ks += 32) {
           const int kg = ks + (lane >> 4) * 8;
@@ -1542,14 +1552,14 @@ __global__ __launch_bounds__(256, BM == 32 ? 3 : 2) void gemm_presplit_kernel(Ge
           v.x *= unscale, v.y *= unscale, v.z *= unscale, v.w *= unscale;
           if (c16 && col + 3 < N) {
             *reinterpret_cast<float4*>(C + row * ldc + col) = v;
-            if (g.cmax) fold_max(mx, v.x), fold_max(mx, v.y), fold_max(mx, v.z), fold_max(mx, v.w);
+            if (fold) fold_max(mx, v.x), fold_max(mx, v.y), fold_max(mx, v.z), fold_max(mx, v.w);
           } else {
             const float e4[4] = {v.x, v.y, v.z, v.w};
 #pragma unroll
             for (int e = 0; e < 4; ++e)
               if (col + e < N) {
                 C[row * ldc + col + e] = e4[e];
-                if (g.cmax) fold_max(mx, e4[e]);
+                if (fold) fold_max(mx, e4[e]);
               }
           }
         }
@@ -1557,7 +1567,7 @@ __global__ __launch_bounds__(256, BM == 32 ? 3 : 2) void gemm_presplit_kernel(Ge
       __syncthreads();  // the next tile stages into the same LDS
     }
   }
-  if (g.cmax) publish_max((double)mx, g.cmax);
+  if (fold) publish_max((double)mx, g.cmax);
 }
 
 // ------------------------------------------- 3M split-precision GEMM --
@@ -2099,6 +2109,17 @@ static bool raw_b_fits(int dt, const GemmArgs& g) {
          b_span * (dt == C64 ? 8 : 4) < (int64_t)(1u << 30);
 }
 
+// Split k for the pre-split GEMM with pre-split B: shapes with few output tiles
+// and a long k (the last products of closed networks) get about 1024 workgroups,
+// each over at least 2048 halves of k; `avail` bytes for the partial products.
+static KSplit plan_presplit_split(int dt, const GemmArgs& g, int64_t kp, int bm, size_t avail) {
+  const int64_t N = dt == C64 ? 2 * g.n : g.n;
+  const int64_t tiles = g.nbo * g.nbi * ((g.m + bm - 1) / bm) * ((N + 127) / 128);
+  if (tiles >= 512 || g.nbo * g.nbi > 65535) return KSplit{};
+  return split_k(kp, (1024 + tiles - 1) / tiles, 2048, PRESPLIT_BK,
+                 dtype_size(dt) * (size_t)(g.nbo * g.nbi * g.m * g.n), avail);
+}
+
 // Split-precision products pay off on MFMA-bound shapes only; memory-bound
 // skinny steps run the FP32 kernels (same time, full FP32 accuracy).
 // Split products pay off with 128-row tiles from m = 64 and with 32-row tiles
@@ -2314,7 +2335,22 @@ static int launch_gemm_kernels(int dt, int prec, int kernel, const GemmArgs& g0,
         else PRESPLIT_MODES(1);
         return GEMM_PRESPLIT_A;
       }
+      // pre-split B: split k over workgroups where the output has few tiles
+      const size_t avail = g.scratch_bytes > 256 + psz.total ? g.scratch_bytes - 256 - psz.total : 0;
+      const KSplit ks = plan_presplit_split(dt, g, psz.kp, bm, avail);
+      if (ks.n_split > 1) {
+        ps.kc = ks.k_chunk, ps.part = base + psz.total;
+        grid.z = (unsigned)ks.n_split;
+      }
       PRESPLIT_MODES(0);
+      if (ks.n_split > 1) {
+        SplitK sk{};
+        sk.n_split = ks.n_split, sk.part = ps.part;
+        const int64_t n_out = g.nbo * g.nbi * g.m * g.n;
+        const dim3 rg((unsigned)std::min<int64_t>((n_out + 255) / 256, 65536));
+        if (cx) LAUNCH((splitk_reduce_kernel<float2>), rg, dim3(256), 0, s, g, sk);
+        else LAUNCH((splitk_reduce_kernel<float>), rg, dim3(256), 0, s, g, sk);
+      }
 #undef PRESPLIT_MODES
 #undef PRESPLIT_LAUNCH
       return GEMM_PRESPLIT;
@@ -2394,9 +2430,19 @@ size_t gemm_scratch_bytes(int dt, int prec, int kernel, const GemmArgs& g) {
                          prec == C3M ? (g.m <= 64 ? 64 : 128) : mfma_rows_for(dt, kernel, g.m));
   size_t part = ks.n_split > 1 ? (size_t)ks.n_split * dtype_size(dt) * (size_t)(g.nbo * g.nbi * g.m * g.n) : 0;
   if (split_precision(prec) && (dt == F32 || dt == C64)) {
-    if (kernel == GEMM_PRESPLIT) part = presplit_sizes(dt, g).total;
+    if (kernel == GEMM_PRESPLIT) {
+      const PreSplitSizes psz = presplit_sizes(dt, g);
+      const KSplit ks = plan_presplit_split(dt, g, psz.kp, g.m <= 32 ? 32 : 128, MAX_PARTIAL_BYTES);
+      part = psz.total + (ks.n_split > 1 ? (size_t)ks.n_split * dtype_size(dt) * (size_t)(g.nbo * g.nbi * g.m * g.n) : 0);
+    }
     if (kernel == GEMM_PRESPLIT_A) {
-      part = raw_b_fits(dt, g) ? 2 * presplit_sizes(dt, g).a_half : presplit_sizes(dt, g).total;
+      if (raw_b_fits(dt, g)) {
+        part = 2 * presplit_sizes(dt, g).a_half;
+      } else {  // B pre-split as well (GEMM_PRESPLIT)
+        const PreSplitSizes psz = presplit_sizes(dt, g);
+        const KSplit ks = plan_presplit_split(dt, g, psz.kp, g.m <= 32 ? 32 : 128, MAX_PARTIAL_BYTES);
+        part = psz.total + (ks.n_split > 1 ? (size_t)ks.n_split * dtype_size(dt) * (size_t)(g.nbo * g.nbi * g.m * g.n) : 0);
+      }
       if (split_3m(prec) && dt == C64 && raw_b_fits(dt, g)) part = std::max(part, 2 * s3m_a_half(g));
     }
   }

@@ -4,7 +4,8 @@ Both entry points choose the same kernels with the same scratch sizes, so the
 results must be bit-identical; every case runs twice in a row, the second time
 on a smaller problem (the shared device scratch array keeps the first call's
 contents). The kernels cover every scratch user: split-K partial sums, FP32
-MFMA with k ranges, pre-split 16-bit operand halves and FP16X2 operand maxima.
+MFMA with k ranges, pre-split 16-bit operand halves (with and without k split
+over workgroups) and FP16X2 operand maxima.
 """
 import numpy as np
 import pytest
@@ -22,6 +23,7 @@ CASES = [
     (np.complex64, 2, 4, (2, 300, 256, 200), (1, 100, 64, 50)),
     (np.float32, 3, 8, (2, 500, 512, 300), (1, 130, 70, 129)),
     (np.complex64, 3, 1, (1, 256, 256, 256), (1, 64, 40, 72)),
+    (np.complex64, 3, 4, (1, 64, 20000, 64), (1, 40, 9000, 30)),  # pre-split, k split over workgroups
 ]
 
 
