@@ -100,6 +100,16 @@ __device__ inline void publish_max(double mx, unsigned long long* cmax) {
 __device__ inline int64_t b_index(const GemmArgs& g, int64_t p, int64_t j) {
   return g.bcol ? (int64_t)g.brow[p] + (int64_t)g.bcol[j] : p * g.ldb + j;
 }
+// Column offset of a factored B (GemmArgs::bcg > 0).
+__device__ inline int64_t fac_col(const GemmArgs& g, int64_t j) {
+  int64_t off = 0;
+  for (int q = 0; q < g.bcg; ++q) {
+    const int64_t d = g.bcd[q];
+    off += (j % d) * g.bcs[q];
+    j /= d;
+  }
+  return off;
+}
 // The two halves of b_index, for loaders whose column stays fixed across k tiles.
 __device__ inline int64_t b_row(const GemmArgs& g, int64_t p) { return g.bcol ? (int64_t)g.brow[p] : p * g.ldb; }
 __device__ inline int64_t b_col(const GemmArgs& g, int64_t j) { return g.bcol ? (int64_t)g.bcol[j] : j; }
@@ -222,7 +232,8 @@ template <> struct Vec<double, 2> { using W = double2; };
 template <> struct Vec<double, 1> { using W = double; };
 template <> struct Vec<double2, 1> { using W = double2; };
 
-template <class T, int MM, int V>
+// FAC: B through a factored column offset and int64 row offsets (GemmArgs::bcg).
+template <class T, int MM, int V, bool FAC>
 __global__ __launch_bounds__(256) void gemm_stream_kernel(GemmArgs g) {
   constexpr int KU = 8;
   using W = typename Vec<T, V>::W;
@@ -241,11 +252,13 @@ __global__ __launch_bounds__(256) void gemm_stream_kernel(GemmArgs g) {
       for (int i = 0; i < MM; ++i)
 #pragma unroll
         for (int v = 0; v < V; ++v) acc[i][v] = CM<T>::zero();
+      const int64_t cj = FAC ? fac_col(g, j0) : 0;
+      auto bat = [&](int64_t p) { return B + (FAC ? g.brow64[p] + cj : b_index(g, p, j0)); };
       int64_t p = 0;
       for (; p + KU <= g.k; p += KU) {
         W b[KU];
 #pragma unroll
-        for (int u = 0; u < KU; ++u) b[u] = *reinterpret_cast<const W*>(B + b_index(g, p + u, j0));
+        for (int u = 0; u < KU; ++u) b[u] = *reinterpret_cast<const W*>(bat(p + u));
 #pragma unroll
         for (int u = 0; u < KU; ++u) {
           const T* bu = reinterpret_cast<const T*>(&b[u]);
@@ -259,7 +272,7 @@ __global__ __launch_bounds__(256) void gemm_stream_kernel(GemmArgs g) {
         }
       }
       for (; p < g.k; ++p) {
-        W bw = *reinterpret_cast<const W*>(B + b_index(g, p, j0));
+        W bw = *reinterpret_cast<const W*>(bat(p));
         const T* bu = reinterpret_cast<const T*>(&bw);
 #pragma unroll
         for (int i = 0; i < MM; ++i)
@@ -1962,7 +1975,8 @@ static void run_stream_t(const GemmArgs& g, hipStream_t s) {
   if (g.cmax) gx = std::min<unsigned>(gx, std::max<unsigned>(1, TRACK_WORKGROUPS / zgrid(g)));
   dim3 grid(gx, zgrid(g));
   if ((uint64_t)grid.y * gx > (1u << 22)) grid.y = std::max<unsigned>(1, (1u << 22) / gx);
-  LAUNCH((gemm_stream_kernel<T, MM, V>), grid, dim3(256), 0, s, g);
+  if (g.bcg) LAUNCH((gemm_stream_kernel<T, MM, V, true>), grid, dim3(256), 0, s, g);
+  else LAUNCH((gemm_stream_kernel<T, MM, V, false>), grid, dim3(256), 0, s, g);
 }
 
 // 16-byte column vectors when every row start of B and C is 16-byte aligned.
@@ -1971,8 +1985,8 @@ static int stream_width(int dt, const GemmArgs& g) {
   int v = 16 / es;
   auto ok = [&](int w) {
     const int64_t b = (int64_t)w * es;
-    // gathered columns: only within runs of consecutive offsets
-    const bool cols = g.bcol ? g.bcol_run % w == 0 : g.ldb % w == 0;
+    // gathered or factored columns: only within runs of consecutive offsets
+    const bool cols = g.bcg ? g.bcs[0] == 1 && g.bcd[0] % w == 0 : g.bcol ? g.bcol_run % w == 0 : g.ldb % w == 0;
     return g.n % w == 0 && cols && g.ldc % w == 0 && g.sbo % w == 0 && g.sbi % w == 0 && g.sco % w == 0 &&
            g.sci % w == 0 && (uintptr_t)g.b % b == 0 && (uintptr_t)g.c % b == 0;
   };
@@ -2023,6 +2037,7 @@ static bool run_stream(int dt, const GemmArgs& g, hipStream_t s) {
 // 0: not a skinny shape; 1: streaming kernel (folds the result max); 2: smallm / smalln kernel
 static int run_skinny(int dt, const GemmArgs& g, hipStream_t s) {
   if (run_stream(dt, g, s)) return 1;
+  if (g.bcg) throw Error("a factored B operand needs the streaming kernel (n >= 64, small m)");
   if (g.bcol) return 0;  // the smallm / smalln kernels read B row-major only
   if (g.m <= 32 && g.n >= 64) {
     g_last_kind = K_GEMM_SMALL;
@@ -2222,6 +2237,7 @@ static void split3m_launch(const GemmArgs& g, const PreSplit& ps, dim3 grid, hip
 static int launch_gemm_kernels(int dt, int prec, int kernel, const GemmArgs& g0, hipStream_t s) {
   GemmArgs g = g0;
   if (kernel == GEMM_AUTO) kernel = auto_kernel(dt, prec, g);
+  if (g.bcg) kernel = GEMM_SKINNY;     // factored B: read by the streaming kernel only
   if (g.k == 0) kernel = GEMM_SIMPLE;  // empty sums: every kernel but this one assumes k tiles
   // A through gather tables: read by the simple kernel and by the split passes of
   // the pre-split kernels (one batch entry); any other choice runs the simple kernel

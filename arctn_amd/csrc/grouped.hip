@@ -264,6 +264,10 @@ struct Single {  // an op launched on its own
   // GEMM: B read through gather tables instead of a permute (offsets into the
   // plan's table vector, then byte offsets into the group's descriptor blob), or -1
   int64_t brow = -1, bcol = -1;
+  // GEMM: B too large for int32 tables, read through a factored column offset
+  // (gemm.bcg groups) and int64 row offsets: offset into the plan's int64 table
+  // vector, then a byte offset into the group's blob, or -1
+  int64_t brow64 = -1;
   // GEMM: A read through gather tables by the split pass of a split-precision
   // product instead of a permute (the same offsets), or -1
   int64_t arow = -1, acol = -1;
@@ -292,6 +296,7 @@ struct PlanProg {
   std::vector<Round> rounds;
   size_t arena = 0;  // bytes
   std::vector<int32_t> tables;  // gather tables of the B operands of large GEMMs
+  std::vector<int64_t> tables64;  // row offsets of factored B operands
   int n_max = 0;                // result-max slots (large results read by later large GEMMs)
 };
 
@@ -311,6 +316,27 @@ void merged_axes(const std::vector<size_t>& shape, const std::vector<size_t>& pe
       out.push_back(GAxis{st[a], (uint32_t)shape[a], 0});
     }
   }
+}
+
+// B's free legs (output axes n_con.. of perm) as at most 6 groups of axes that are
+// adjacent in memory, innermost first: GemmArgs::bcg / bcd / bcs. false when the
+// free legs are not in memory order or form more groups.
+bool factor_columns(const std::vector<size_t>& shape, const std::vector<size_t>& perm, size_t n_con, GemmArgs& f) {
+  std::vector<int64_t> st(shape.size(), 1);
+  for (size_t a = shape.size(); a-- > 1;) st[a - 1] = st[a] * (int64_t)shape[a];
+  f.bcg = 0;
+  for (size_t d = perm.size(); d-- > n_con;) {
+    const size_t a = perm[d];
+    if (d + 1 < perm.size() && perm[d + 1] <= a) return false;  // not in memory order
+    if (f.bcg && perm[d + 1] == a + 1) {  // adjacent in memory: extends the current group
+      f.bcd[f.bcg - 1] *= (int64_t)shape[a];
+    } else {
+      if (f.bcg == 6) return false;
+      f.bcd[f.bcg] = (int64_t)shape[a], f.bcs[f.bcg] = st[a];
+      ++f.bcg;
+    }
+  }
+  return f.bcg > 0;
 }
 
 // Plans one plan's ops into `n_rounds` rounds: 0 preps, 1 + s step s, the last finalize.
@@ -419,6 +445,8 @@ PlanProg build_plan(const Plan& P, int dt, bool strip, int prec, size_t n_rounds
     }
     std::vector<int32_t> rows, cols;
     int32_t col_run = 1, row_run = 1;
+    GemmArgs fac{};  // factored columns of B (bcg > 0), and their int64 row offsets
+    std::vector<int64_t> fac_rows;
     bool gather_b = large && st.bt == 1 && st.sum_b.empty() && plan_permute(shapes[st.ri], b_perm, 1, es).kind != 0;
     if (gather_b) {
       if (n_con > 1) {
@@ -433,7 +461,30 @@ PlanProg build_plan(const Plan& P, int dt, bool strip, int prec, size_t n_rounds
         }
       }
       gather_b = gather_tables(shapes[st.ri], b_perm, n_con, rows, cols, col_run, row_run);
-      if (!gather_b) a_perm = st.a_perm, b_perm = st.b_perm;
+      // offsets beyond int32 (operands of more than 2^31 elements): B's columns as a
+      // mixed-radix function of the column index when its free legs keep their
+      // memory order, rows in int64, for the streaming kernel (short A)
+      if (!gather_b && st.m <= (dtype_complex(dt) ? 32u : 64u) && st.n >= 64 &&
+          factor_columns(shapes[st.ri], b_perm, n_con, fac)) {
+        fac_rows.clear();
+        const auto& sh = shapes[st.ri];
+        std::vector<int64_t> str(sh.size(), 1);
+        for (size_t q = sh.size(); q-- > 1;) str[q - 1] = str[q] * (int64_t)sh[q];
+        size_t nrow = 1;
+        for (size_t d = 0; d < n_con; ++d) nrow *= sh[b_perm[d]];
+        for (size_t e = 0; e < nrow; ++e) {
+          size_t rem = e;
+          int64_t off = 0;
+          for (size_t d = n_con; d-- > 0;) {
+            off += (int64_t)(rem % sh[b_perm[d]]) * str[b_perm[d]];
+            rem /= sh[b_perm[d]];
+          }
+          fac_rows.push_back(off);
+        }
+      } else {
+        fac.bcg = 0;
+      }
+      if (!gather_b && !fac.bcg) a_perm = st.a_perm, b_perm = st.b_perm;
     }
     // A of a large split-precision product: read through gather tables by the
     // split pass, which then does the permute's work
@@ -444,7 +495,7 @@ PlanProg build_plan(const Plan& P, int dt, bool strip, int prec, size_t n_rounds
                           plan_permute(shapes[st.li], a_perm, 1, es).kind != 0 &&
                           gather_tables(shapes[st.li], a_perm, a_perm.size() - n_con, arows, acols, crun, arun);
     GRef a = gather_a ? slot[st.li] : operand(st.li, st.sum_a, a_perm);
-    GRef b = gather_b ? slot[st.ri] : operand(st.ri, st.sum_b, b_perm);
+    GRef b = gather_b || fac.bcg ? slot[st.ri] : operand(st.ri, st.sum_b, b_perm);
     GRef c = newbuf(cnum * es, t);
     use(a, t), use(b, t);
     if (large) {
@@ -465,6 +516,13 @@ PlanProg build_plan(const Plan& P, int dt, bool strip, int prec, size_t n_rounds
       // operand maxima of earlier large results (a permute or a gather keeps the max)
       if (st.sum_a.empty()) sg.amax = res_max[st.li];
       if (st.sum_b.empty()) sg.bmax = res_max[st.ri];
+      if (fac.bcg) {
+        g.bcg = fac.bcg;
+        std::copy(fac.bcd, fac.bcd + 6, g.bcd);
+        std::copy(fac.bcs, fac.bcs + 6, g.bcs);
+        sg.brow64 = (int64_t)prog.tables64.size();
+        prog.tables64.insert(prog.tables64.end(), fac_rows.begin(), fac_rows.end());
+      }
       if (gather_b) {
         sg.brow = (int64_t)prog.tables.size();
         prog.tables.insert(prog.tables.end(), rows.begin(), rows.end());
@@ -611,10 +669,12 @@ GroupedExecutor::GroupedExecutor(std::vector<std::shared_ptr<const Plan>> plans,
       int n_max = 0;
       for (size_t i = 0; i < ng; ++i) max_at[i] = n_max, n_max += progs[G.g0 + i].n_max;
       G.maxtab_off = append(nullptr, (size_t)std::max(n_max, 1) * 8);
-      std::vector<size_t> tab_at(ng);  // byte offset of each plan's gather tables in the blob
+      std::vector<size_t> tab_at(ng), tab64_at(ng);  // byte offsets of each plan's tables in the blob
       for (size_t i = 0; i < ng; ++i) {
         const auto& tb = progs[G.g0 + i].tables;
         tab_at[i] = tb.empty() ? 0 : append(tb.data(), tb.size() * sizeof(int32_t));
+        const auto& t64 = progs[G.g0 + i].tables64;
+        tab64_at[i] = t64.empty() ? 0 : append(t64.data(), t64.size() * sizeof(int64_t));
       }
       G.launches.resize(n_rounds);
       for (size_t r = 0; r < n_rounds; ++r) {
@@ -643,6 +703,7 @@ GroupedExecutor::GroupedExecutor(std::vector<std::shared_ptr<const Plan>> plans,
             sg.in = fix(sg.in), sg.in2 = fix(sg.in2), sg.out = fix(sg.out), sg.plan = (uint32_t)i;
             if (sg.brow >= 0) sg.brow = (int64_t)(tab_at[i] + sg.brow * 4), sg.bcol = (int64_t)(tab_at[i] + sg.bcol * 4);
             if (sg.arow >= 0) sg.arow = (int64_t)(tab_at[i] + sg.arow * 4), sg.acol = (int64_t)(tab_at[i] + sg.acol * 4);
+            if (sg.brow64 >= 0) sg.brow64 = (int64_t)(tab64_at[i] + sg.brow64 * 8);
             if (sg.amax >= 0) sg.amax += max_at[i];
             if (sg.bmax >= 0) sg.bmax += max_at[i];
             L.mid.push_back(sg);
@@ -755,6 +816,7 @@ void GroupedExecutor::run(const std::vector<const void* const*>& inputs, const s
               g.brow = reinterpret_cast<const int32_t*>(dev + sg.brow), g.bcol = reinterpret_cast<const int32_t*>(dev + sg.bcol);
             if (sg.arow >= 0)
               g.arow = reinterpret_cast<const int32_t*>(dev + sg.arow), g.acol = reinterpret_cast<const int32_t*>(dev + sg.acol);
+            if (sg.brow64 >= 0) g.brow64 = reinterpret_cast<const int64_t*>(dev + sg.brow64);
             if (sg.amax >= 0) g.amax = maxtab + sg.amax;
             if (sg.bmax >= 0) g.bmax = maxtab + sg.bmax;
             launch_gemm(dt, prec_, GEMM_AUTO, g, s);

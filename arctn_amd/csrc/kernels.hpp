@@ -134,6 +134,13 @@ struct GemmArgs {
   // brow holds runs of this many consecutive offsets, each starting at a multiple
   // of the run length (B's contracted legs innermost in memory)
   int32_t brow_run = 1;
+  // B columns as a mixed-radix function of the column index, for operands too
+  // large for the int32 gather tables (read by the streaming kernel only): column
+  // j sits at sum_q ((j / (bcd[0] ... bcd[q-1])) % bcd[q]) * bcs[q] elements (group
+  // 0 innermost, bcg groups), row p at brow64[p]
+  int32_t bcg = 0;
+  int64_t bcd[6] = {}, bcs[6] = {};
+  const int64_t* brow64 = nullptr;
   // FP16X2: max |re|, |im| of A / of B as double bits when already known (folded
   // into the epilogue of the GEMM that produced the operand); null = measured by
   // a pass before the GEMM. An upper bound is enough: it only sets the scale.
