@@ -100,13 +100,16 @@ __device__ inline void publish_max(double mx, unsigned long long* cmax) {
 __device__ inline int64_t b_index(const GemmArgs& g, int64_t p, int64_t j) {
   return g.bcol ? (int64_t)g.brow[p] + (int64_t)g.bcol[j] : p * g.ldb + j;
 }
-// Column offset of a factored B (GemmArgs::bcg > 0).
+// Column offset of a factored B (GemmArgs::bcg > 0); the column count is below
+// 2^32, so the digits come from 32-bit divisions (64-bit ones cost the
+// streaming kernel several times its multiply-adds).
 __device__ inline int64_t fac_col(const GemmArgs& g, int64_t j) {
+  uint32_t r = (uint32_t)j;
   int64_t off = 0;
   for (int q = 0; q < g.bcg; ++q) {
-    const int64_t d = g.bcd[q];
-    off += (j % d) * g.bcs[q];
-    j /= d;
+    const uint32_t d = (uint32_t)g.bcd[q], t = r / d;
+    off += (int64_t)(r - t * d) * g.bcs[q];
+    r = t;
   }
   return off;
 }

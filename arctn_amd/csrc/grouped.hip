@@ -464,7 +464,7 @@ PlanProg build_plan(const Plan& P, int dt, bool strip, int prec, size_t n_rounds
       // offsets beyond int32 (operands of more than 2^31 elements): B's columns as a
       // mixed-radix function of the column index when its free legs keep their
       // memory order, rows in int64, for the streaming kernel (short A)
-      if (!gather_b && st.m <= (dtype_complex(dt) ? 32u : 64u) && st.n >= 64 &&
+      if (!gather_b && st.m <= (dtype_complex(dt) ? 32u : 64u) && st.n >= 64 && st.n <= UINT32_MAX &&
           factor_columns(shapes[st.ri], b_perm, n_con, fac)) {
         fac_rows.clear();
         const auto& sh = shapes[st.ri];

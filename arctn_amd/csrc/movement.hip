@@ -759,7 +759,9 @@ void launch_scale_pow2(int dt, void* x, size_t numel, const uint64_t* maxbits, i
   dispatch_dt(dt, [&](auto tag) {
     using T = decltype(tag);
     g_last_kind = K_STRIP;
-    LAUNCH(scale_pow2_kernel<T>, grid_for(numel, 256), 256, 0, s, (T*)x, (uint64_t)numel, (const unsigned long long*)maxbits,
+    // a grid-stride grid of at most 2048 workgroups: most calls find the max in range
+    // and return at once, so the launch costs little more than its first wave
+    LAUNCH(scale_pow2_kernel<T>, grid_for(numel, 256, 2048), 256, 0, s, (T*)x, (uint64_t)numel, (const unsigned long long*)maxbits,
            (long long*)expo, keep, (unsigned long long*)scaled_max);
   });
 }
