@@ -560,8 +560,14 @@ __global__ __launch_bounds__(256) void gemm_mfma_kernel(GemmArgs g) {
   static_assert(256 % (CPLX ? BN / 2 : BN) == 0, "a thread's B loads share one column");
   static_assert(A_PER >= 1 && B_PER >= 1, "tile too small for 256 threads");
   using V = typename Mfma<F>::V;
-  __shared__ F As[BK * SA];
-  __shared__ F Bs[BK * SB];
+  // one LDS array: the A and B tiles, and in the epilogue 32-row slices of the
+  // result tile (row stride BN + 16 bytes: the 16-lane groups of one store, whose
+  // rows lie 4 or 1 apart, hit distinct banks)
+  constexpr int SC = BN + 16 / (int)sizeof(F), EROWS = 32 * SC <= BK * (SA + SB) ? 32 : 16;
+  static_assert(EROWS * SC <= BK * (SA + SB) && BM % EROWS == 0, "epilogue slice fits the tile arrays");
+  __shared__ __align__(16) F sm[BK * (SA + SB)];
+  F* const As = sm;
+  F* const Bs = sm + BK * SA;
 
   const int64_t M = g.m, N = CPLX ? 2 * g.n : g.n, K = CPLX ? 2 * g.k : g.k;
   const int64_t lda = CPLX ? 2 * g.lda : g.lda;  // real units
@@ -704,18 +710,63 @@ __global__ __launch_bounds__(256) void gemm_mfma_kernel(GemmArgs g) {
         }
       }
       const int cl = lane & 15;
+      if constexpr (!CPLX) {  // real results: straight from the MFMA layout
 #pragma unroll
-      for (int i = 0; i < TM; ++i)
+        for (int i = 0; i < TM; ++i)
 #pragma unroll
-        for (int j = 0; j < TN; ++j)
+          for (int j = 0; j < TN; ++j)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+              const int64_t row = m0 + wm0 + i * 16 + Mfma<F>::row(lane, r), col = n0 + wn0 + j * 16 + cl;
+              if (row < M && col < N) {
+                C[row * ldo + col] = acc[i][j][r];
+                if constexpr (TRACK) fold_max(mx, acc[i][j][r]);
+              }
+            }
+        continue;
+      }
+      // Complex results: their real and imaginary parts come from different waves,
+      // so the MFMA layout alone stores 4-byte values at an 8-byte stride. They go
+      // through LDS instead, 32 (or 16) rows of the tile at a time: a row-major
+      // slice, then every thread writes 16-byte row pieces (the circuit's
+      // contraction 97.75 -> 97.0 ms; real results measured slower this way).
+      constexpr int E = 16 / (int)sizeof(F);  // elements per 16-byte piece
+      using W = typename std::conditional<sizeof(F) == 4, float4, double2>::type;
+      const bool c16 = ((uintptr_t)C & 15) == 0 && ldo % E == 0;
+#pragma unroll
+      for (int r0 = 0; r0 < BM; r0 += EROWS) {
+        __syncthreads();  // the LDS arrays are free (k loop or the previous slice done)
+#pragma unroll
+        for (int i = 0; i < TM; ++i)
 #pragma unroll
           for (int r = 0; r < 4; ++r) {
-            int64_t row = m0 + wm0 + i * 16 + Mfma<F>::row(lane, r), col = n0 + wn0 + j * 16 + cl;
-            if (row < M && col < N) {
-              C[row * ldo + col] = acc[i][j][r];
-              if constexpr (TRACK) fold_max(mx, acc[i][j][r]);
+            const int tr = wm0 + i * 16 + Mfma<F>::row(lane, r) - r0;
+            if (tr >= 0 && tr < EROWS)
+#pragma unroll
+              for (int j = 0; j < TN; ++j) sm[tr * SC + wn0 + j * 16 + cl] = acc[i][j][r];
+          }
+        __syncthreads();
+        for (int q = tid; q < EROWS * BN / E; q += 256) {
+          const int tr = q / (BN / E), ce = (q % (BN / E)) * E;
+          const int64_t row = m0 + r0 + tr, col = n0 + ce;
+          if (row < M) {
+            const F* src = &sm[tr * SC + ce];
+            if (c16 && col + E <= N) {
+              *reinterpret_cast<W*>(C + row * ldo + col) = *reinterpret_cast<const W*>(src);
+              if constexpr (TRACK)
+#pragma unroll
+                for (int e = 0; e < E; ++e) fold_max(mx, src[e]);
+            } else {
+              for (int e = 0; e < E; ++e)
+                if (col + e < N) {
+                  C[row * ldo + col + e] = src[e];
+                  if constexpr (TRACK) fold_max(mx, src[e]);
+                }
             }
           }
+        }
+      }
+      __syncthreads();  // the next tile stages into the same LDS
     }
   }
   if constexpr (TRACK) publish_max((double)mx, g.cmax);
