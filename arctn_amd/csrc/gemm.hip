@@ -2210,15 +2210,15 @@ static int native_mfma_kernel(int dt, int64_t m) {
 static int auto_kernel(int dt, int prec, const GemmArgs& g) {
   const bool split_prec = split_precision(prec) && (dt == F32 || dt == C64);
   if (split_prec && split_worth(dt, g)) {
-    // Up to eight row tiles of A for complex operands, two for real ones (32 rows
-    // for m <= 32, else 128): A split by a pass, B read in its own precision and
-    // split while staging (measured on gfx950, complex64: m=512 k=128 n=32768 x256
-    // 20.5 ms against 22.4 for a splitting pass over B and the pre-split GEMM,
-    // m=512 k=512 n=8192 15.6 against 17.1, m=1024 k=512 n=8192 x64 7.4 against
-    // 7.8; on the uneven float32 workload four row tiles ran 3% slower). With more
-    // row tiles every tile re-reads and re-splits B, and one splitting pass over
-    // B is faster.
-    return (g.m + 127) / 128 <= (dtype_complex(dt) ? 8 : 2) ? GEMM_PRESPLIT_A : GEMM_PRESPLIT;
+    // Up to eight row tiles of A (32 rows for m <= 32, else 128): A split by a
+    // pass, B read in its own precision and split while staging (measured on
+    // gfx950, complex64: m=512 k=128 n=32768 x256 20.5 ms against 22.4 for a
+    // splitting pass over B and the pre-split GEMM, m=512 k=512 n=8192 15.6
+    // against 17.1, m=1024 k=512 n=8192 x256 25.2 against 27.0; the uneven float32
+    // workload 571 ms with two row tiles at most, 568 with eight). With more row
+    // tiles every tile re-reads and re-splits B, and one splitting pass over B is
+    // faster. (Long k, whose raw-B row table does not fit LDS, runs pre-split B.)
+    return (g.m + 127) / 128 <= 8 ? GEMM_PRESPLIT_A : GEMM_PRESPLIT;
   }
   if (mfma_worth(dt, g)) return native_mfma_kernel(dt, g.m);
   if (splitk_worth(g)) return GEMM_SPLITK;
