@@ -151,15 +151,19 @@ bool gather_tables(const std::vector<size_t>& shape, const std::vector<size_t>& 
     size_t len = 1;
     for (size_t d = d0; d < d1; ++d) len *= shape[perm[d]];
     out.assign(len, 0);
-    for (size_t e = 0; e < len; ++e) {  // row-major decode over output axes d0..d1-1
-      size_t rem = e;
-      int64_t off = 0;
-      for (size_t d = d1; d-- > d0;) {
-        const size_t dim = shape[perm[d]];
-        off += (int64_t)(rem % dim) * st[perm[d]];
-        rem /= dim;
-      }
+    // row-major enumeration over output axes d0..d1-1 (last fastest), the offset
+    // carried like an odometer
+    std::vector<size_t> idx(d1 - d0, 0);
+    int64_t off = 0;
+    for (size_t e = 0; e < len; ++e) {
       out[e] = (int32_t)off;
+      for (size_t q = d1 - d0; q-- > 0;) {
+        const size_t a = perm[d0 + q];
+        off += st[a];
+        if (++idx[q] < shape[a]) break;
+        off -= st[a] * (int64_t)shape[a];
+        idx[q] = 0;
+      }
     }
   };
   table(0, n_con, rows);

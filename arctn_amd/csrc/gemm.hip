@@ -101,17 +101,34 @@ __device__ inline int64_t b_index(const GemmArgs& g, int64_t p, int64_t j) {
   return g.bcol ? (int64_t)g.brow[p] + (int64_t)g.bcol[j] : p * g.ldb + j;
 }
 // Column offset of a factored B (GemmArgs::bcg > 0); the column count is below
-// 2^32, so the digits come from 32-bit divisions (64-bit ones cost the
-// streaming kernel several times its multiply-adds).
+// 2^32, so the digits come from 32-bit divisions by multiply-high (divisions by
+// a runtime divisor cost the streaming kernel about as much as its multiply-adds).
 __device__ inline int64_t fac_col(const GemmArgs& g, int64_t j) {
   uint32_t r = (uint32_t)j;
   int64_t off = 0;
   for (int q = 0; q < g.bcg; ++q) {
-    const uint32_t d = (uint32_t)g.bcd[q], t = r / d;
-    off += (int64_t)(r - t * d) * g.bcs[q];
+    uint32_t t = 0;
+    if (g.bcl[q]) {
+      const uint32_t h = __umulhi(g.bcm[q], r);
+      t = (h + ((r - h) >> 1)) >> (g.bcl[q] - 1);
+    }
+    off += (int64_t)(r - t * (uint32_t)g.bcd[q]) * g.bcs[q];
     r = t;
   }
   return off;
+}
+
+// Granlund-Montgomery round-up constants: l = ceil(log2 d), mul = floor(2^32 (2^l - d)
+// / d) + 1 (below 2^32), exact for every 32-bit dividend; d = 1 gets l = 0.
+void fast_div_u32(uint32_t d, uint32_t& mul, int32_t& shift) {
+  if (d <= 1) {
+    mul = 0, shift = 0;
+    return;
+  }
+  int l = 0;
+  while (((uint64_t)1 << l) < d) ++l;
+  mul = (uint32_t)((((uint64_t)1 << 32) * (((uint64_t)1 << l) - d)) / d + 1);
+  shift = l;
 }
 // The two halves of b_index, for loaders whose column stays fixed across k tiles.
 __device__ inline int64_t b_row(const GemmArgs& g, int64_t p) { return g.bcol ? (int64_t)g.brow[p] : p * g.ldb; }
@@ -547,9 +564,10 @@ template <> struct Mfma<double> {
 //   the f64 instruction.
 // TRACK: fold the result max into g.cmax (a separate instantiation, so the
 // plain kernel carries no extra live register or epilogue work). GATHER: B is
-// read through the gather tables (a separate instantiation: the row-offset
-// loads in the k loop cost the row-major kernel a fifth of its rate).
-template <class F, bool CPLX, int BM, int BN, int BK, bool TRACK, bool GATHER>
+// read through the gather tables (1) or a factored column offset with int64 row
+// offsets (2, GemmArgs::bcg), 0 row-major (separate instantiations: the
+// row-offset loads in the k loop cost the row-major kernel a fifth of its rate).
+template <class F, bool CPLX, int BM, int BN, int BK, bool TRACK, int GATHER>
 __global__ __launch_bounds__(256) void gemm_mfma_kernel(GemmArgs g) {
   constexpr int WM = BM / 2, WN = BN / 2;
   constexpr int TM = WM / 16, TN = WN / 16;
@@ -587,7 +605,7 @@ __global__ __launch_bounds__(256) void gemm_mfma_kernel(GemmArgs g) {
   // ahead of their B loads. Vector loads: a scalar load in flight would hold up
   // the LDS waits of the MFMA loop.
   struct RowOffs {
-    int32_t v[B_PER];
+    typename std::conditional<GATHER == 2, int64_t, int32_t>::type v[B_PER];
   };
   auto rows_at = [&](int64_t k0) {
     RowOffs r{};
@@ -596,7 +614,7 @@ __global__ __launch_bounds__(256) void gemm_mfma_kernel(GemmArgs g) {
       for (int u = 0; u < B_PER; ++u) {
         const int e = tid + 256 * u;
         const int64_t gr = CPLX ? k0 / 2 + e / (BN / 2) : k0 + e / BN;
-        r.v[u] = gr < (CPLX ? kend / 2 : kend) ? g.brow[gr] : 0;
+        r.v[u] = gr < (CPLX ? kend / 2 : kend) ? (GATHER == 2 ? g.brow64[gr] : g.brow[gr]) : 0;
       }
     return r;
   };
@@ -625,7 +643,7 @@ __global__ __launch_bounds__(256) void gemm_mfma_kernel(GemmArgs g) {
       // every B element a thread loads lies in one column (256 is a multiple of BN)
       const int64_t bcn = CPLX ? n0 / 2 + tid % (BN / 2) : n0 + tid % BN;
       const bool bcn_ok = bcn < (CPLX ? g.n : N);
-      const int64_t bco = bcn_ok ? (GATHER ? (int64_t)g.bcol[bcn] : bcn) : 0;
+      const int64_t bco = !bcn_ok ? 0 : GATHER == 2 ? fac_col(g, bcn) : GATHER ? (int64_t)g.bcol[bcn] : bcn;
       auto load = [&](int64_t k0, const RowOffs& ro) {
 #pragma unroll
         for (int u = 0; u < A_PER; ++u) {
@@ -1912,6 +1930,7 @@ static void dispatch_dt(int dt, F&& f) {
 static unsigned zgrid(const GemmArgs& g) { return (unsigned)std::min<int64_t>(g.nbo * g.nbi, 65535); }
 
 static void run_simple(int dt, const GemmArgs& g, hipStream_t s, int opa = 0, int opb = 0) {
+  if (g.bcg && g.k > 0) throw Error("the simple GEMM kernel does not read a factored B operand");
   dispatch_dt(dt, [&](auto tag) {
     using T = decltype(tag);
     uint64_t mn = (uint64_t)g.m * g.n;
@@ -1945,12 +1964,19 @@ static void run_mfma(int dt, const GemmArgs& g0, hipStream_t s) {
   g_last_kind = sizeof(F) == 4 ? K_GEMM_MFMA_F32 : K_GEMM_MFMA_F64;
   // with split k the reduce kernel tracks the result max
   const bool track = g.cmax && !g.part;
-  if (g.bcol) {
-    if (track) LAUNCH((gemm_mfma_kernel<F, CPLX, BM, BN, BK, true, true>), grid, dim3(256), 0, s, g);
-    else LAUNCH((gemm_mfma_kernel<F, CPLX, BM, BN, BK, false, true>), grid, dim3(256), 0, s, g);
+  if (g.bcg) {
+    if constexpr (sizeof(F) == 4) {  // factored B: float32 / complex64 only
+      if (track) LAUNCH((gemm_mfma_kernel<F, CPLX, BM, BN, BK, true, 2>), grid, dim3(256), 0, s, g);
+      else LAUNCH((gemm_mfma_kernel<F, CPLX, BM, BN, BK, false, 2>), grid, dim3(256), 0, s, g);
+    } else {
+      throw Error("a factored B operand is read in float32 / complex64 only");
+    }
+  } else if (g.bcol) {
+    if (track) LAUNCH((gemm_mfma_kernel<F, CPLX, BM, BN, BK, true, 1>), grid, dim3(256), 0, s, g);
+    else LAUNCH((gemm_mfma_kernel<F, CPLX, BM, BN, BK, false, 1>), grid, dim3(256), 0, s, g);
   } else {
-    if (track) LAUNCH((gemm_mfma_kernel<F, CPLX, BM, BN, BK, true, false>), grid, dim3(256), 0, s, g);
-    else LAUNCH((gemm_mfma_kernel<F, CPLX, BM, BN, BK, false, false>), grid, dim3(256), 0, s, g);
+    if (track) LAUNCH((gemm_mfma_kernel<F, CPLX, BM, BN, BK, true, 0>), grid, dim3(256), 0, s, g);
+    else LAUNCH((gemm_mfma_kernel<F, CPLX, BM, BN, BK, false, 0>), grid, dim3(256), 0, s, g);
   }
   if (ks.n_split > 1) {
     SplitK sk{};
@@ -2291,7 +2317,9 @@ static void split3m_launch(const GemmArgs& g, const PreSplit& ps, dim3 grid, hip
 static int launch_gemm_kernels(int dt, int prec, int kernel, const GemmArgs& g0, hipStream_t s) {
   GemmArgs g = g0;
   if (kernel == GEMM_AUTO) kernel = auto_kernel(dt, prec, g);
-  if (g.bcg) kernel = GEMM_SKINNY;     // factored B: read by the streaming kernel only
+  // factored B: read by the FP32 MFMA kernels (the explicit row tiles) and the streaming kernel
+  if (g.bcg && kernel != GEMM_SKINNY && !(kernel >= GEMM_MFMA_R32 && kernel <= GEMM_MFMA_R128))
+    kernel = (dt == F32 || dt == C64) && mfma_worth(dt, g) ? native_mfma_kernel(dt, g.m) : GEMM_SKINNY;
   if (g.k == 0) kernel = GEMM_SIMPLE;  // empty sums: every kernel but this one assumes k tiles
   // A through gather tables: read by the simple kernel and by the split passes of
   // the pre-split kernels (one batch entry); any other choice runs the simple kernel
@@ -2313,7 +2341,7 @@ static int launch_gemm_kernels(int dt, int prec, int kernel, const GemmArgs& g0,
     run_simple(dt, g, s);
     return GEMM_SIMPLE;
   }
-  if (prec == C3M) {  // complex64 MFMA-shaped steps as three real products
+  if (prec == C3M && !g.bcg) {  // complex64 MFMA-shaped steps as three real products
     if (dt != C64) throw Error("precision c3m applies to complex64 only");
     run_c3m(g, s);
     return GEMM_MFMA;
@@ -2484,6 +2512,21 @@ bool split_products_apply(int dt, int prec, int64_t m, int64_t k, int64_t n) {
   GemmArgs g{};
   g.m = m, g.k = k, g.n = n;
   return split_precision(prec) && (dt == F32 || dt == C64) && split_worth(dt, g);
+}
+
+bool gemm_reads_factored(int dt, int prec, int64_t m, int64_t k, int64_t n) {
+  GemmArgs g{};
+  g.m = m, g.k = k, g.n = n, g.nbo = g.nbi = 1;
+  if (!(dt == F32 || dt == C64) || n > (int64_t)UINT32_MAX) return false;
+  const int kern = auto_kernel(dt, prec, g);
+  if (kern == GEMM_SKINNY) return n >= 64 && m <= (dtype_complex(dt) ? 32 : 64);
+  return kern >= GEMM_MFMA_R32 && kern <= GEMM_MFMA_R128 && prec != C3M;
+}
+
+bool gemm_auto_streams(int dt, int prec, int64_t m, int64_t k, int64_t n) {
+  GemmArgs g{};
+  g.m = m, g.k = k, g.n = n, g.nbo = g.nbi = 1;
+  return auto_kernel(dt, prec, g) == GEMM_SKINNY && n >= 64 && m <= (dtype_complex(dt) ? 32 : 64);
 }
 
 bool gemm_runs_simple(int dt, int prec, int kernel, const GemmArgs& g) {

@@ -336,6 +336,10 @@ bool factor_columns(const std::vector<size_t>& shape, const std::vector<size_t>&
       ++f.bcg;
     }
   }
+  for (int q = 0; q < f.bcg; ++q) {
+    if (f.bcd[q] > (int64_t)UINT32_MAX) return false;
+    fast_div_u32((uint32_t)f.bcd[q], f.bcm[q], f.bcl[q]);
+  }
   return f.bcg > 0;
 }
 
@@ -460,12 +464,23 @@ PlanProg build_plan(const Plan& P, int dt, bool strip, int prec, size_t n_rounds
           b_perm[q] = bc[q];
         }
       }
-      gather_b = gather_tables(shapes[st.ri], b_perm, n_con, rows, cols, col_run, row_run);
-      // offsets beyond int32 (operands of more than 2^31 elements): B's columns as a
-      // mixed-radix function of the column index when its free legs keep their
-      // memory order, rows in int64, for the streaming kernel (short A)
-      if (!gather_b && st.m <= (dtype_complex(dt) ? 32u : 64u) && st.n >= 64 && st.n <= UINT32_MAX &&
-          factor_columns(shapes[st.ri], b_perm, n_con, fac)) {
+      // B's columns as a mixed-radix function of the column index (its free legs
+      // kept in memory order; rows in int64; no column table, any operand size)
+      // where the streaming kernel runs the product with many columns, and where the
+      // int32 gather tables (at most 2^26 entries) would not do and the heuristic
+      // runs the product on the FP32 MFMA or the streaming kernel (float32 /
+      // complex64; the split and 3M forms read tables or row-major B); else B is
+      // permuted.
+      const bool can_stream = (dt == F32 || dt == C64) && st.m <= (dtype_complex(dt) ? 32u : 64u) && st.n >= 64 &&
+                              st.n <= UINT32_MAX;
+      const bool table_ok = st.k + st.n <= (1u << 26);
+      bool factored = can_stream && st.n > (1u << 20) &&
+                      gemm_auto_streams(dt, prec, (int64_t)st.m, (int64_t)st.k, (int64_t)st.n) &&
+                      factor_columns(shapes[st.ri], b_perm, n_con, fac);
+      gather_b = !factored && table_ok && gather_tables(shapes[st.ri], b_perm, n_con, rows, cols, col_run, row_run);
+      if (!factored && !gather_b && gemm_reads_factored(dt, prec, (int64_t)st.m, (int64_t)st.k, (int64_t)st.n))
+        factored = factor_columns(shapes[st.ri], b_perm, n_con, fac);
+      if (factored) {
         fac_rows.clear();
         const auto& sh = shapes[st.ri];
         std::vector<int64_t> str(sh.size(), 1);
@@ -490,9 +505,9 @@ PlanProg build_plan(const Plan& P, int dt, bool strip, int prec, size_t n_rounds
     // split pass, which then does the permute's work
     std::vector<int32_t> arows, acols;
     int32_t arun = 1, crun = 1;
-    const bool gather_a = large && st.bt == 1 && st.sum_a.empty() &&
+    const bool gather_a = large && st.bt == 1 && st.sum_a.empty() && !fac.bcg &&
                           split_products_apply(dt, prec, (int64_t)st.m, (int64_t)st.k, (int64_t)st.n) &&
-                          plan_permute(shapes[st.li], a_perm, 1, es).kind != 0 &&
+                          plan_permute(shapes[st.li], a_perm, 1, es).kind != 0 && st.m + st.k <= (1u << 24) &&
                           gather_tables(shapes[st.li], a_perm, a_perm.size() - n_con, arows, acols, crun, arun);
     GRef a = gather_a ? slot[st.li] : operand(st.li, st.sum_a, a_perm);
     GRef b = gather_b || fac.bcg ? slot[st.ri] : operand(st.ri, st.sum_b, b_perm);
@@ -520,6 +535,8 @@ PlanProg build_plan(const Plan& P, int dt, bool strip, int prec, size_t n_rounds
         g.bcg = fac.bcg;
         std::copy(fac.bcd, fac.bcd + 6, g.bcd);
         std::copy(fac.bcs, fac.bcs + 6, g.bcs);
+        std::copy(fac.bcm, fac.bcm + 6, g.bcm);
+        std::copy(fac.bcl, fac.bcl + 6, g.bcl);
         sg.brow64 = (int64_t)prog.tables64.size();
         prog.tables64.insert(prog.tables64.end(), fac_rows.begin(), fac_rows.end());
       }

@@ -135,11 +135,16 @@ struct GemmArgs {
   // of the run length (B's contracted legs innermost in memory)
   int32_t brow_run = 1;
   // B columns as a mixed-radix function of the column index, for operands too
-  // large for the int32 gather tables (read by the streaming kernel only): column
+  // large for the int32 gather tables: column
   // j sits at sum_q ((j / (bcd[0] ... bcd[q-1])) % bcd[q]) * bcs[q] elements (group
-  // 0 innermost, bcg groups), row p at brow64[p]
+  // 0 innermost, bcg groups), row p at brow64[p]; read by the streaming and the
+  // FP32 MFMA kernels (float32 / complex64)
   int32_t bcg = 0;
   int64_t bcd[6] = {}, bcs[6] = {};
+  // division by bcd[q] as a multiply-high (round-up method): q = (h + ((j - h) >> 1))
+  // >> (bcl[q] - 1) with h = umulhi(bcm[q], j); bcl[q] = 0 for bcd[q] = 1
+  uint32_t bcm[6] = {};
+  int32_t bcl[6] = {};
   const int64_t* brow64 = nullptr;
   // FP16X2: max |re|, |im| of A / of B as double bits when already known (folded
   // into the epilogue of the GEMM that produced the operand); null = measured by
@@ -170,6 +175,14 @@ enum GemmKernel {
 int launch_gemm(int dt, int prec, int kernel, const GemmArgs& g, hipStream_t s);
 // Device scratch launch_gemm can use for this GEMM (FP16X2 scales, split-K partials).
 size_t gemm_scratch_bytes(int dt, int prec, int kernel, const GemmArgs& g);
+// The multiply-high constants of GemmArgs::bcm / bcl for dividing 32-bit values by d.
+void fast_div_u32(uint32_t d, uint32_t& mul, int32_t& shift);
+// Whether the heuristic runs an unbatched m x k x n product on a kernel that reads
+// a factored B (GemmArgs::bcg): float32 / complex64 on the FP32 MFMA kernels (not
+// the split or 3M forms) or the streaming kernel.
+bool gemm_reads_factored(int dt, int prec, int64_t m, int64_t k, int64_t n);
+// Whether the heuristic runs an unbatched m x k x n product on the streaming kernel.
+bool gemm_auto_streams(int dt, int prec, int64_t m, int64_t k, int64_t n);
 // Whether this GEMM runs on the simple kernel under kernel choice `kernel`.
 bool gemm_runs_simple(int dt, int prec, int kernel, const GemmArgs& g);
 // Whether the heuristic choice for an m x k x n product (element units) runs

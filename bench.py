@@ -444,7 +444,8 @@ def bench_grouped(args):
                           "instances_within_bound": n_ok, "max_err_over_bound": max(e / b for e, b in zip(errs, bound)),
                           "within_bar": ok}
         log(f"[bench] {prec}: {ms:.1f} ms/step, {flops / ms / 1e9:.2f} TFLOP/s, {rep['n_launches']} launches, "
-            f"max rel err {max(errs):.2e} (median {np.median(errs):.1e}), {n_ok}/{len(errs)} within bound")
+            f"max rel err {max(errs):.2e} (median {np.median(errs):.1e}), {n_ok}/{len(errs)} within bound, "
+            f"built in {build_s:.1f} s, workspace {rep['workspace_bytes'] / 1e9:.2f} GB")
         if ok and (best is None or ms < variants[best]["ms"]):
             best = prec
         del G
