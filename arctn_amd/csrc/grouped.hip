@@ -264,7 +264,7 @@ struct Single {  // an op launched on its own
   // GEMM: B read through gather tables instead of a permute (offsets into the
   // plan's table vector, then byte offsets into the group's descriptor blob), or -1
   int64_t brow = -1, bcol = -1;
-  // GEMM: B too large for int32 tables, read through a factored column offset
+  // GEMM: B read through a factored column offset instead of a permute
   // (gemm.bcg groups) and int64 row offsets: offset into the plan's int64 table
   // vector, then a byte offset into the group's blob, or -1
   int64_t brow64 = -1;
