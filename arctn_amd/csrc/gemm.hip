@@ -1212,15 +1212,16 @@ __global__ __launch_bounds__(256) void split_a_kernel(GemmArgs g, PreSplit ps) {
   }
 }
 
-// B [k][n] -> transposed [zb][n][Kp] hi/lo through a TK_SPLIT_BT (k) x 64 (n) LDS
-// tile (measured on gfx950: a 64-deep tile ran the complex64 k=512 n=8192 x256
-// pass in 6.3 ms against 4.3 for 32).
-constexpr int TK_SPLIT_BT = 32;
+// B [k][n] -> transposed [zb][n][Kp] hi/lo through a tk_split_bt (k) x 64 (n) LDS
+// tile: 64 halves of k per tile, so that a column's piece of each output array is
+// one whole 128-byte line (measured on gfx950: a 64-deep complex64 tile ran the
+// k=512 n=8192 x256 pass in 6.3 ms against 4.3 for 32).
+constexpr int tk_split_bt(bool cplx) { return cplx ? 32 : 64; }
 template <bool CPLX, bool F16>
 __global__ __launch_bounds__(256) void split_bt_kernel(GemmArgs g, PreSplit ps) {
   using H = typename Half<F16>::H;
   using E = typename std::conditional<CPLX, float2, float>::type;
-  constexpr int TK = TK_SPLIT_BT, TN = 64, EH = CPLX ? 2 : 1;  // halves per element
+  constexpr int TK = tk_split_bt(CPLX), TN = 64, EH = CPLX ? 2 : 1;  // halves per element
   __shared__ E tile[TK][TN + 1];
   const int64_t kp = ps.kp, inner = g.sbi ? g.nbi : 1, nm = n_mats(g.sbo, g.sbi, g.nbo, g.nbi);
   const int64_t tk = (kp / EH + TK - 1) / TK, tn = (g.n + TN - 1) / TN, total = nm * tk * tn;
@@ -2386,7 +2387,7 @@ static int launch_gemm_kernels(int dt, int prec, int kernel, const GemmArgs& g0,
       ps.bhi = raw_b ? nullptr : base + 2 * psz.a_half, ps.blo = raw_b ? nullptr : base + 2 * psz.a_half + psz.b_half;
       ps.kp = psz.kp;
       const int64_t na = n_mats(g.sao, g.sai, g.nbo, g.nbi) * g.m * psz.kp / 4;
-      const int64_t tiles_b = n_mats(g.sbo, g.sbi, g.nbo, g.nbi) * ((psz.kp / (cx ? 2 : 1) + TK_SPLIT_BT - 1) / TK_SPLIT_BT) * ((g.n + 63) / 64);
+      const int64_t tiles_b = n_mats(g.sbo, g.sbi, g.nbo, g.nbi) * ((psz.kp / (cx ? 2 : 1) + tk_split_bt(cx) - 1) / tk_split_bt(cx)) * ((g.n + 63) / 64);
       const dim3 ga((unsigned)std::min<int64_t>((na + 255) / 256, 65536)), gb((unsigned)std::min<int64_t>(tiles_b, 65536));
 #define PRESPLIT_LAUNCH(C_, F_, R_)                                                                        \
   do {                                                                                                     \
