@@ -291,6 +291,11 @@ int atn_tensor_op(int dtype, int op, size_t rank, const size_t* shape, const siz
         launch_gather(es, g, idx, 1, d_in, d_out, s);
         break;
       }
+      case 5: {  // max magnitude: one double (0 for an empty or all-zero tensor)
+        launch_fill_zero(d_out, sizeof(double), s);
+        if (prod(sh) > 0) launch_maxabs(dtype, d_in, prod(sh), static_cast<uint64_t*>(d_out), s);
+        break;
+      }
       default: throw Error("unknown op");
     }
   });

@@ -673,18 +673,45 @@ void launch_fill_zero(void* p, size_t bytes, hipStream_t s) {
 // Power-of-two normalisation: multiplying by 2^-e is exact, so the mantissa
 // times 2^exponent reproduces the unstripped value bit for bit (barring
 // underflow); the reference divides by the exact max magnitude instead.
-template <class T>
-__global__ void maxabs_kernel(const T* __restrict__ x, uint64_t n, unsigned long long* maxbits) {
-  double m = 0.0;
-  for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (uint64_t)gridDim.x * blockDim.x) {
-    double v = Arith<T>::mag(x[e]);
+// Max magnitude (max(|re|, |im|) for complex elements, the max over every real
+// component) as double bits, atomically raised. S: the component type. Components
+// go through 16-byte loads (four in flight per thread) where x is 16-byte aligned,
+// the tail (or the whole array otherwise) one component at a time; the max is
+// taken in S (exact) and the workgroup's max raises *maxbits once.
+template <class S>
+__global__ __launch_bounds__(256) void maxabs_kernel(const S* __restrict__ x, uint64_t ns, unsigned long long* maxbits) {
+  using V = typename std::conditional<sizeof(S) == 4, float4, double2>::type;
+  constexpr uint64_t W = 16 / sizeof(S);
+  S m = 0;
+  auto take = [&](S v) {
+    v = fabs(v);
     m = v > m ? v : m;  // NaN never wins; a NaN result stays unscaled
+  };
+  auto take_v = [&](const V& v) {
+    if constexpr (W == 4) take(v.x), take(v.y), take(v.z), take(v.w);
+    else take(v.x), take(v.y);
+  };
+  const uint64_t nv = (uintptr_t)x % 16 == 0 ? ns / W : 0;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x, t0 = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const V* xv = reinterpret_cast<const V*>(x);
+  uint64_t i = t0;
+  for (; i + 3 * stride < nv; i += 4 * stride) {
+    const V a = xv[i], b = xv[i + stride], c = xv[i + 2 * stride], d = xv[i + 3 * stride];
+    take_v(a), take_v(b), take_v(c), take_v(d);
   }
+  for (; i < nv; i += stride) take_v(xv[i]);
+  for (uint64_t e = nv * W + t0; e < ns; e += stride) take(x[e]);
   for (int o = 32; o > 0; o >>= 1) {
-    double t = __shfl_down(m, o, 64);
+    const S t = __shfl_down(m, o, 64);
     m = t > m ? t : m;
   }
-  if ((threadIdx.x & 63) == 0 && m > 0.0) atomicMax(maxbits, (unsigned long long)__double_as_longlong(m));
+  __shared__ S wmax[4];
+  if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < (int)(blockDim.x / 64); ++w) m = wmax[w] > m ? wmax[w] : m;
+    if (m > 0) atomicMax(maxbits, (unsigned long long)__double_as_longlong((double)m));
+  }
 }
 
 // scaled_max (optional): receives the max after the rescale (max * 2^-e, or the
@@ -751,7 +778,11 @@ void launch_maxabs(int dt, const void* x, size_t numel, uint64_t* maxbits, hipSt
   dispatch_dt(dt, [&](auto tag) {
     using T = decltype(tag);
     g_last_kind = K_STRIP;
-    LAUNCH(maxabs_kernel<T>, grid_for(numel, 256, 2048), 256, 0, s, (const T*)x, (uint64_t)numel, (unsigned long long*)maxbits);
+    using S = typename std::conditional<std::is_same<T, float>::value || std::is_same<T, float2>::value, float, double>::type;
+    const uint64_t ns = (uint64_t)numel * (dtype_complex(dt) ? 2 : 1);
+    // a quarter of the 16-byte vectors per thread and pass, at most 1024 workgroups
+    LAUNCH(maxabs_kernel<S>, grid_for((ns * sizeof(S) / 16 + 3) / 4, 256, 1024), 256, 0, s, (const S*)x, ns,
+           (unsigned long long*)maxbits);
   });
 }
 void launch_scale_pow2(int dt, void* x, size_t numel, const uint64_t* maxbits, int64_t* expo, hipStream_t s, int keep,

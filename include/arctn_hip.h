@@ -207,7 +207,10 @@ int atn_adjoint_backward(atn_adjoint* a, const void* const* d_inputs, const void
 /* Element-level kernels, exported for parity tests of the data-movement path
  * (src/tensor.rs:453-669): op 0 permute(arg = axes), 1 sum_axis(arg[0]),
  * 2 trace_pair(arg[0], arg[1]), 3 diag_pair(arg[0], arg[1]),
- * 4 select_axis(arg[0], arg[1]). Device pointers; output shape implied. */
+ * 4 select_axis(arg[0], arg[1]), 5 max magnitude (max over the elements of
+ * max(|re|, |im|); NaN components never win) as one double in d_out, the value
+ * behind the power-of-two rescales of the strip and the FP16 operand scales
+ * (no reference counterpart). Device pointers; output shape implied. */
 int atn_tensor_op(int dtype, int op, size_t rank, const size_t* shape, const size_t* arg, size_t n_arg,
                   const void* d_in, void* d_out, void* stream);
 

@@ -1,7 +1,8 @@
 """Kernel-level parity of the HIP data-movement and GEMM kernels (GPU).
 
 Data movement (permute, sum_axis, trace_pair, diag_pair, select_axis) must be
-bit-identical to the oracle's restatement of src/tensor.rs:453-669. GEMMs are
+bit-identical to the oracle's restatement of src/tensor.rs:453-669; the max
+magnitude behind the power-of-two rescales exact against NumPy. GEMMs are
 checked against float64/complex128 NumPy products with a precision-scaled
 tolerance.
 """
@@ -78,6 +79,46 @@ def test_reductions_and_gathers_bit_exact(cuda, dtype):
     want = O.tensor_op("diag_pair", b, 1, 3)
     assert np.array_equal(want, np.moveaxis(np.diagonal(b, axis1=1, axis2=3), -1, 1))
     assert _gpu_op(3, b, [1, 3], want.shape).tobytes() == want.tobytes()
+
+
+@pytest.mark.parametrize("dtype", DTYPES)
+def test_max_magnitude(cuda, dtype):
+    """Op 5: max over the elements of max(|re|, |im|), the value the strip's power-of-
+    two rescales and the FP16 operand scales start from: exact against NumPy over
+    sizes around the 16-byte vector loads and the grid, 16-byte aligned and
+    unaligned starts, the max planted first and last, NaN components never winning."""
+    import torch
+    from arctn_amd import _lib
+
+    L = _lib.lib()
+    code = {np.dtype(np.float32): 0, np.dtype(np.float64): 1, np.dtype(np.complex64): 2,
+            np.dtype(np.complex128): 3}[np.dtype(dtype)]
+    rng = np.random.default_rng(5)
+    out = torch.empty(1, dtype=torch.float64, device="cuda")
+
+    def gpu(t):
+        sh = (C.c_size_t * 1)(t.numel())
+        _lib.check(L.atn_tensor_op(code, 5, 1, sh, None, 0, t.data_ptr(), out.data_ptr(), None))
+        torch.cuda.synchronize()
+        return float(out.cpu())
+
+    def want(a):
+        r = a.view(a.real.dtype) if np.iscomplexobj(a) else a
+        return float(np.nanmax(np.abs(r))) if r.size else 0.0
+
+    for n in (1, 2, 3, 4, 5, 7, 8, 17, 1000, 4099, (1 << 20) + 3, (1 << 24) + 1):
+        a = _rand((n + 1,), dtype, rng)
+        for where in (0, n // 2, n):
+            b = a.copy()
+            b[where] = b[where] * 64
+            t = torch.from_numpy(b).cuda()
+            for lo in (0, 1):  # element 1 on: not 16-byte aligned
+                assert gpu(t[lo:]) == want(b[lo:]), (n, where, lo)
+    assert gpu(torch.zeros(33, dtype=torch.from_numpy(np.zeros(1, dtype)).dtype, device="cuda")) == 0.0
+    a = _rand((4097,), dtype, rng)
+    a[100] = np.nan
+    a[4000] = 1000.0
+    assert gpu(torch.from_numpy(a).cuda()) == 1000.0
 
 
 GEMM_SHAPES = [(1, 1, 1, 1), (1, 3, 4, 5), (2, 17, 33, 9), (1, 128, 64, 128), (3, 130, 70, 129), (1, 256, 256, 256),
