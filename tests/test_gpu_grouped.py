@@ -139,3 +139,34 @@ def test_grouped_split_precision(cuda, precision):
             scale = np.abs(ref).max()
             bound = max(2e-5 * scale, 4 * np.abs(cpu32 - ref).max())
             assert np.abs(got - ref).max() <= bound, (precision, strip, np.abs(got - ref).max() / scale)
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.complex64])
+@pytest.mark.parametrize("shape", ["stream", "mfma"])
+def test_grouped_factored_b_matches_matmul(cuda, dtype, shape):
+    """One product whose B has its contracted legs interleaved with its free legs in
+    memory ([f1, k1, f2, k2, f3]), read in place through a factored column offset:
+    by the streaming kernel (m = 4, n = 2^21) and by the FP32 MFMA kernel (m = k = 16,
+    n = 2^26: int32 gather tables would pass 2^26 entries). Checked against torch's
+    matmul of the permuted operands on the device (FP32 products, k <= 16)."""
+    import torch
+    from arctn_amd import GroupedContraction
+
+    m, k1, k2, f = (4, 2, 2, (32, 64, 1024)) if shape == "stream" else (16, 4, 4, (64, 1024, 1024))
+    net = {"inputs": [[0, 1, 2], [3, 1, 4, 2, 5]], "output": [0, 3, 4, 5],
+           "size_dict": {0: m, 1: k1, 2: k2, 3: f[0], 4: f[1], 5: f[2]}}
+    tdt = torch.complex64 if dtype == np.complex64 else torch.float32
+    gen = torch.Generator(device="cuda").manual_seed(21)
+    a = torch.randn((m, k1, k2), dtype=tdt, device="cuda", generator=gen)
+    b = torch.randn((f[0], k1, f[1], k2, f[2]), dtype=tdt, device="cuda", generator=gen)
+    plan = Plan(net["inputs"], net["output"], net["size_dict"], [(0, 1)])
+    g = GroupedContraction([plan], dtype, strip=False)
+    (got,), (e,) = g.run([[a, b]])
+    assert e == 0.0
+    ref = torch.matmul(a.reshape(m, k1 * k2), b.permute(1, 3, 0, 2, 4).contiguous().reshape(k1 * k2, -1))
+    ref = ref.reshape(got.shape)
+    err = float((got - ref).abs().max()) / float(ref.abs().max())
+    print(f"factored B {shape} {np.dtype(dtype).name}: rel err {err:.2e}", flush=True)
+    assert err <= 1e-5
+    del g, got, ref, b
+    torch.cuda.empty_cache()
