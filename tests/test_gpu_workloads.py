@@ -114,9 +114,10 @@ def test_circuit_6x6_output_blocks_vs_oracle_fixture(cuda, form):
 
 def test_uneven_512_instances_vs_float64_oracle(cuda):
     """Every one of the 512 uneven 3-regular networks (n in [40, 120], bond dims
-    {2, 3, 4}), grouped execution in float32 with the strip, against the float64
-    CPU oracle, each within its bar (fixtures.uneven_bounds: max(1e-5, 16 x the
-    root mean square of four CPU float32 errors of the instance))."""
+    {2, 3, 4}), grouped execution in float32 with the strip, FP32 products and the
+    FP16x2 split form, against the float64 CPU oracle, each within its bar
+    (fixtures.uneven_bounds: max(1e-5, 16 x the root mean square of four CPU float32
+    errors of the instance))."""
     import torch
     from arctn_amd import GroupedContraction, Plan
     from arctn_amd.plans.make_plans import load_uneven
@@ -132,12 +133,15 @@ def test_uneven_512_instances_vs_float64_oracle(cuda):
     insts, _ = load_uneven("uneven_3regular_512")
     plans = [Plan(n["inputs"], n["output"], n["size_dict"], p) for n, _, p in insts]
     arrs = [[torch.from_numpy(x).cuda() for x in a] for _, a, _ in insts]
-    g = GroupedContraction(plans, np.float32, strip=True, precision="native")
-    outs, exps = g.run(arrs)
-    bad = []
-    for i, (o, e) in enumerate(zip(outs, exps)):
-        ref = doc["f64"][i]
-        err = abs(float(o.cpu()) * 10.0 ** e - ref) / abs(ref)
-        if not err <= bounds[i]:
-            bad.append((i, err, bounds[i]))
-    assert not bad, bad[:10]
+    for prec in ("native", "fp16x2"):  # FP32 products; the benchmark's form (split large products)
+        print(f"uneven 512 precision {prec}", flush=True)
+        g = GroupedContraction(plans, np.float32, strip=True, precision=prec)
+        outs, exps = g.run(arrs)
+        bad = []
+        for i, (o, e) in enumerate(zip(outs, exps)):
+            ref = doc["f64"][i]
+            err = abs(float(o.cpu()) * 10.0 ** e - ref) / abs(ref)
+            if not err <= bounds[i]:
+                bad.append((i, err, bounds[i]))
+        assert not bad, (prec, bad[:10])
+        del g, outs
