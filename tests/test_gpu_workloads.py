@@ -8,8 +8,9 @@ make_uneven_fixture.py from the oracle in complex128 / float64):
   blocks of 256 internal slices: internal slices [0, 16) of several blocks in
   every precision, split products forced on every step;
 * 512 uneven random 3-regular networks: every instance's scalar, grouped
-  execution in float32, against float64 with a per-instance bound derived from
-  four CPU float32 contractions of the instance.
+  execution in float32 (FP32 products and the FP16x2 split form), against float64
+  with a per-instance bound derived from four CPU float32 contractions of the
+  instance.
 """
 import json
 import os
