@@ -296,6 +296,52 @@ def _rows(base, rows):
     return _ROWS[0].apply(base, rows)
 
 
+_BRICK_CONSTS = {}
+
+
+def _brickwork_consts(dev, cdt):
+    """ket |0>, CZ, Z and the identity as device tensors, built once per device and dtype."""
+    import torch
+
+    key = (str(dev), cdt)
+    got = _BRICK_CONSTS.get(key)
+    if got is None:
+        got = _BRICK_CONSTS[key] = (
+            torch.tensor([1.0, 0.0], dtype=cdt, device=dev),
+            torch.diag(torch.tensor([1, 1, 1, -1], dtype=cdt, device=dev)).reshape(2, 2, 2, 2),
+            torch.diag(torch.tensor([1.0, -1.0], dtype=cdt, device=dev)),
+            torch.eye(2, dtype=cdt, device=dev))
+    return got
+
+
+def _brickwork_gates(angles):
+    """Every single-qubit gate U = RZ(phi) . RY(theta) of ``angles`` (layers,
+    n_qubits, 2) and its conjugate, as one [2 * layers * n_qubits, 2, 2] table
+    (ket gates first, then bra gates), by a handful of batched torch operations."""
+    import torch
+
+    cdt = torch.complex128 if angles.dtype == torch.float64 else torch.complex64
+    th, ph = angles[..., 0], angles[..., 1]
+    c, s = torch.cos(0.5 * th).to(cdt), torch.sin(0.5 * th).to(cdt)
+    e = torch.polar(torch.ones_like(ph), -0.5 * ph)  # exp(-i phi / 2)
+    ec = e.conj()
+    # RZ . RY = [[e c, -e s], [conj(e) s, conj(e) c]]
+    u = torch.stack([torch.stack([e * c, -e * s], -1), torch.stack([ec * s, ec * c], -1)], -2)
+    return torch.stack([u, u.conj()]).reshape(-1, 2, 2), cdt
+
+
+def brickwork_table(slots, angles):
+    """The doubled brickwork network's inputs for ``autograd.contract_table``:
+    (gate table, index, fixed tensors). Input i is row index[i] of the table (the
+    gate of its slot: ket or bra half, layer, qubit) or fixed tensor -index[i] - 1
+    (|0>, CZ, Z, identity); gradients reach ``angles`` through the table."""
+    table, cdt = _brickwork_gates(angles)
+    layers, nq = angles.shape[0], angles.shape[1]
+    fixed_of = {"ket0": -1, "cz": -2, "z": -3}
+    index = [(sl[3] * layers + sl[1]) * nq + sl[2] if sl[0] == "u" else fixed_of.get(sl[0], -4) for sl in slots]
+    return table, index, _brickwork_consts(angles.device, cdt)
+
+
 def brickwork_torch_tensors(slots, angles):
     """Input tensors of the doubled brickwork network as torch tensors built
     from ``angles`` (shape (layers, n_qubits, 2): RY angle, RZ angle), so that
@@ -309,22 +355,12 @@ def brickwork_torch_tensors(slots, angles):
     whole batch per gate in the backward pass)."""
     import torch
 
-    cdt = torch.complex128 if angles.dtype == torch.float64 else torch.complex64
     dev = angles.device
-    th, ph = angles[..., 0], angles[..., 1]
-    c, s = torch.cos(0.5 * th).to(cdt), torch.sin(0.5 * th).to(cdt)
-    e = torch.polar(torch.ones_like(ph), -0.5 * ph)  # exp(-i phi / 2)
-    ec = e.conj()
-    # RZ . RY = [[e c, -e s], [conj(e) s, conj(e) c]]
-    u = torch.stack([torch.stack([e * c, -e * s], -1), torch.stack([ec * s, ec * c], -1)], -2)
-    uu = torch.stack([u, u.conj()])  # [ket / bra, layer, qubit, 2, 2]
-    layers, nq = u.shape[0], u.shape[1]
+    table, cdt = _brickwork_gates(angles)
+    layers, nq = angles.shape[0], angles.shape[1]
     rows = [(sl[3] * layers + sl[1]) * nq + sl[2] for sl in slots if sl[0] == "u"]
-    gates = iter(_rows(uu.reshape(-1, 2, 2), torch.tensor(rows, device=dev)))
-    ket0 = torch.tensor([1.0, 0.0], dtype=cdt, device=dev)
-    cz = torch.diag(torch.tensor([1, 1, 1, -1], dtype=cdt, device=dev)).reshape(2, 2, 2, 2)
-    z = torch.diag(torch.tensor([1.0, -1.0], dtype=cdt, device=dev))
-    eye = torch.eye(2, dtype=cdt, device=dev)
+    gates = iter(_rows(table, torch.tensor(rows, device=dev)))
+    ket0, cz, z, eye = _brickwork_consts(dev, cdt)
     out = []
     for sl in slots:
         if sl[0] == "ket0":

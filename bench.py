@@ -508,15 +508,17 @@ def bench_grouped(args):
 def bench_gradient(args):
     """20-qubit brickwork <psi|Z0 Z1|psi>, 10 layers of RY.RZ + CZ: a step is one
     complex64 forward plus the hand-written backward to all 400 angles
-    (checkpoint every 8 steps); gradients checked against central differences
-    of a complex128 forward and the CPU oracle's step-by-step gradient."""
+    (checkpoint every 8 steps), the gates handed over as rows of one gate table
+    (autograd.contract_table); gradients checked against the per-tensor form
+    (autograd.contract), central differences of a complex128 forward and the CPU
+    oracle's step-by-step gradient."""
     import numpy as np
     import torch
 
     from arctn_amd import Plan
     from arctn_amd import planner as P
     from arctn_amd import workloads as W
-    from arctn_amd.autograd import contract
+    from arctn_amd.autograd import contract, contract_table
     from oracle import pyoracle as O
 
     nq, layers = 20, 10
@@ -528,24 +530,36 @@ def bench_gradient(args):
     th = torch.tensor(ang, dtype=torch.float32, device="cuda", requires_grad=True)
 
     def step():
+        table, index, fixed = W.brickwork_table(slots, th)
+        val = contract_table(plan, table, index, fixed, checkpoint_every=8).real
+        (g,) = torch.autograd.grad(val, th)
+        return val, g
+
+    def step_tensors():  # every gate its own tensor (autograd.contract)
         val = contract(plan, *W.brickwork_torch_tensors(slots, th), checkpoint_every=8).real
         (g,) = torch.autograd.grad(val, th)
         return val, g
 
-    for _ in range(args.warmup):
-        step()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        val, g32 = step()
-    torch.cuda.synchronize()
-    ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    def timed(fn):
+        for _ in range(args.warmup):
+            fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            val, g = fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) * 1e3 / args.steps, val, g
+
+    ms, val, g32 = timed(step)
     g32 = g32.cpu().numpy()
     log(f"[bench] forward + backward (400 gradients): {ms:.2f} ms/step")
     if args.skip_ref:  # profiler runs: the timed steps only
         print(json.dumps({"metric": "forward + backward time for 400 angle gradients", "ms_per_step": ms,
                           "value": 400.0 / (ms * 1e-3), "unit": "gradients/s", "checks": "skipped"}), flush=True)
         return
+    ms_t, _, g_t = timed(step_tensors)
+    diff_t = float(np.abs(g_t.cpu().numpy() - g32).max())
+    log(f"[bench] the same step with every gate its own tensor: {ms_t:.2f} ms/step; gradients differ by {diff_t:.1e}")
     th64 = torch.tensor(ang, dtype=torch.float64, device="cuda", requires_grad=True)
     (g64,) = torch.autograd.grad(contract(plan, *W.brickwork_torch_tensors(slots, th64)).real, th64)
     g64 = g64.cpu().numpy()
@@ -585,6 +599,8 @@ def bench_gradient(args):
         "config": {"workload": args.workload, "qubits": nq, "layers": layers, "angles": 400, "checkpoint_every": 8,
                    "n_steps": info["n_steps"], "log10_flops": info["log10_flops"]},
         "grad_err_vs_fd": err32, "grad64_err_vs_cpu_oracle": err_cpu, "value_at_angles": float(val.detach()),
+        "inputs": "gate table rows (autograd.contract_table)",
+        "per_tensor_form": {"ms_per_step": round(ms_t, 3), "grad_max_abs_diff": diff_t},
         "cpu_baseline": {"value": 400.0 / cpu_s, "unit": "gradients/s", "cores": 1, "kind": "port",
                          "sample": f"oracle step-by-step complex128 gradient of every input, {cpu_s:.2f} s "
                                    "(the chain to the angles excluded)"},

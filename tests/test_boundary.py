@@ -208,6 +208,34 @@ def test_device_arguments_checked_before_the_abi():
         plan.check_device_args([a, b], 7)
 
 
+def test_contract_table_arguments_checked_before_the_abi():
+    """contract_table checks the index against the plan (count, rows and fixed tensors
+    that exist, row and fixed shapes), the fixed tensors' dtype and the device before
+    any pointer reaches the C ABI; these checks need no GPU."""
+    import torch
+
+    from arctn_amd.autograd import contract_table
+
+    net = {"inputs": [[0, 1], [1, 2], [2]], "output": [0], "size_dict": {0: 2, 1: 2, 2: 2}}
+    plan = Plan(net["inputs"], net["output"], net["size_dict"], [(0, 1), (2, 3)])
+    table = torch.zeros(3, 2, 2, dtype=torch.complex64)
+    v = torch.zeros(2, dtype=torch.complex64)
+    with pytest.raises(ValueError, match="张量个数"):
+        contract_table(plan, table, [0, 1], [v])
+    with pytest.raises(ValueError, match="does not exist"):
+        contract_table(plan, table, [0, 3, -1], [v])
+    with pytest.raises(ValueError, match="does not exist"):
+        contract_table(plan, table, [0, 1, -2], [v])
+    with pytest.raises(ValueError, match="shape"):
+        contract_table(plan, table, [0, 1, 2], [v])  # a 2x2 row where the plan has a vector
+    with pytest.raises(ValueError, match="shape"):
+        contract_table(plan, table, [0, 1, -1], [torch.zeros(3, dtype=torch.complex64)])
+    with pytest.raises(ValueError, match="dtype"):
+        contract_table(plan, table, [0, 1, -1], [v.to(torch.complex128)])
+    with pytest.raises(ValueError, match="GPU"):
+        contract_table(plan, table, [0, 1, -1], [v])
+
+
 def test_brickwork_gate_tensors_and_their_gradient():
     """The torch-built inputs of the doubled brickwork network equal the NumPy
     ones, and the gradient through the batched gate construction (one node hands
