@@ -963,6 +963,12 @@ __device__ inline int pow2_shift(unsigned long long bits) {
   return 15 - e;
 }
 
+// The operands' pending power-of-two exponents (GemmArgs::apend / bpend), divided
+// out of the product with the FP16 operand scales.
+__device__ inline int pending_exp(const GemmArgs& g) {
+  return (g.apend ? (int)*g.apend : 0) + (g.bpend ? (int)*g.bpend : 0);
+}
+
 // Staging: A rows are k-contiguous in LDS (ahi/alo[m][k]); B is stored
 // transposed (bhi/blo[n][k]) so both MFMA fragments are one 16-byte LDS read.
 // VEC: A rows are 16-byte aligned (float4 loads of 4 consecutive k).
@@ -979,11 +985,12 @@ __global__ __launch_bounds__(256, 2) void gemm_split_kernel(GemmArgs g) {
   H* bhi = alo + BM * SK;
   H* blo = bhi + BN * SK;
 
-  float sa = 1.f, sb = 1.f, unscale = 1.f;
+  const int pe = pending_exp(g);
+  float sa = 1.f, sb = 1.f, unscale = ldexpf(1.f, -pe);
   float mx = 0.f;
   if constexpr (F16) {
     int ea = pow2_shift(*g.amax), eb = pow2_shift(*g.bmax);
-    sa = ldexpf(1.f, ea), sb = ldexpf(1.f, eb), unscale = ldexpf(1.f, -(ea + eb));
+    sa = ldexpf(1.f, ea), sb = ldexpf(1.f, eb), unscale = ldexpf(1.f, -(ea + eb + pe));
   }
   const int64_t M = g.m, N = CPLX ? 2 * g.n : g.n, K = CPLX ? 2 * g.k : g.k;
   const int64_t lda = CPLX ? 2 * g.lda : g.lda, ldc = CPLX ? 2 * g.ldc : g.ldc;
@@ -1374,10 +1381,11 @@ __global__ __launch_bounds__(256, BM == 32 ? 3 : 2) void gemm_presplit_kernel(Ge
   H* const blo = bhi + TILE_B;
   uint32_t* const tab = reinterpret_cast<uint32_t*>(blo + TILE_B);
 
-  float unscale = 1.f, sb = 1.f;
+  const int pe = pending_exp(g);
+  float unscale = ldexpf(1.f, -pe), sb = 1.f;
   float mx = 0.f;
   if constexpr (F16) {
-    unscale = ldexpf(1.f, -(pow2_shift(*g.amax) + pow2_shift(*g.bmax)));
+    unscale = ldexpf(1.f, -(pow2_shift(*g.amax) + pow2_shift(*g.bmax) + pe));
     sb = ldexpf(1.f, pow2_shift(*g.bmax));
   }
   const int64_t M = g.m, N = CPLX ? 2 * g.n : g.n, kp = ps.kp;
@@ -1727,11 +1735,12 @@ __global__ __launch_bounds__(256, 2) void gemm_split3m_kernel(GemmArgs g, PreSpl
   H* const blo = bhi + 3 * PB;
   uint32_t* const tab = reinterpret_cast<uint32_t*>(blo + 3 * PB);
 
-  float unscale = 1.f, sb = 1.f;
+  const int pe = pending_exp(g);
+  float unscale = ldexpf(1.f, -pe), sb = 1.f;
   float mx = 0.f;
   if constexpr (F16) {
     const int ea = pow2_shift(*g.amax) - 1, eb = pow2_shift(*g.bmax) - 1;
-    unscale = ldexpf(1.f, -(ea + eb));
+    unscale = ldexpf(1.f, -(ea + eb + pe));
     sb = ldexpf(1.f, eb);
   }
   const int64_t M = g.m, N = 2 * g.n, kq = ps.kp, ldc = 2 * g.ldc;
@@ -2328,6 +2337,10 @@ static int launch_gemm_kernels(int dt, int prec, int kernel, const GemmArgs& g0,
                   (kernel == GEMM_PRESPLIT || kernel == GEMM_PRESPLIT_A) && g.scratch &&
                   g.scratch_bytes >= gemm_scratch_bytes(dt, prec, kernel, g)))
     kernel = GEMM_SIMPLE;
+  if ((g.apend || g.bpend) && g.k > 0 &&
+      !(split_precision(prec) && (dt == F32 || dt == C64) &&
+        (kernel == GEMM_MFMA || kernel == GEMM_PRESPLIT || kernel == GEMM_PRESPLIT_A)))
+    throw Error("pending operand exponents are divided out by the split-precision kernels only");
   if (kernel == GEMM_SPLITK) {
     run_splitk(dt, g, s);
     return GEMM_SPLITK;

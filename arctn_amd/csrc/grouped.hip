@@ -275,6 +275,10 @@ struct Single {  // an op launched on its own
   // maxima, or -1 (measured by a pass); GEMM without strip, STRIP: the slot that
   // receives this result's max, or -1
   int amax = -1, bmax = -1, max_out = -1;
+  // GEMM: pending-exponent slots of A / B (GemmArgs::apend / bpend) in the same
+  // table, or -1; STRIP: the slot receiving this result's exponent when its
+  // rescale is left to its reader, or -1 (the result is rescaled in place)
+  int apend = -1, bpend = -1, pend_out = -1;
   size_t A = 0, d = 0, Bm = 0, C = 0;  // SUM: [A, d, C]; DIAG/TRACE: [A, d, Bm, d, C]; COPY: A bytes
 };
 
@@ -400,6 +404,20 @@ PlanProg build_plan(const Plan& P, int dt, bool strip, int prec, size_t n_rounds
   // result-max slot of each plan slot produced by a large GEMM (stripped runs:
   // the power-of-two rescale writes the max after it), or -1
   std::vector<int> res_max(slot.size(), -1);
+  // The step reading each plan slot, and whether a step's product runs on the
+  // split-precision kernels (large, launched on its own): a large stripped result
+  // read by such a product is not rescaled in place; its exponent goes to a slot
+  // (res_pend) that the product divides out with its FP16 operand scales, exactly
+  // the values the rescale would have given (every op between, sums and permutes,
+  // is linear).
+  std::vector<int> reader(slot.size(), -1);
+  for (size_t si = 0; si < C.steps.size(); ++si) reader[C.steps[si].li] = reader[C.steps[si].ri] = (int)si;
+  auto split_large = [&](const PairStep& st) {
+    const uint64_t cn = (uint64_t)st.bt * st.m * st.n;
+    const bool lg = cn >= LARGE_NUMEL || cn * st.k >= LARGE_MACS || (int64_t)st.k >= LARGE_K;
+    return lg && split_products_apply(dt, prec, (int64_t)st.m, (int64_t)st.k, (int64_t)st.n);
+  };
+  std::vector<int> res_pend(slot.size(), -1);
   std::vector<std::vector<size_t>> shapes(slot.size());
   for (size_t i = 0; i < C.n_inputs; ++i) {  // round 0: traces / diagonals (src/compiled.rs:401-404)
     GRef cur{INPUT0 + (uint32_t)i, 0};
@@ -531,6 +549,7 @@ PlanProg build_plan(const Plan& P, int dt, bool strip, int prec, size_t n_rounds
       // operand maxima of earlier large results (a permute or a gather keeps the max)
       if (st.sum_a.empty()) sg.amax = res_max[st.li];
       if (st.sum_b.empty()) sg.bmax = res_max[st.ri];
+      sg.apend = res_pend[st.li], sg.bpend = res_pend[st.ri];
       if (fac.bcg) {
         g.bcg = fac.bcg;
         std::copy(fac.bcd, fac.bcd + 6, g.bcd);
@@ -559,6 +578,8 @@ PlanProg build_plan(const Plan& P, int dt, bool strip, int prec, size_t n_rounds
       sg.kind = Single::STRIP;
       sg.in = c, sg.A = cnum, sg.final = last;
       if (!last) sg.max_out = res_max[st.out] = prog.n_max++;
+      if (!last && reader[st.out] >= 0 && split_large(C.steps[(size_t)reader[st.out]]))
+        sg.pend_out = res_pend[st.out] = prog.n_max++;
       R.big_strips.push_back(sg);
     } else if (strip) {
       R.strips.push_back(GStripDesc{c, cnum, 0, 0, last ? 1u : 0u});
@@ -723,11 +744,14 @@ GroupedExecutor::GroupedExecutor(std::vector<std::shared_ptr<const Plan>> plans,
             if (sg.brow64 >= 0) sg.brow64 = (int64_t)(tab64_at[i] + sg.brow64 * 8);
             if (sg.amax >= 0) sg.amax += max_at[i];
             if (sg.bmax >= 0) sg.bmax += max_at[i];
+            if (sg.apend >= 0) sg.apend += max_at[i];
+            if (sg.bpend >= 0) sg.bpend += max_at[i];
             L.mid.push_back(sg);
           }
           for (Single sg : R.big_strips) {
             sg.in = fix(sg.in), sg.plan = (uint32_t)i;
             if (sg.max_out >= 0) sg.max_out += max_at[i];
+            if (sg.pend_out >= 0) sg.pend_out += max_at[i];
             L.big_strips.push_back(sg);
           }
         }
@@ -836,6 +860,8 @@ void GroupedExecutor::run(const std::vector<const void* const*>& inputs, const s
             if (sg.brow64 >= 0) g.brow64 = reinterpret_cast<const int64_t*>(dev + sg.brow64);
             if (sg.amax >= 0) g.amax = maxtab + sg.amax;
             if (sg.bmax >= 0) g.bmax = maxtab + sg.bmax;
+            if (sg.apend >= 0) g.apend = reinterpret_cast<const int64_t*>(maxtab + sg.apend);
+            if (sg.bpend >= 0) g.bpend = reinterpret_cast<const int64_t*>(maxtab + sg.bpend);
             launch_gemm(dt, prec_, GEMM_AUTO, g, s);
             break;
           }
@@ -892,7 +918,8 @@ void GroupedExecutor::run(const std::vector<const void* const*>& inputs, const s
                 launch_strip_exact(dt, at(sg.in), sg.A, (uint64_t*)(slots + sg.plan), log10_last + sg.plan, s);
               else
                 launch_scale_pow2(dt, at(sg.in), sg.A, (const uint64_t*)(slots + sg.plan), (int64_t*)(expo + sg.plan), s,
-                                  STRIP_KEEP, sg.max_out >= 0 ? (uint64_t*)(maxtab + sg.max_out) : nullptr);
+                                  STRIP_KEEP, sg.max_out >= 0 ? (uint64_t*)(maxtab + sg.max_out) : nullptr,
+                                  sg.pend_out >= 0 ? (int64_t*)(maxtab + sg.pend_out) : nullptr);
             }
             ATN_HIP(hipGetLastError());
           }

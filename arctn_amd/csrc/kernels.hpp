@@ -92,8 +92,11 @@ void launch_maxabs(int dt, const void* x, size_t numel, uint64_t* maxbits, hipSt
 // keep > 0: leave x (and expo) unchanged while the max's exponent e lies in
 // [-keep, keep] — the power-of-two scale is only needed against over/underflow.
 // scaled_max (optional): receives the max of x after the call as double bits.
+// pend (optional): x is left as it is; the exponent e of the rescale goes to
+// *pend (and is added to expo as usual) for x's reader to divide out
+// (GemmArgs::apend / bpend), and scaled_max receives the max of x unchanged.
 void launch_scale_pow2(int dt, void* x, size_t numel, const uint64_t* maxbits, int64_t* expo, hipStream_t s,
-                       int keep = 0, uint64_t* scaled_max = nullptr);
+                       int keep = 0, uint64_t* scaled_max = nullptr, int64_t* pend = nullptr);
 // The last step of a stripped run: divide by the exact max magnitude m (Scalar::abs,
 // f64) and add log10(m) to log10_exp (src/compiled.rs:412-430); maxbits is scratch.
 void launch_strip_exact(int dt, void* x, size_t numel, uint64_t* maxbits, double* log10_exp, hipStream_t s);
@@ -151,6 +154,13 @@ struct GemmArgs {
   // a pass before the GEMM. An upper bound is enough: it only sets the scale.
   const unsigned long long* amax = nullptr;
   const unsigned long long* bmax = nullptr;
+  // Optional pending power-of-two exponents (int64) of A / of B: the operand holds
+  // its values times 2^pend (a stripped result whose rescale was left to its
+  // reader), and the product is divided by 2^(apend + bpend) in the same exact
+  // output scale as the FP16 operand scales. Read by the split-precision kernels
+  // only (launch_gemm rejects them elsewhere).
+  const int64_t* apend = nullptr;
+  const int64_t* bpend = nullptr;
   // Optional gather tables of A, read by the simple kernel only (launch_gemm
   // runs that kernel when they are set): A element (i, p) of a batch entry sits
   // at arow[i] + acol[p] instead of i * lda + p.

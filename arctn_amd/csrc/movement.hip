@@ -716,16 +716,24 @@ __global__ __launch_bounds__(256) void maxabs_kernel(const S* __restrict__ x, ui
 
 // scaled_max (optional): receives the max after the rescale (max * 2^-e, or the
 // max itself when nothing is rescaled) as double bits, for a later product's
-// FP16 operand scale.
+// FP16 operand scale. pend: record e there for x's reader instead of scaling x
+// (scaled_max then receives the max itself).
 template <class T>
 __global__ void scale_pow2_kernel(T* __restrict__ x, uint64_t n, const unsigned long long* maxbits, long long* expo, int keep,
-                                  unsigned long long* scaled_max) {
+                                  unsigned long long* scaled_max, long long* pend) {
   const double m = __longlong_as_double((long long)*maxbits);
   const bool first = blockIdx.x == 0 && threadIdx.x == 0;
   int e = 0;
   if (m > 0.0 && isfinite(m)) {
     frexp(m, &e);
     if (keep > 0 && e >= -keep && e <= keep) e = 0;
+  }
+  if (pend) {
+    if (first) {
+      *pend = e, *expo += e;
+      if (scaled_max) *scaled_max = (unsigned long long)__double_as_longlong(m);
+    }
+    return;
   }
   if (first && scaled_max) *scaled_max = (unsigned long long)__double_as_longlong(e ? ldexp(m, -e) : m);
   if (e == 0) return;
@@ -786,14 +794,15 @@ void launch_maxabs(int dt, const void* x, size_t numel, uint64_t* maxbits, hipSt
   });
 }
 void launch_scale_pow2(int dt, void* x, size_t numel, const uint64_t* maxbits, int64_t* expo, hipStream_t s, int keep,
-                       uint64_t* scaled_max) {
+                       uint64_t* scaled_max, int64_t* pend) {
   dispatch_dt(dt, [&](auto tag) {
     using T = decltype(tag);
     g_last_kind = K_STRIP;
     // a grid-stride grid of at most 2048 workgroups: most calls find the max in range
-    // and return at once, so the launch costs little more than its first wave
-    LAUNCH(scale_pow2_kernel<T>, grid_for(numel, 256, 2048), 256, 0, s, (T*)x, (uint64_t)numel, (const unsigned long long*)maxbits,
-           (long long*)expo, keep, (unsigned long long*)scaled_max);
+    // and return at once, so the launch costs little more than its first wave (one
+    // workgroup when the exponent is only recorded)
+    LAUNCH(scale_pow2_kernel<T>, pend ? 1u : grid_for(numel, 256, 2048), 256, 0, s, (T*)x, (uint64_t)numel,
+           (const unsigned long long*)maxbits, (long long*)expo, keep, (unsigned long long*)scaled_max, (long long*)pend);
   });
 }
 
