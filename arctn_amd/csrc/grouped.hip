@@ -190,7 +190,11 @@ __global__ __launch_bounds__(256) void grouped_maxabs_kernel(const GStripDesc* _
     if (threadIdx.x == 0) {
       m = wmax[0];
       for (int w = 1; w < 4; ++w) m = wmax[w] > m ? wmax[w] : m;
-      if (m > 0.0) atomicMax(&slots[p.plan], (unsigned long long)__double_as_longlong(m));
+      // the slot only grows: skip the atomic unless this block's max is larger
+      // than what the slot holds (a stale read only costs an atomic)
+      const unsigned long long b = (unsigned long long)__double_as_longlong(m);
+      if (m > 0.0 && b > __hip_atomic_load(&slots[p.plan], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+        atomicMax(&slots[p.plan], b);
     }
     __syncthreads();
   }
