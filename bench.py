@@ -509,7 +509,9 @@ def bench_gradient(args):
     """20-qubit brickwork <psi|Z0 Z1|psi>, 10 layers of RY.RZ + CZ: a step is one
     complex64 forward plus the hand-written backward to all 400 angles
     (checkpoint every 8 steps), the gates handed over as rows of one gate table
-    (autograd.contract_table); gradients checked against the per-tensor form
+    (autograd.contract_table), the large forward products in each precision form
+    (native, FP16x2, BF16x3; the fastest whose gradient is within 1e-4 of central
+    differences is reported); gradients checked against the per-tensor form
     (autograd.contract), central differences of a complex128 forward and the CPU
     oracle's step-by-step gradient."""
     import numpy as np
@@ -529,11 +531,13 @@ def bench_gradient(args):
     ang = W.brickwork_angles(nq, layers, 0)
     th = torch.tensor(ang, dtype=torch.float32, device="cuda", requires_grad=True)
 
-    def step():
-        table, index, fixed = W.brickwork_table(slots, th)
-        val = contract_table(plan, table, index, fixed, checkpoint_every=8).real
-        (g,) = torch.autograd.grad(val, th)
-        return val, g
+    def step_in(prec):
+        def step():
+            table, index, fixed = W.brickwork_table(slots, th)
+            val = contract_table(plan, table, index, fixed, precision=prec, checkpoint_every=8).real
+            (g,) = torch.autograd.grad(val, th)
+            return val, g
+        return step
 
     def step_tensors():  # every gate its own tensor (autograd.contract)
         val = contract(plan, *W.brickwork_torch_tensors(slots, th), checkpoint_every=8).real
@@ -550,15 +554,24 @@ def bench_gradient(args):
         torch.cuda.synchronize()
         return (time.perf_counter() - t0) * 1e3 / args.steps, val, g
 
-    ms, val, g32 = timed(step)
-    g32 = g32.cpu().numpy()
-    log(f"[bench] forward + backward (400 gradients): {ms:.2f} ms/step")
-    if args.skip_ref:  # profiler runs: the timed steps only
+    # candidate forms: the large forward products in each precision (the adjoint
+    # products stay FP32); the fastest whose gradient meets the bar against central
+    # differences is the one reported, every candidate listed
+    precs = ["native", "fp16x2", "bf16x3"] if args.precision == "auto" else [args.precision]
+    if args.skip_ref:  # profiler runs: the timed steps of one form only
+        ms, _, _ = timed(step_in(args.precision if args.precision != "auto" else "native"))
+        log(f"[bench] forward + backward (400 gradients): {ms:.2f} ms/step")
         print(json.dumps({"metric": "forward + backward time for 400 angle gradients", "ms_per_step": ms,
                           "value": 400.0 / (ms * 1e-3), "unit": "gradients/s", "checks": "skipped"}), flush=True)
         return
+    cands = {}
+    for prec in precs:
+        ms_p, val_p, g_p = timed(step_in(prec))
+        cands[prec] = (ms_p, val_p, g_p.cpu().numpy())
+        log(f"[bench] forward + backward (400 gradients), {prec}: {ms_p:.2f} ms/step")
+    g_nat = cands["native"][2] if "native" in cands else cands[precs[0]][2]
     ms_t, _, g_t = timed(step_tensors)
-    diff_t = float(np.abs(g_t.cpu().numpy() - g32).max())
+    diff_t = float(np.abs(g_t.cpu().numpy() - g_nat).max())
     log(f"[bench] the same step with every gate its own tensor: {ms_t:.2f} ms/step; gradients differ by {diff_t:.1e}")
     th64 = torch.tensor(ang, dtype=torch.float64, device="cuda", requires_grad=True)
     (g64,) = torch.autograd.grad(contract(plan, *W.brickwork_torch_tensors(slots, th64)).real, th64)
@@ -578,9 +591,15 @@ def bench_gradient(args):
                     fd[l, q, j] = (vals[0] - vals[1]) / (2 * h)
     fd_s = time.perf_counter() - t0
     scale = max(1.0, float(np.abs(fd).max()))
-    err32 = float(np.abs(g32 - fd).max()) / scale
-    log(f"[bench] complex64 gradient vs central differences: max |diff| / max(1, |g|) = {err32:.2e} "
-        f"(800 complex128 forwards, {fd_s:.1f} s)")
+    errs = {p: float(np.abs(c[2] - fd).max()) / scale for p, c in cands.items()}
+    for p in precs:
+        log(f"[bench] complex64 gradient ({p}) vs central differences: max |diff| / max(1, |g|) = {errs[p]:.2e}")
+    log(f"[bench] (800 complex128 forwards, {fd_s:.1f} s)")
+    ok = [p for p in precs if errs[p] <= 1e-4]
+    best = min(ok, key=lambda p: cands[p][0]) if ok else precs[0]
+    ms, val, g32 = cands[best]
+    err32 = errs[best]
+    log(f"[bench] form: {best}, {ms:.2f} ms/step")
     thc = torch.tensor(ang, dtype=torch.float64, requires_grad=True)
     tsc = W.brickwork_torch_tensors(slots, thc)
     arrays = [t.detach().numpy() for t in tsc]
@@ -597,8 +616,10 @@ def bench_gradient(args):
         "warmup": args.warmup, "ms_per_step": round(ms, 3), "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "complex64", "data": "synthetic (seeded angles)",
         "config": {"workload": args.workload, "qubits": nq, "layers": layers, "angles": 400, "checkpoint_every": 8,
-                   "n_steps": info["n_steps"], "log10_flops": info["log10_flops"]},
+                   "n_steps": info["n_steps"], "log10_flops": info["log10_flops"], "precision": best},
         "grad_err_vs_fd": err32, "grad64_err_vs_cpu_oracle": err_cpu, "value_at_angles": float(val.detach()),
+        "within_bar": err32 <= 1e-4,
+        "variants": {p: {"ms_per_step": round(cands[p][0], 3), "grad_err_vs_fd": errs[p]} for p in precs},
         "inputs": "gate table rows (autograd.contract_table)",
         "per_tensor_form": {"ms_per_step": round(ms_t, 3), "grad_max_abs_diff": diff_t},
         "cpu_baseline": {"value": 400.0 / cpu_s, "unit": "gradients/s", "cores": 1, "kind": "port",

@@ -235,3 +235,27 @@ def test_contract_table_matches_per_tensor_contract(cuda, dtype):
     t2 = t.detach().clone().requires_grad_(True)
     (gr,) = torch.autograd.grad((torch.matmul(t2[0], t2[0]) * w).sum().real, t2)
     assert torch.allclose(gt, gr, rtol=1e-5 if cdt == torch.complex64 else 1e-12, atol=1e-6)
+
+
+@pytest.mark.parametrize("precision", ["fp16x2", "bf16x3"])
+def test_brickwork_20q_split_precision_gradient(cuda, precision):
+    """The 20-qubit brickwork's 400 angle gradients with the large forward products in
+    a split precision form (the benchmark picks the fastest form within 1e-4 of central
+    differences): within 1e-5 of the native form's gradient relative to max |g|, and not
+    bit-identical to it (the split products did run)."""
+    import torch
+    from arctn_amd.autograd import contract_table
+
+    nq, layers = 20, 10
+    net, slots = W.brickwork_structure(nq, layers)
+    plan = Plan(net["inputs"], net["output"], net["size_dict"], P.greedy(net))
+    th = torch.tensor(W.brickwork_angles(nq, layers, 0), dtype=torch.float32, device="cuda", requires_grad=True)
+    grads = {}
+    for prec in ("native", precision):
+        table, index, fixed = W.brickwork_table(slots, th)
+        val = contract_table(plan, table, index, fixed, precision=prec, checkpoint_every=8).real
+        (grads[prec],) = torch.autograd.grad(val, th)
+    g0, g1 = grads["native"], grads[precision]
+    scale = float(g0.abs().max())
+    assert float((g1 - g0).abs().max()) <= 1e-5 * scale
+    assert not torch.equal(g0, g1)
